@@ -1,0 +1,121 @@
+/*
+ * rhllc_mesh.h -- host-side mesh and file-format tools for the RHLLC step.
+ *
+ * Plain C ABI (extern "C", pointers and sizes only).  Everything here runs on
+ * the CPU; it needs no GPU and is built into libradrhd_mesh.so.
+ *
+ * Record formats are the reference's on-disk geometry records:
+ *   geo_face_t  40 bytes  lib/grids/include/solvers_struct.h:87-102
+ *     u32  id_l:25 | id_l_node:6 | is_regular:1
+ *     i32  id_r:26 (signed; < 0 is a boundary code) | id_r_node:6
+ *     f64  n[3]   outward normal of cell id_l      (offset 8)
+ *     f64  S      face area                        (offset 32)
+ *   geo_cell_t  56 bytes  lib/grids/include/solvers_struct.h:120-154
+ *     i32  id_faces[4]                             (offset 0)
+ *     f64  V                                       (offset 16)
+ *     f64  center[3]                               (offset 24)
+ *     u8   sign_n  bit j = 1 <=> cell is id_l of face j   (offset 48)
+ *     u8   node:6 | is_bound:2                     (offset 49)
+ *   flux_t      40 bytes  lib/grids/include/solvers_struct.h:51-70
+ *     f64  d, v[3], p   (primitive W = rho, v, p  or conserved U = D, m, E)
+ *
+ * Face numbering and orientation follow ConvertBinToGeo
+ * (lib/geometry/src/convert_bin_to_geo.cpp:3-48): half-faces (cell i, slot j)
+ * are scanned in increasing i*4+j; the first visit of a face creates it with
+ * id_l = i, the normal of (i, j) and sign bit j = 1; the partner slot gets
+ * sign bit 0.  Boundary half-faces get id_r = boundary code and bump is_bound.
+ */
+#ifndef RHLLC_MESH_H
+#define RHLLC_MESH_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RHLLC_GEO_FACE_BYTES 40
+#define RHLLC_GEO_CELL_BYTES 56
+#define RHLLC_FLUX_BYTES 40
+
+/* Boundary codes, lib/global_types.h:25-31 (e_boundary_types_t). */
+#define RHLLC_BOUND_FREE (-1)
+#define RHLLC_BOUND_LOCK (-2)
+#define RHLLC_BOUND_OUT_SOURCE (-3)
+#define RHLLC_BOUND_INNER_SOURCE (-4)
+#define RHLLC_BOUND_OUTER_SURFACE (-5)
+
+/* Periodicity mask bits for rhllc_box_desc.periodic. */
+#define RHLLC_PERIODIC_X 1
+#define RHLLC_PERIODIC_Y 2
+#define RHLLC_PERIODIC_Z 4
+
+/* A box of nx*ny*nz hexahedra, each cut into 6 Kuhn tetrahedra. */
+typedef struct rhllc_box_desc {
+  int32_t nx, ny, nz;
+  double origin[3];  /* lower corner */
+  double length[3];  /* edge lengths of the box */
+  int32_t periodic;  /* RHLLC_PERIODIC_* mask; non-periodic sides get boundary code -1 */
+  double jitter;     /* vertex jitter, fraction of the hex edge (0.05 = 5%) */
+  uint64_t seed;     /* jitter uses seed, the cell permutation seed + 1 */
+  int32_t permute;   /* non-zero: randomly permute the cell numbering */
+} rhllc_box_desc;
+
+/* Number of cells and faces the box will have. */
+int rhllc_box_sizes(const rhllc_box_desc *d, int64_t *n_cells, int64_t *n_faces);
+
+/*
+ * Build the box into caller-owned arrays:
+ *   geo_cells : n_cells * 56 bytes (geo_cell_t records)
+ *   geo_faces : n_faces * 40 bytes (geo_face_t records)
+ * Cells are numbered after the optional permutation; faces follow the
+ * ConvertBinToGeo rule above.  Returns 0 on success.
+ */
+int rhllc_box_build(const rhllc_box_desc *d, void *geo_cells, void *geo_faces);
+
+/*
+ * h_min as computed by utils/src/get_average_volume.cpp:31-46:
+ * min over cells of (3/4) * V / (sum of the 4 face areas), the areas summed in
+ * slot order.  Needs the face records to look up the areas.
+ */
+double rhllc_mesh_h_min(int64_t n_cells, const void *geo_cells, const void *geo_faces);
+
+/* Initial states of the synthetic workloads (flux_t W records, n_cells*40 B). */
+#define RHLLC_INIT_SHOCK_TUBE 0  /* x<0.5: rho=10,p=13.33; else rho=1,p=1e-6; v=0 */
+#define RHLLC_INIT_BLAST 1       /* rho=1; p=1000 for |x-c|<0.1, else 1; v=0     */
+#define RHLLC_INIT_SHEAR 2       /* vx=0.995 tanh(y/0.01), rho 1/0.1, p=1, seeded vy */
+#define RHLLC_INIT_UNIFORM 3     /* params[0..4] = rho, vx, vy, vz, p everywhere    */
+int rhllc_init_state(int32_t kind, int64_t n_cells, const void *geo_cells,
+                     const double *params, uint64_t seed, void *W_out);
+
+/*
+ * Seeded radiation fields for the full-form workload: per cell (FP32)
+ * Ur log-uniform in [lo, hi], Fr = Ur * f * w (f ~ U[0,1], w isotropic),
+ * kappa_a, kappa_s ~ U[kmin, kmax].
+ */
+int rhllc_init_radiation(int64_t n_cells, uint64_t seed, double ur_lo, double ur_hi,
+                         double kmin, double kmax, float *Ur, float *Fr3, float *ka,
+                         float *ks);
+
+/*
+ * Relabel-only locality order: a permutation new->old of the cells sorted by
+ * the Morton code of their centres, and new->old of the faces sorted by the
+ * new index of their id_l cell (ties keep the old face order).  The mesh
+ * itself (orientation, slots, id_l/id_r roles) is not changed.
+ */
+int rhllc_morton_order(int64_t n_cells, const void *geo_cells, int64_t n_faces,
+                       const void *geo_faces, int32_t *cell_new_to_old,
+                       int32_t *face_new_to_old);
+
+/* Read/write "int32 n, then n records" files (reader_bin.h:113-135,232-252). */
+int64_t rhllc_file_count(const char *path);
+int rhllc_file_read_records(const char *path, size_t rec_bytes, int64_t n, void *out);
+int rhllc_file_write_records(const char *path, size_t rec_bytes, int64_t n, const void *recs);
+
+const char *rhllc_mesh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RHLLC_MESH_H */

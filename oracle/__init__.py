@@ -1,0 +1,294 @@
+"""Test oracle for the RHLLC step.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of bench.py may import this package, and only as the
+checker.  It wraps:
+
+* ``liboracle_rhllc.so`` -- the C restatement in ``rhllc_oracle.c`` of the
+  reference's ``rhllc::Hllc3dStab`` and its callees (FP64, AoS records);
+* ``_ref/libref_rhllc.so`` -- the reference's own sources compiled by
+  ``make ref`` (present only where the reference tree was available at build
+  time).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle_rhllc.so")
+REF_LIB = os.path.join(HERE, "_ref", "libref_rhllc.so")
+
+FLUX = np.dtype((np.float64, 5))
+ELEM_BYTES, FACE_BYTES = 136, 80
+
+
+class Physics(C.Structure):
+    _fields_ = [
+        ("gamma", C.c_double),
+        ("has_out_source", C.c_int),
+        ("out_source_W", C.c_double * 5),
+        ("out_source_U", C.c_double * 5),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("c2p_fail", C.c_int64 * 5),
+        ("fix_fail", C.c_int64),
+        ("lor_die", C.c_int64),
+        ("newton_iters", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {
+            "c2p_fail": list(self.c2p_fail)[1:],
+            "fix_fail": self.fix_fail,
+            "lor_die": self.lor_die,
+            "newton_iters": self.newton_iters,
+        }
+
+
+_orc = None
+_ref = None
+
+
+def build(ref: bool = True) -> None:
+    """Compile the oracle (and the reference build when the tree is present)."""
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle_rhllc.so"], check=True)
+    if ref and os.path.isdir("/root/reference/solvers/rhllc"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def lib():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_LIB):
+            raise RuntimeError("oracle/liboracle_rhllc.so is not built (make -C oracle)")
+        L = C.CDLL(ORACLE_LIB)
+        vp, i64, dbl, i32 = C.c_void_p, C.c_int64, C.c_double, C.c_int
+        L.orc_p2c_n.argtypes = [vp, i64, vp, vp]
+        L.orc_c2p_n.argtypes = [vp, i64, vp, vp, vp, vp]
+        L.orc_pressure_fix_n.argtypes = [vp, i64, vp, vp, vp]
+        L.orc_rotation.argtypes = [vp, vp]
+        L.orc_flux_n.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_load.argtypes = [i64, vp, vp, i64, vp, vp, vp]
+        L.orc_step.argtypes = [vp, i64, vp, i64, vp, dbl, vp]
+        L.orc_step.restype = dbl
+        L.orc_run.argtypes = [vp, i64, vp, i64, vp, dbl, dbl, i32, vp, vp, vp]
+        L.orc_run.restype = dbl
+        L.orc_edd_tensor.argtypes = [i32, i32, dbl, vp]
+        L.orc_edd_tensor.restype = dbl
+        L.orc_rad_source.argtypes = [vp, dbl, vp, dbl, dbl, i32, vp]
+        L.orc_run_rad.argtypes = [vp, i64, vp, i64, vp, dbl, dbl, i32, vp, vp, vp, vp, i32, vp, vp, vp]
+        L.orc_run_rad.restype = dbl
+        L.orc_num_threads.restype = i32
+        _orc = L
+    return _orc
+
+
+def have_ref() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        if not have_ref():
+            raise RuntimeError("oracle/_ref/libref_rhllc.so is not built (make -C oracle ref)")
+        L = C.CDLL(REF_LIB)
+        vp, i64, dbl, i32 = C.c_void_p, C.c_int64, C.c_double, C.c_int
+        L.ref_layout.argtypes = [vp]
+        L.ref_gamma.restype = dbl
+        L.ref_set_log_dir.argtypes = [C.c_char_p]
+        L.ref_p2c_n.argtypes = [i64, vp, vp]
+        L.ref_c2p_n.argtypes = [i64, vp, vp, vp]
+        L.ref_pressure_fix_n.argtypes = [i64, vp, vp, vp]
+        L.ref_rotation_n.argtypes = [i64, vp, vp]
+        L.ref_flux_n.argtypes = [i64, vp, vp, vp, vp, vp, vp, vp]
+        L.ref_run.argtypes = [i64, vp, vp, i64, vp, dbl, dbl, i32, vp, vp, vp, vp]
+        L.ref_set_log_dir(("/tmp/").encode())
+        _ref = L
+    return _ref
+
+
+def physics(gamma: float = 5.0 / 3.0) -> Physics:
+    ph = Physics()
+    ph.gamma = gamma
+    return ph
+
+
+# ---------------------------------------------------------------- oracle calls
+def p2c(W, gamma=5.0 / 3.0):
+    """GetConvValueStab over n states; returns (W_clamped, U)."""
+    W = np.array(W, dtype=np.float64, order="C").reshape(-1, 5)
+    U = np.zeros_like(W)
+    ph = physics(gamma)
+    lib().orc_p2c_n(C.byref(ph), W.shape[0], W.ctypes.data, U.ctypes.data)
+    return W, U
+
+
+def c2p(U, gamma=5.0 / 3.0, W_init=None):
+    """GetPhysValueStab over n states; returns (W, codes, iters).  W keeps W_init where c2p fails."""
+    U = np.ascontiguousarray(U, dtype=np.float64).reshape(-1, 5)
+    W = np.zeros_like(U) if W_init is None else np.array(W_init, dtype=np.float64, order="C").reshape(-1, 5)
+    codes = np.zeros(U.shape[0], np.int32)
+    iters = np.zeros(U.shape[0], np.int32)
+    ph = physics(gamma)
+    lib().orc_c2p_n(C.byref(ph), U.shape[0], U.ctypes.data, W.ctypes.data, codes.ctypes.data, iters.ctypes.data)
+    return W, codes, iters
+
+
+def pressure_fix(U, W, gamma=5.0 / 3.0):
+    U = np.array(U, dtype=np.float64, order="C").reshape(-1, 5)
+    W = np.array(W, dtype=np.float64, order="C").reshape(-1, 5)
+    codes = np.zeros(U.shape[0], np.int32)
+    ph = physics(gamma)
+    lib().orc_pressure_fix_n(C.byref(ph), U.shape[0], U.ctypes.data, W.ctypes.data, codes.ctypes.data)
+    return U, W, codes
+
+
+def rotation(n):
+    n = np.ascontiguousarray(n, dtype=np.float64).reshape(3)
+    T = np.zeros(9)
+    lib().orc_rotation(n.ctypes.data, T.ctypes.data)
+    return T.reshape(3, 3)
+
+
+def flux(UL, UR, WL, WR, nS4, gamma=5.0 / 3.0):
+    arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 5) for a in (UL, UR, WL, WR)]
+    nS4 = np.ascontiguousarray(nS4, dtype=np.float64).reshape(-1, 4)
+    n = nS4.shape[0]
+    F = np.zeros((n, 5))
+    speed = np.zeros(n)
+    ph = physics(gamma)
+    lib().orc_flux_n(C.byref(ph), n, *[a.ctypes.data for a in arrs], nS4.ctypes.data, F.ctypes.data, speed.ctypes.data)
+    return F, speed
+
+
+class State:
+    """AoS elem_t / face_t records of the oracle for one mesh."""
+
+    def __init__(self, mesh, W0, gamma=5.0 / 3.0):
+        self.n_cells, self.n_faces = mesh.n_cells, mesh.n_faces
+        self.cells = np.zeros(self.n_cells * ELEM_BYTES, np.uint8)
+        self.faces = np.zeros(self.n_faces * FACE_BYTES, np.uint8)
+        W0 = np.ascontiguousarray(W0, dtype=np.float64)
+        lib().orc_load(
+            self.n_cells, mesh.cells.ctypes.data, W0.ctypes.data, self.n_faces, mesh.faces.ctypes.data,
+            self.cells.ctypes.data, self.faces.ctypes.data,
+        )
+        self.ph = physics(gamma)
+        self.stats = Stats()
+
+    def _view(self, off):
+        v = self.cells.view(np.float64).reshape(self.n_cells, ELEM_BYTES // 8)
+        return v[:, off // 8: off // 8 + 5]
+
+    @property
+    def W(self):
+        return np.array(self._view(0))
+
+    @property
+    def U(self):
+        return np.array(self._view(40))
+
+    @property
+    def face_flux(self):
+        return np.array(self.faces.view(np.float64).reshape(self.n_faces, FACE_BYTES // 8)[:, :5])
+
+    def run(self, cfl, h_min, steps):
+        taus = np.zeros(steps)
+        speeds = np.zeros(steps)
+        t = lib().orc_run(
+            C.byref(self.ph), self.n_cells, self.cells.ctypes.data, self.n_faces, self.faces.ctypes.data,
+            cfl, h_min, steps, taus.ctypes.data, speeds.ctypes.data, C.byref(self.stats),
+        )
+        return t, taus, speeds
+
+    def run_rad(self, cfl, h_min, steps, Ur, Fr, ka, ks, emission=0):
+        taus = np.zeros(steps)
+        speeds = np.zeros(steps)
+        Ur, Fr, ka, ks = (np.ascontiguousarray(a, dtype=np.float32) for a in (Ur, Fr, ka, ks))
+        t = lib().orc_run_rad(
+            C.byref(self.ph), self.n_cells, self.cells.ctypes.data, self.n_faces, self.faces.ctypes.data,
+            cfl, h_min, steps, Ur.ctypes.data, Fr.ctypes.data, ka.ctypes.data, ks.ctypes.data, emission,
+            taus.ctypes.data, speeds.ctypes.data, C.byref(self.stats),
+        )
+        return t, taus, speeds
+
+    def step(self, tau):
+        return lib().orc_step(
+            C.byref(self.ph), self.n_cells, self.cells.ctypes.data, self.n_faces, self.faces.ctypes.data, tau,
+            C.byref(self.stats),
+        )
+
+
+def rad_source(W, Ur, Fr, ka, ks, emission=0):
+    W = np.ascontiguousarray(W, dtype=np.float64).reshape(5)
+    Fr = np.ascontiguousarray(Fr, dtype=np.float64).reshape(3)
+    G = np.zeros(4)
+    lib().orc_rad_source(W.ctypes.data, float(Ur), Fr.ctypes.data, float(ka), float(ks), emission, G.ctypes.data)
+    return G
+
+
+# ------------------------------------------------------------- reference calls
+def ref_run(mesh, W0, cfl, h_min, steps):
+    L = ref_lib()
+    W0 = np.ascontiguousarray(W0, dtype=np.float64)
+    Wo = np.zeros((mesh.n_cells, 5))
+    Uo = np.zeros((mesh.n_cells, 5))
+    taus = np.zeros(steps)
+    speeds = np.zeros(steps)
+    L.ref_run(
+        mesh.n_cells, mesh.cells.ctypes.data, W0.ctypes.data, mesh.n_faces, mesh.faces.ctypes.data, cfl, h_min,
+        steps, Wo.ctypes.data, Uo.ctypes.data, taus.ctypes.data, speeds.ctypes.data,
+    )
+    return Wo, Uo, taus, speeds
+
+
+def ref_p2c(W):
+    W = np.array(W, dtype=np.float64, order="C").reshape(-1, 5)
+    U = np.zeros_like(W)
+    ref_lib().ref_p2c_n(W.shape[0], W.ctypes.data, U.ctypes.data)
+    return W, U
+
+
+def ref_c2p(U, W_init=None):
+    U = np.ascontiguousarray(U, dtype=np.float64).reshape(-1, 5)
+    W = np.zeros_like(U) if W_init is None else np.array(W_init, dtype=np.float64, order="C").reshape(-1, 5)
+    codes = np.zeros(U.shape[0], np.int32)
+    ref_lib().ref_c2p_n(U.shape[0], U.ctypes.data, W.ctypes.data, codes.ctypes.data)
+    return W, codes
+
+
+def ref_pressure_fix(U, W):
+    U = np.array(U, dtype=np.float64, order="C").reshape(-1, 5)
+    W = np.array(W, dtype=np.float64, order="C").reshape(-1, 5)
+    codes = np.zeros(U.shape[0], np.int32)
+    ref_lib().ref_pressure_fix_n(U.shape[0], U.ctypes.data, W.ctypes.data, codes.ctypes.data)
+    return U, W, codes
+
+
+def ref_rotation(normals):
+    normals = np.ascontiguousarray(normals, dtype=np.float64).reshape(-1, 3)
+    T = np.zeros((normals.shape[0], 9))
+    ref_lib().ref_rotation_n(normals.shape[0], normals.ctypes.data, T.ctypes.data)
+    return T.reshape(-1, 3, 3)
+
+
+def ref_flux(UL, UR, WL, WR, nS4):
+    arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 5) for a in (UL, UR, WL, WR)]
+    nS4 = np.ascontiguousarray(nS4, dtype=np.float64).reshape(-1, 4)
+    n = nS4.shape[0]
+    F = np.zeros((n, 5))
+    speed = np.zeros(n)
+    ref_lib().ref_flux_n(n, *[a.ctypes.data for a in arrs], nS4.ctypes.data, F.ctypes.data, speed.ctypes.data)
+    return F, speed

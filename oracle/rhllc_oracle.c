@@ -1,0 +1,599 @@
+/*
+ * rhllc_oracle.c -- CPU restatement of the reference's RHLLC step (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker that tests/, the smoke check and the
+ * CPU-baseline leg of bench.py use to judge the HIP path.  The product never
+ * links, loads or calls this file.
+ *
+ * Build: gcc -O2 -fno-fast-math -ffp-contract=off -fopenmp (oracle/Makefile).
+ * Contraction must stay off: every expression below keeps the reference's
+ * operation order so results are bitwise comparable.
+ *
+ * Each function names the reference lines it follows.  Parity pins:
+ *   orc_p2c            GetConvValueStab   -- pinned: compiled reference (oracle/_ref,
+ *                                            tests/golden vectors) + round trip c2p(p2c(W)) = W
+ *   orc_c2p            GetPhysValueStab   -- pinned: compiled reference + long-double bracketed root
+ *   orc_pressure_fix   PhysPressureFix    -- pinned: compiled reference vectors
+ *   orc_rotation       GetRotationMatrix  -- pinned: compiled reference + orthonormality/det
+ *   orc_flux           GetFluxStab        -- pinned: compiled reference vectors (all 4 branches)
+ *                                            + consistency F(U,U) and mirror symmetry
+ *   orc_step / orc_run Hllc3dStab + loop  -- pinned: compiled reference trajectories (200 steps)
+ *                                            + free-stream preservation, periodic conservation
+ *   orc_edd_tensor / orc_rad_source       -- pinned: compiled reference (RAD_RHD build) vectors
+ *                                            when available; closed forms (isotropic limit)
+ *
+ * Summation order of Eigen's 3-vector expressions (the reference uses
+ * Eigen 3.4 fixed-size types, vectorised with 2-double packets on x86-64):
+ * u.dot(v) sums as (a0+a1)+a2; the coefficients of T*v sum as (a0+a1)+a2 for
+ * rows 0 and 1 (one packet) and a0+(a1+a2) for row 2 (scalar tail); those of
+ * T.transpose()*v as (a0+a1)+a2.  These orders reproduce the compiled
+ * reference bit for bit (tests/test_oracle_vs_ref.py).
+ */
+#include "rhllc_oracle.h"
+
+#include <math.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAX_ITER 40 /* rhllc_flux_stab.cpp:8 */
+static const double kMinPressure = 1e-12; /* lib/global_consts.h:15 */
+
+/* Summation order of 3-term sums: 0 = (a0+a1)+a2, 1 = a0+(a1+a2); the MV
+ * masks hold one bit per result row. */
+#ifndef ORC_ORDER_DOT
+#define ORC_ORDER_DOT 0
+#endif
+#ifndef ORC_ORDER_MV
+#define ORC_ORDER_MV 4 /* rows 0,1: (a0+a1)+a2; row 2: a0+(a1+a2) */
+#endif
+#ifndef ORC_ORDER_MVT
+#define ORC_ORDER_MVT 0
+#endif
+
+static inline double sum3(int order, double a0, double a1, double a2) {
+  return order ? a0 + (a1 + a2) : (a0 + a1) + a2;
+}
+static inline double dot3(const double *a, const double *b) {
+  return sum3(ORC_ORDER_DOT, a[0] * b[0], a[1] * b[1], a[2] * b[2]);
+}
+/* r = T * v with T row-major */
+static inline void mat_vec(const double T[9], const double v[3], double r[3]) {
+  double t[3];
+  for (int i = 0; i < 3; ++i) t[i] = sum3((ORC_ORDER_MV >> i) & 1, T[3 * i] * v[0], T[3 * i + 1] * v[1], T[3 * i + 2] * v[2]);
+  r[0] = t[0], r[1] = t[1], r[2] = t[2];
+}
+/* r = T^T * v */
+static inline void mat_t_vec(const double T[9], const double v[3], double r[3]) {
+  double t[3];
+  for (int i = 0; i < 3; ++i) t[i] = sum3((ORC_ORDER_MVT >> i) & 1, T[i] * v[0], T[3 + i] * v[1], T[6 + i] * v[2]);
+  r[0] = t[0], r[1] = t[1], r[2] = t[2];
+}
+/* std::max / std::min of the C++ library, NaN behaviour included */
+static inline double smax(double a, double b) { return (a < b) ? b : a; }
+static inline double smin(double a, double b) { return (b < a) ? b : a; }
+#define SIGN(a) ((a) < 0.0 ? -1.0 : 1.0) /* lib/global_def.h:62 */
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* rhllc_flux_stab.cpp:10-38 (EOS IDEAL) */
+void orc_p2c(const orc_physics_t *ph, orc_flux_t *W, orc_flux_t *U) {
+  const double Gg = ph->gamma / (ph->gamma - 1.0);
+  const double h = 1 + Gg * W->p / W->d;
+  double g = dot3(W->v, W->v);
+  if (g >= 1.0) {
+    const double beta_fix = 0.9999;
+    g = beta_fix / sqrt(g);
+    W->v[0] *= g, W->v[1] *= g, W->v[2] *= g;
+    g = beta_fix * beta_fix;
+  }
+  g = 1.0 / (1.0 - g);
+  const double scrh = W->d * h * g;
+  g = sqrt(g);
+  U->d = W->d * g;
+  U->v[0] = scrh * W->v[0], U->v[1] = scrh * W->v[1], U->v[2] = scrh * W->v[2];
+  U->p = scrh - W->p;
+}
+
+/* rhllc_flux_stab.cpp:40-121 (EOS IDEAL): secant on u = Gamma*|v| at p = kMinPressure;
+ * rewrites U.E on success (:115). */
+int orc_pressure_fix(const orc_physics_t *ph, orc_flux_t *U, orc_flux_t *W) {
+  const double p = kMinPressure;
+  const double D = U->d;
+  const double m2 = dot3(U->v, U->v);
+  const double m = sqrt(m2);
+  const double umax = m / D;
+  double u0 = umax;
+  double lor = sqrt(1.0 + u0 * u0);
+  double plor = p * lor;
+  const double alpha = ph->gamma / (ph->gamma - 1.0);
+  double Dh = D + plor * alpha;
+  double f0 = m / Dh - u0;
+  double u1 = (-D + sqrt(D * D + 4.0 * m * alpha * p)) / (2.0 * alpha * p);
+  int done = 0;
+  double f1, du;
+  for (int k = 1; k < MAX_ITER; k++) {
+    lor = sqrt(1.0 + u1 * u1);
+    plor = p * lor;
+    Dh = D + plor * alpha;
+    f1 = m / Dh - u1;
+    if (done == 1) break;
+    du = (u1 - u0) / (f1 - f0) * f1;
+    u0 = u1;
+    f0 = f1;
+    u1 -= du;
+    u1 = smin(u1, umax);
+    u1 = smax(u1, 0.0);
+    if (fabs(f1) < 1.e-9) done = 1;
+  }
+  if (done != 1) return 2;
+  lor = sqrt(1.0 + u1 * u1);
+  plor = p * lor;
+  Dh = D + plor * alpha;
+  W->d = U->d / lor;
+  W->p = p;
+  U->p = Dh * lor - p;
+  f0 = 1.0 / (Dh * lor);
+  W->v[0] = U->v[0] * f0, W->v[1] = U->v[1] * f0, W->v[2] = U->v[2] * f0;
+  return 0;
+}
+
+/* rhllc_flux_stab.cpp:123-220 (EOS IDEAL): Newton on f(p) = D h Gamma - E - p.
+ * Return codes as there; 5 marks the DIE_IF(lor2 < 1.0) abort (:161). */
+int orc_c2p(const orc_physics_t *ph, const orc_flux_t *U, orc_flux_t *W, int *iters) {
+  const double tol = 1.e-11;
+  const double eps2 = 1.e-12;
+  const double Gg = ph->gamma / (ph->gamma - 1.0);
+  const double D = U->d;
+  const double E = U->p;
+  const double m2 = dot3(U->v, U->v);
+  const double Q = E - sqrt(m2 + D * D);
+  if (iters) *iters = 0;
+  if (Q < 0.0) return 1;
+  const double m = sqrt(m2);
+  double p = smax(m - E, 0.0);
+  const double D_1 = 1.0 / D;
+  const double pmin = sqrt(m2 / (1.0 - eps2)) - E;
+  double alpha, alpha2, lor2, lor, tau, theta, h, dh_dp, dh_dtau, yp = 0, dp, dyp;
+  p = smax(p, pmin);
+  int iter;
+  for (iter = 0; iter < MAX_ITER; iter++) {
+    alpha = E + p;
+    alpha2 = alpha * alpha;
+    lor2 = 1.0 - m2 / alpha2;
+    lor2 = 1.0 / lor2;
+    if (lor2 < 1.0) {
+      if (iters) *iters = iter + 1;
+      return 5;
+    }
+    lor = sqrt(lor2);
+    tau = lor * D_1;
+    theta = p * tau;
+    h = 1.0 + Gg * theta;
+    dh_dp = Gg * tau;
+    dh_dtau = Gg * p;
+    yp = D * h * lor - E - p;
+    dyp = D * lor * dh_dp - m2 * lor2 * lor / (alpha2 * alpha) * (lor * dh_dtau + D * h) - 1.0;
+    dp = yp / dyp;
+    p -= dp;
+    if (p < pmin) p = pmin;
+    if (fabs(dp) < tol * E) break;
+  }
+  if (iters) *iters = iter < MAX_ITER ? iter + 1 : MAX_ITER;
+  if (p < 0.0) return 2;
+  if (isnan(p)) return 4;
+  if (iter >= MAX_ITER || fabs(yp / (E + p)) > 1.e-4 || p < (m - E)) return 3;
+  alpha = E + p;
+  alpha2 = alpha * alpha;
+  lor2 = alpha2 / (alpha2 - m2);
+  lor = sqrt(lor2);
+  tau = lor * D_1;
+  W->d = 1.0 / tau;
+  W->p = p;
+  const double den = U->p + p;
+  W->v[0] = U->v[0] / den, W->v[1] = U->v[1] / den, W->v[2] = U->v[2] / den;
+  return 0;
+}
+
+/* lib/geometry/src/linear_alg.cpp:3-42; T row-major, rows (n, t1, t2). */
+void orc_rotation(const double n[3], double T[9]) {
+  const double eps = 1e-12;
+  for (int i = 0; i < 9; ++i) T[i] = 0.0;
+  if (fabs(n[2] * n[2] - 1) > eps) {
+    T[0] = n[0];
+    T[1] = n[1];
+    T[2] = n[2];
+    const double sqr = sqrt(1 - n[2] * n[2]);
+    const double sqr_inv = 1. / sqr;
+    T[3] = -n[1] * sqr_inv;
+    T[4] = n[0] * sqr_inv;
+    T[6] = -n[0] * n[2] * sqr_inv;
+    T[7] = -n[1] * n[2] * sqr_inv;
+    T[8] = sqr;
+  } else if (n[2] > 0) {
+    T[2] = 1;
+    T[4] = 1;
+    T[6] = -1;
+  } else {
+    T[2] = -1;
+    T[4] = -1;
+    T[6] = 1;
+  }
+}
+
+/* rhllc_flux_stab.cpp:231-238 */
+static inline void signal_speed(double cs2, double Vn, double V2, double *cmin, double *cmax) {
+  const double Vn2 = Vn * Vn;
+  const double Vt2 = V2 - Vn2;
+  const double sroot = sqrt(cs2 * (1 - Vn2 - Vt2 * cs2) * (1 - V2));
+  *cmax = (Vn * (1.0 - cs2) + sroot) / (1.0 - V2 * cs2);
+  *cmin = (Vn * (1.0 - cs2) - sroot) / (1.0 - V2 * cs2);
+}
+
+/* rhllc_flux_stab.cpp:240-431 (EOS IDEAL, HLLC branch of Mignone & Bodo 2006).
+ * F receives S * (flux rotated back to the global frame); returns max(lL, lR). */
+double orc_flux(const orc_physics_t *ph, const orc_flux_t *cUL, const orc_flux_t *cUR, const orc_flux_t *cWL,
+                const orc_flux_t *cWR, const double n[3], double S, orc_flux_t *Fout) {
+  const double G1 = ph->gamma;
+  const double Gg = G1 / (G1 - 1.0);
+  double T[9];
+  orc_rotation(n, T);
+  orc_flux_t UL = *cUL, UR = *cUR, WL = *cWL, WR = *cWR;
+  mat_vec(T, cUL->v, UL.v);
+  mat_vec(T, cUR->v, UR.v);
+  mat_vec(T, cWL->v, WL.v);
+  mat_vec(T, cWR->v, WR.v);
+  const double *VL = WL.v, *VR = WR.v;
+  const double dL = WL.d, dR = WR.d, pL = WL.p, pR = WR.p;
+  const double VVL = dot3(VL, VL), VVR = dot3(VR, VR);
+  const double hL = 1 + Gg * pL / dL;
+  const double hR = 1 + Gg * pR / dR;
+  const double csL2 = ((G1 * pL) / (dL * hL));
+  const double csR2 = ((G1 * pR) / (dR * hR));
+  double cmaxL, cmaxR, cminL, cminR;
+  signal_speed(csL2, VL[0], VVL, &cminL, &cmaxL);
+  signal_speed(csR2, VR[0], VVR, &cminR, &cmaxR);
+  const double lL = smin(cminL, cminR);
+  const double lR = smax(cmaxL, cmaxR);
+  orc_flux_t F;
+  if (lL >= 0.0) {
+    F.d = UL.d * VL[0];
+    for (int i = 0; i < 3; ++i) F.v[i] = UL.v[i] * VL[0];
+    F.p = UL.v[0];
+    F.v[0] += pL;
+  } else if (lR <= 0.0) {
+    F.d = UR.d * VR[0];
+    for (int i = 0; i < 3; ++i) F.v[i] = UR.v[i] * VR[0];
+    F.p = UR.v[0];
+    F.v[0] += pR;
+  } else {
+    orc_flux_t FL, FR;
+    FL.d = UL.d * VL[0];
+    for (int i = 0; i < 3; ++i) FL.v[i] = UL.v[i] * VL[0];
+    FL.p = UL.v[0];
+    FR.d = UR.d * VR[0];
+    for (int i = 0; i < 3; ++i) FR.v[i] = UR.v[i] * VR[0];
+    FR.p = UR.v[0];
+    const double AL = lL * UL.p - FL.p;
+    const double AR = lR * UR.p - FR.p;
+    const double BL = lL * UL.v[0] - FL.v[0] - pL;
+    const double BR = lR * UR.v[0] - FR.v[0] - pR;
+    const double a = AR * lL - AL * lR;
+    const double b = AL + BL * lR - AR - BR * lL;
+    const double c = BR - BL;
+    const double scrh = -0.5 * (b + SIGN(b) * sqrt(b * b - 4.0 * a * c));
+    const double us = c / scrh;
+    const double ps = (AL * us - BL) / (1.0 - us * lL);
+    if (us >= 0.0) {
+      orc_flux_t usl;
+      const double dif = 1.0 / (lL - us);
+      usl.d = UL.d * (lL - VL[0]) * dif;
+      usl.v[0] = (lL * (UL.p + ps) - UL.v[0]) * us * dif;
+      usl.v[1] = UL.v[1] * (lL - VL[0]) * dif;
+      usl.v[2] = UL.v[2] * (lL - VL[0]) * dif;
+      usl.p = UL.p + (usl.v[0] - UL.v[0]) / lL;
+      usl.d -= UL.d, usl.v[0] -= UL.v[0], usl.v[1] -= UL.v[1], usl.v[2] -= UL.v[2], usl.p -= UL.p;
+      usl.d *= lL, usl.v[0] *= lL, usl.v[1] *= lL, usl.v[2] *= lL, usl.p *= lL;
+      F.d = FL.d + usl.d;
+      for (int i = 0; i < 3; ++i) F.v[i] = FL.v[i] + usl.v[i];
+      F.p = FL.p + usl.p;
+      F.v[0] += pL;
+    } else {
+      orc_flux_t usr;
+      const double dif = 1.0 / (lR - us);
+      usr.d = UR.d * (lR - VR[0]) * dif;
+      usr.v[0] = (lR * (UR.p + ps) - UR.v[0]) * us * dif;
+      usr.v[1] = UR.v[1] * (lR - VR[0]) * dif;
+      usr.v[2] = UR.v[2] * (lR - VR[0]) * dif;
+      usr.p = UR.p + (usr.v[0] - UR.v[0]) / lR;
+      usr.d -= UR.d, usr.v[0] -= UR.v[0], usr.v[1] -= UR.v[1], usr.v[2] -= UR.v[2], usr.p -= UR.p;
+      usr.d *= lR, usr.v[0] *= lR, usr.v[1] *= lR, usr.v[2] *= lR, usr.p *= lR;
+      F.d = FR.d + usr.d;
+      for (int i = 0; i < 3; ++i) F.v[i] = FR.v[i] + usr.v[i];
+      F.p = FR.p + usr.p;
+      F.v[0] += pR;
+    }
+  }
+  orc_flux_t out = F;
+  mat_t_vec(T, F.v, out.v);
+  out.d *= S, out.v[0] *= S, out.v[1] *= S, out.v[2] *= S, out.p *= S;
+  *Fout = out;
+  return smax(lL, lR);
+}
+
+static inline int32_t face_id_r(const orc_geo_face_t *g) { return ((int32_t)(g->w1 << 6)) >> 6; }
+static inline int32_t face_id_l(const orc_geo_face_t *g) { return (int32_t)(g->w0 & 0x1FFFFFFu); }
+
+/* rhllc_utils.cpp:45-133 with GEOMETRY_TYPE Cone (the reference default,
+ * include/solvers_config.h:29): every boundary code except e_bound_out_source
+ * (-3) copies the cell; -3 takes a fixed ghost state.  Unknown codes (< -5)
+ * copy the cell here (the reference aborts). */
+void orc_bound(const orc_physics_t *ph, const orc_face_t *f, const orc_elem_t *cells, orc_flux_t *Wg,
+               orc_flux_t *Ug) {
+  const int32_t id_l = face_id_l(&f->geo);
+  const int32_t id_r = face_id_r(&f->geo);
+  const orc_elem_t *cell = &cells[id_l];
+  if (id_r >= 0) {
+    *Ug = cells[id_r].conv_val;
+    *Wg = cells[id_r].phys_val;
+    return;
+  }
+  *Ug = cell->conv_val;
+  *Wg = cell->phys_val;
+  if (id_r == -3 && ph->has_out_source) {
+    *Ug = ph->out_source_U;
+    *Wg = ph->out_source_W;
+  }
+}
+
+/* rhllc_3d_stab.cpp:30-53: flux sum in slot order, scale by tau/V, c2p with
+ * the PhysPressureFix fallback.  Counters go to the thread-local *st. */
+static inline void cell_update(const orc_physics_t *ph, orc_elem_t *el, const orc_face_t *faces, double tau,
+                               orc_stats_t *st) {
+  orc_flux_t s = {0, {0, 0, 0}, 0};
+  for (int j = 0; j < 4; ++j) {
+    const orc_flux_t *f = &faces[el->geo.id_faces[j]].f;
+    if ((el->geo.sign_n >> j) & 1) {
+      s.d += f->d, s.v[0] += f->v[0], s.v[1] += f->v[1], s.v[2] += f->v[2], s.p += f->p;
+    } else {
+      s.d -= f->d, s.v[0] -= f->v[0], s.v[1] -= f->v[1], s.v[2] -= f->v[2], s.p -= f->p;
+    }
+  }
+  const double k = tau / el->geo.V;
+  s.d *= k, s.v[0] *= k, s.v[1] *= k, s.v[2] *= k, s.p *= k;
+  orc_flux_t *U = &el->conv_val;
+  U->d -= s.d, U->v[0] -= s.v[0], U->v[1] -= s.v[1], U->v[2] -= s.v[2], U->p -= s.p;
+  int it = 0;
+  const int code = orc_c2p(ph, U, &el->phys_val, &it);
+  st->newton_iters += it;
+  if (code) {
+    if (code == 5)
+      st->lor_die += 1;
+    else
+      st->c2p_fail[code] += 1;
+    if (orc_pressure_fix(ph, U, &el->phys_val)) st->fix_fail += 1;
+  }
+}
+
+static void stats_add(orc_stats_t *dst, const orc_stats_t *src) {
+  for (int k = 0; k < 5; ++k) dst->c2p_fail[k] += src->c2p_fail[k];
+  dst->fix_fail += src->fix_fail;
+  dst->lor_die += src->lor_die;
+  dst->newton_iters += src->newton_iters;
+}
+
+/* rhllc_3d_stab.cpp:10-65 */
+double orc_step(const orc_physics_t *ph, int64_t n_cells, orc_elem_t *cells, int64_t n_faces, orc_face_t *faces,
+                double tau, orc_stats_t *st) {
+  double max_signal_speed = 0;
+#pragma omp parallel
+  {
+    double max_speed = 0;
+    orc_stats_t loc;
+    memset(&loc, 0, sizeof(loc));
+#pragma omp for schedule(static)
+    for (int64_t i = 0; i < n_faces; i++) {
+      orc_face_t *f = &faces[i];
+      orc_flux_t Wg, Ug;
+      orc_bound(ph, f, cells, &Wg, &Ug);
+      const orc_elem_t *cl = &cells[face_id_l(&f->geo)];
+      const double s = orc_flux(ph, &cl->conv_val, &Ug, &cl->phys_val, &Wg, f->geo.n, f->geo.S, &f->f);
+      max_speed = smax(max_speed, s);
+    }
+#pragma omp for schedule(static)
+    for (int64_t i = 0; i < n_cells; i++) cell_update(ph, &cells[i], faces, tau, &loc);
+#pragma omp critical
+    {
+      max_signal_speed = smax(max_signal_speed, max_speed);
+      if (st) stats_add(st, &loc);
+    }
+  }
+  return max_signal_speed;
+}
+
+/* rhllc_utils.cpp:25-33 */
+static void phys_to_conv(const orc_physics_t *ph, int64_t n, orc_elem_t *cells) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) orc_p2c(ph, &cells[i].phys_val, &cells[i].conv_val);
+}
+
+/* rhllc_main.cpp:28-56 with rhllc_init.cpp:190-192 and rhllc_utils.cpp:15-23 */
+double orc_run(const orc_physics_t *ph, int64_t n_cells, orc_elem_t *cells, int64_t n_faces, orc_face_t *faces,
+               double cfl, double h_min, int steps, double *taus, double *speeds, orc_stats_t *st) {
+  phys_to_conv(ph, n_cells, cells);
+  double speed = 1;
+  double tau = cfl * h_min / speed;
+  double t = 0;
+  for (int k = 0; k < steps; ++k) {
+    speed = orc_step(ph, n_cells, cells, n_faces, faces, tau, st);
+    if (taus) taus[k] = tau;
+    if (speeds) speeds[k] = speed;
+    t += tau;
+    tau = cfl * h_min / speed;
+  }
+  return t;
+}
+
+void orc_load(int64_t n_cells, const void *geo_cells, const double *W, int64_t n_faces, const void *geo_faces,
+              orc_elem_t *cells, orc_face_t *faces) {
+  const uint8_t *gc = (const uint8_t *)geo_cells;
+  const uint8_t *gf = (const uint8_t *)geo_faces;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n_cells; ++i) {
+    memset(&cells[i], 0, sizeof(orc_elem_t));
+    memcpy(&cells[i].geo, gc + 56 * i, 56);
+    cells[i].phys_val.d = W[5 * i + 0];
+    cells[i].phys_val.v[0] = W[5 * i + 1];
+    cells[i].phys_val.v[1] = W[5 * i + 2];
+    cells[i].phys_val.v[2] = W[5 * i + 3];
+    cells[i].phys_val.p = W[5 * i + 4];
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n_faces; ++i) {
+    memset(&faces[i], 0, sizeof(orc_face_t));
+    memcpy(&faces[i].geo, gf + 40 * i, 40);
+  }
+}
+
+void orc_p2c_n(const orc_physics_t *ph, int64_t n, orc_flux_t *W, orc_flux_t *U) {
+  for (int64_t i = 0; i < n; ++i) orc_p2c(ph, &W[i], &U[i]);
+}
+void orc_c2p_n(const orc_physics_t *ph, int64_t n, const orc_flux_t *U, orc_flux_t *W, int32_t *codes,
+               int32_t *iters) {
+  for (int64_t i = 0; i < n; ++i) {
+    int it = 0;
+    codes[i] = orc_c2p(ph, &U[i], &W[i], &it);
+    if (iters) iters[i] = it;
+  }
+}
+void orc_pressure_fix_n(const orc_physics_t *ph, int64_t n, orc_flux_t *U, orc_flux_t *W, int32_t *codes) {
+  for (int64_t i = 0; i < n; ++i) codes[i] = orc_pressure_fix(ph, &U[i], &W[i]);
+}
+void orc_flux_n(const orc_physics_t *ph, int64_t n, const orc_flux_t *UL, const orc_flux_t *UR,
+                const orc_flux_t *WL, const orc_flux_t *WR, const double *nS4, orc_flux_t *F, double *speed) {
+  for (int64_t i = 0; i < n; ++i) speed[i] = orc_flux(ph, &UL[i], &UR[i], &WL[i], &WR[i], nS4 + 4 * i, nS4[4 * i + 3], &F[i]);
+}
+
+/* ---------------- radiation source (config: full form) ---------------- */
+
+/* radRHD_utils.cpp:8-42 */
+double orc_edd_tensor(int i, int j, double U, const double F[3]) {
+  double ni, nj, f2, chi_2, edt;
+  double flux_module = dot3(F, F);
+  if (flux_module < 1e-150 || U < 1e-150) {
+    if (i == j) return 0.33333333333333;
+    return 0.0;
+  }
+  flux_module = sqrt(flux_module);
+  ni = F[i] / flux_module;
+  nj = F[j] / flux_module;
+  if (flux_module > U) return ni * nj;
+  f2 = flux_module / U;
+  f2 = f2 * f2;
+  chi_2 = (1.5 + 2.0 * f2) / (5.0 + 2.0 * sqrt(4.0 - 3.0 * f2));
+  edt = (3.0 * chi_2 - 0.5) * ni * nj;
+  if (i == j) edt += 0.5 - chi_2;
+  return edt;
+}
+
+/* lib/global_consts.h:12-83, lib/physics/include/gas_state.h:22-27, lib/physics/src/plunk.cpp:87-90 */
+static const double kPI = 3.1415926535897932384626433832795;
+static const double kC_Light = 3 * 1e10;
+static const double kM_hydrogen = 1.6735575 * 1e-24;
+static const double k_boltzmann = 1.3807 * 1e-16;
+static const double kStefanBoltzmann = 5.670374419 * 1e-5;
+static const double kGravity = 6.6726 * 1e-8;
+static const double kDist = 1e13;
+#define kVelocity kC_Light
+#define kMass ((kVelocity * kVelocity * kDist) / kGravity)
+#define kTime (kDist / kVelocity)
+#define kDensity (kMass / (kDist * kDist * kDist))
+#define kPressure (kMass / (kDist * kTime * kTime))
+#define kRadiation (kMass / (kTime * kTime * kTime))
+
+static inline double temperature(double rho, double prs) {
+  const double g = (kM_hydrogen / k_boltzmann) * kPressure / kDensity;
+  return g * prs / rho;
+}
+static inline double b_planck(double T) {
+  const double T2 = T * T;
+  return (T2 * T2) * kStefanBoltzmann / kPI;
+}
+
+/* radRHD_utils.cpp:44-101 with D = EddTensor(j, i, Ur, Fr) */
+void orc_rad_source(const orc_flux_t *W, double Ur, const double Fr[3], double abs_op, double scat_op, int emission,
+                    double G[4]) {
+  const double tot_op = abs_op + scat_op;
+  const double rho = W->d, prs = W->p;
+  const double *v = W->v;
+  double gamma = dot3(v, v);
+  double u2 = gamma;
+  const double gamma2 = 1.0 / (1.0 - gamma);
+  u2 *= gamma2;
+  gamma = sqrt(gamma2);
+  double u[3] = {gamma * v[0], gamma * v[1], gamma * v[2]};
+  double uuD = 0.;
+  const double uF = dot3(u, Fr);
+  double uD[3];
+  for (int i = 0; i < 3; i++) {
+    uD[i] = 0.;
+    for (int j = 0; j < 3; j++) {
+      const double D = orc_edd_tensor(j, i, Ur, Fr);
+      uD[i] += u[j] * D;
+      uuD += u[i] * u[j] * D;
+    }
+  }
+  const double rhogamma = rho * gamma;
+  const double q = emission ? -abs_op * rho * b_planck(temperature(rho, prs)) / kRadiation : 0.0;
+  G[0] = q * gamma + rhogamma * (abs_op - scat_op * (u2 + uuD)) * Ur - rho * (abs_op - scat_op * (u2 + gamma2)) * uF;
+  for (int i = 0; i < 3; i++)
+    G[i + 1] = q * u[i] - rho * (tot_op * uD[i] + scat_op * u[i] * (gamma2 + uuD)) * Ur +
+               rhogamma * (tot_op * Fr[i] + scat_op * 2.0 * uF * u[i]);
+}
+
+/* radRHD_main.cpp:99-132 (serial coupled loop, minus the transfer solve):
+ * Hllc3dStab, then U += (ds*tau)*G, then GetPhysValueStab (code ignored). */
+double orc_step_rad(const orc_physics_t *ph, int64_t n_cells, orc_elem_t *cells, int64_t n_faces, orc_face_t *faces,
+                    double tau, const float *Ur, const float *Fr3, const float *ka, const float *ks, int emission,
+                    orc_stats_t *st) {
+  const double speed = orc_step(ph, n_cells, cells, n_faces, faces, tau, st);
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < n_cells; ++c) {
+    double G[4];
+    const double Fr[3] = {Fr3[3 * c], Fr3[3 * c + 1], Fr3[3 * c + 2]};
+    orc_rad_source(&cells[c].phys_val, (double)Ur[c], Fr, (double)ka[c], (double)ks[c], emission, G);
+    const double ds = 1;
+    orc_flux_t *U = &cells[c].conv_val;
+    U->p += ds * tau * G[0];
+    U->v[0] += ds * tau * G[1];
+    U->v[1] += ds * tau * G[2];
+    U->v[2] += ds * tau * G[3];
+    int it = 0;
+    orc_c2p(ph, U, &cells[c].phys_val, &it);
+  }
+  return speed;
+}
+
+double orc_run_rad(const orc_physics_t *ph, int64_t n_cells, orc_elem_t *cells, int64_t n_faces, orc_face_t *faces,
+                   double cfl, double h_min, int steps, const float *Ur, const float *Fr3, const float *ka,
+                   const float *ks, int emission, double *taus, double *speeds, orc_stats_t *st) {
+  phys_to_conv(ph, n_cells, cells);
+  double speed = 1;
+  double tau = cfl * h_min / speed;
+  double t = 0;
+  for (int k = 0; k < steps; ++k) {
+    speed = orc_step_rad(ph, n_cells, cells, n_faces, faces, tau, Ur, Fr3, ka, ks, emission, st);
+    if (taus) taus[k] = tau;
+    if (speeds) speeds[k] = speed;
+    t += tau;
+    tau = cfl * h_min / speed;
+  }
+  return t;
+}
