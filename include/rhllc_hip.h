@@ -1,0 +1,153 @@
+/*
+ * rhllc_hip.h -- C ABI of the MI355X (gfx950) RHLLC step, librhllc_hip.so.
+ *
+ * The boundary replaces the reference's step entry points (no plugin system
+ * exists there; these are plain C++ free functions in namespace rhllc):
+ *   rhllc::Hllc3dStab(const Type tau, grid_t &grid)  solvers/rhllc/include/rhllc_calc.h:32
+ *     -> rhllc_step (one step at the caller's tau) / rhllc_run (device-side dt loop)
+ *   rhllc::max_signal_speed                          solvers/rhllc/include/rhllc_utils.h:17
+ *     -> the max_speed out-argument of rhllc_step / rhllc_stats.max_speed
+ *   rhllc::GetTimeStep(hllc_set, cells)             solvers/rhllc/include/rhllc_utils.h:35
+ *     -> computed on the device after every step (rhllc_stats.tau)
+ *   rhllc::Init / HllcPhysToConv                    solvers/rhllc/include/rhllc_init.h:32,
+ *                                                   solvers/rhllc/include/rhllc_utils.h:43
+ *     -> rhllc_upload_state with off_conv < 0
+ * The mesh enters as the reference's geo_cell_t / geo_face_t records
+ * (include/rhllc_mesh.h), the state as any array of records holding flux_t
+ * W and/or U at given byte offsets (elem_t: stride 136, W at 0, U at 40).
+ *
+ * Conventions: every call returns 0 on success or an RHLLC_ERR_* code, with a
+ * message in rhllc_last_error().  Nothing aborts across the ABI: where the
+ * reference calls abort() (DIE_IF after a failed PhysPressureFix,
+ * rhllc_3d_stab.cpp:49-52), the step counts the cell in rhllc_stats.fix_fail
+ * and carries on.  A handle owns its device memory and one HIP stream; all
+ * calls on a handle come from one host thread.
+ */
+#ifndef RHLLC_HIP_H
+#define RHLLC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RHLLC_OK 0
+#define RHLLC_ERR_ARG 1     /* invalid argument or mesh */
+#define RHLLC_ERR_HIP 2     /* HIP runtime failure (no device, out of memory, launch) */
+#define RHLLC_ERR_STATE 3   /* call out of order (e.g. step before upload) */
+#define RHLLC_ERR_NUMERIC 4 /* non-finite max signal speed */
+
+/* Arithmetic of the step. */
+#define RHLLC_PREC_FP64 0  /* all FP64, bitwise equal to the reference compiled strict */
+#define RHLLC_PREC_MIXED 1 /* FP32 face flux and state reads, FP64 U, c2p and pressure */
+
+/* Initial pressure of the c2p Newton iteration. */
+#define RHLLC_GUESS_REFERENCE 0 /* max(|m|-E, 0, pmin) as rhllc_flux_stab.cpp:139-151 */
+#define RHLLC_GUESS_PREVIOUS 1  /* the previous step's pressure */
+#define RHLLC_GUESS_MLP 2       /* the MLP surrogate; previous pressure, then the reference
+                                   start, then PhysPressureFix as fallbacks */
+
+typedef struct rhllc_params {
+  double gamma;       /* adiabatic index of the ideal-gas EOS (reference: 5/3 constexpr) */
+  double cfl;         /* hllc_cfg.json "CFL" */
+  double h_min;       /* hllc_cfg.json "h_min" */
+  int32_t precision;  /* RHLLC_PREC_* */
+  int32_t c2p_guess;  /* RHLLC_GUESS_*; must be REFERENCE for RHLLC_PREC_FP64 parity */
+  int32_t reorder;    /* 1: relabel cells/faces in Morton order on the device (results are
+                         returned in the caller's numbering either way) */
+  int32_t device;     /* HIP device ordinal */
+  int32_t collect_stats; /* 1: Newton histogram + iteration totals each step */
+  int32_t has_out_source; /* ghost state for boundary code -3 (e_bound_out_source) */
+  double out_source_W[5], out_source_U[5];
+} rhllc_params;
+
+#define RHLLC_HIST_BINS 41 /* Newton iterations 0..40 */
+
+typedef struct rhllc_stats {
+  int64_t steps;
+  int64_t c2p_fail[6];     /* cells whose first c2p returned code 1..5 (5: lor2 < 1) */
+  int64_t fix_calls;       /* PhysPressureFix invocations */
+  int64_t fix_fail;        /* PhysPressureFix failures (the reference aborts) */
+  int64_t fallback_cells;  /* GUESS_MLP/PREVIOUS: cells that needed a fallback start */
+  int64_t newton_iters;    /* sum of Newton iterations (collect_stats) */
+  int64_t newton_hist[RHLLC_HIST_BINS]; /* histogram of per-cell iterations (collect_stats) */
+  double max_speed;        /* max signal speed of the last step (rhllc::max_signal_speed) */
+  double tau;              /* tau of the next step: CFL*h_min/max_speed */
+  double t;                /* simulated time accumulated by the device loop */
+  double max_residual;     /* max |f(p)|/(E+p) over cells of the last step (collect_stats) */
+} rhllc_stats;
+
+typedef struct rhllc_solver rhllc_solver;
+
+/* Upload the mesh (reference records) and allocate the state. */
+int rhllc_create(const rhllc_params *params, int64_t n_cells, const void *geo_cells, int64_t n_faces,
+                 const void *geo_faces, rhllc_solver **out);
+int rhllc_destroy(rhllc_solver *h);
+
+/* State transfer from/to host records.  recs points at n_cells records of
+ * `stride` bytes; W (flux_t: rho, v, p) sits at off_phys, U (flux_t: D, m, E)
+ * at off_conv.  Upload with off_conv < 0 computes U = p2c(W) on the device
+ * (HllcPhysToConv, which clamps |v| >= 1 and so may rewrite W) and resets the
+ * time loop as rhllc::Init does: max_speed = 1, tau = CFL*h_min, t = 0.
+ * Download with off_phys or off_conv < 0 skips that part. */
+int rhllc_upload_state(rhllc_solver *h, const void *recs, int64_t stride, int64_t off_phys, int64_t off_conv);
+int rhllc_download_state(rhllc_solver *h, void *recs, int64_t stride, int64_t off_phys, int64_t off_conv);
+
+/* One step at the given tau (rhllc::Hllc3dStab); *max_speed receives the new
+ * rhllc::max_signal_speed.  Blocks until the step is done. */
+int rhllc_step(rhllc_solver *h, double tau, double *max_speed);
+
+/* nsteps steps with tau computed on the device between steps
+ * (tau_{k+1} = CFL*h_min/max_speed_k, rhllc_utils.cpp:15-23), starting from
+ * the device's current tau.  use_graph = 1 replays a captured hipGraph of
+ * the step.  Asynchronous on the handle's stream; rhllc_sync waits. */
+int rhllc_run(rhllc_solver *h, int32_t nsteps, int32_t use_graph);
+int rhllc_sync(rhllc_solver *h);
+
+/* Set the device's next tau and simulated time (e.g. from hllc_cfg.json). */
+int rhllc_set_time(rhllc_solver *h, double tau, double t);
+int rhllc_get_stats(rhllc_solver *h, rhllc_stats *out);
+int rhllc_reset_stats(rhllc_solver *h);
+
+/* Face fluxes of the last step in the caller's face numbering, n_faces x 5
+ * doubles (S * F in the global frame, face_t::f). */
+int rhllc_download_face_flux(rhllc_solver *h, double *F);
+
+/* Radiation fields for the explicit 4-momentum source, applied after the
+ * hydro update as radRHD_main.cpp:117-132 does (M1 Eddington tensor, thermal
+ * emission off).  Per cell in the caller's numbering, FP32.  NULL Ur removes
+ * the source. */
+int rhllc_set_radiation(rhllc_solver *h, const float *Ur, const float *Fr3, const float *kappa_a,
+                        const float *kappa_s);
+
+/* Average device time (ms) of each kernel of the step, measured with HIP
+ * events around each launch on the handle's stream over nsteps eager steps:
+ * out[0] face kernel, out[1] cell kernel, out[2] dt kernel, out[3] whole step. */
+int rhllc_time_kernels(rhllc_solver *h, int32_t nsteps, double out[4]);
+
+/* Sizes the handle moved per step by the algorithmic byte model (both
+ * kernels), for roofline reporting. */
+int rhllc_info(rhllc_solver *h, int64_t *n_cells, int64_t *n_faces, int64_t *device_bytes);
+
+/* Pointwise device evaluations over host arrays (n states, flux_t layout),
+ * for per-function parity checks.  precision selects the FP64 or FP32 path
+ * of the flux; nS4 holds (nx, ny, nz, S) per face. */
+int rhllc_eval_flux(int32_t precision, double gamma, int64_t n, const double *UL, const double *UR,
+                    const double *WL, const double *WR, const double *nS4, double *F, double *speed);
+int rhllc_eval_c2p(double gamma, int64_t n, const double *U, const double *guess, double *W, int32_t *codes,
+                   int32_t *iters);
+int rhllc_eval_p2c(double gamma, int64_t n, double *W, double *U);
+int rhllc_eval_pressure_fix(double gamma, int64_t n, double *U, double *W, int32_t *codes);
+int rhllc_eval_rad_source(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
+                          const double *ks, double *G);
+
+/* Device name and compute units of the current device (for reports). */
+int rhllc_device_query(int32_t device, char *name, int32_t name_len, int32_t *cus, double *hbm_gb);
+
+const char *rhllc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RHLLC_HIP_H */

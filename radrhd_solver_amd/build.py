@@ -1,0 +1,65 @@
+"""In-tree build of the native libraries (no JIT cache, no pip install).
+
+* ``libradrhd_mesh.so``  -- host mesh/format tools (g++, OpenMP).
+* ``librhllc_hip.so``    -- gfx950 HIP kernels + C ABI (hipcc).
+
+HIP code is compiled with ``-ffp-contract=off`` (the FP64 path is bitwise
+comparable with the reference) and with packed-FP32 instructions disabled.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+from ._lib import HIP_LIB, MESH_LIB, PKG_DIR
+
+CSRC = os.path.join(PKG_DIR, "csrc")
+MESH_SRC = [os.path.join(CSRC, "mesh", f) for f in ("kuhn_box.cpp", "mesh_tools.cpp", "morton.cpp")]
+HIP_SRC = [os.path.join(CSRC, "hip", f) for f in ("rhllc_kernels.hip", "rhllc_capi.hip")]
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_mesh() -> str:
+    _run(["g++", "-std=c++17", "-O2", "-fopenmp", "-fPIC", "-shared", "-Wall", "-o", MESH_LIB] + MESH_SRC)
+    return MESH_LIB
+
+
+def build_hip() -> str:
+    objdir = os.path.join(PKG_DIR, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src in HIP_SRC:
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        _run([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
+        objs.append(obj)
+    morton_obj = os.path.join(objdir, "morton.o")
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-c", os.path.join(CSRC, "mesh", "morton.cpp"), "-o", morton_obj])
+    objs.append(morton_obj)
+    tmp = HIP_LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, HIP_LIB)
+    return HIP_LIB
+
+
+def build_all() -> None:
+    build_mesh()
+    build_hip()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

@@ -1,0 +1,267 @@
+"""ctypes declarations of ``include/rhllc_hip.h`` and a thin Python handle.
+
+This mirrors what a maintainer of the reference would bind; the product path
+is the native library -- there is no Python or CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+PREC_FP64, PREC_MIXED = 0, 1
+GUESS_REFERENCE, GUESS_PREVIOUS, GUESS_MLP = 0, 1, 2
+HIST_BINS = 41
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("gamma", C.c_double),
+        ("cfl", C.c_double),
+        ("h_min", C.c_double),
+        ("precision", C.c_int32),
+        ("c2p_guess", C.c_int32),
+        ("reorder", C.c_int32),
+        ("device", C.c_int32),
+        ("collect_stats", C.c_int32),
+        ("has_out_source", C.c_int32),
+        ("out_source_W", C.c_double * 5),
+        ("out_source_U", C.c_double * 5),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64),
+        ("c2p_fail", C.c_int64 * 6),
+        ("fix_calls", C.c_int64),
+        ("fix_fail", C.c_int64),
+        ("fallback_cells", C.c_int64),
+        ("newton_iters", C.c_int64),
+        ("newton_hist", C.c_int64 * HIST_BINS),
+        ("max_speed", C.c_double),
+        ("tau", C.c_double),
+        ("t", C.c_double),
+        ("max_residual", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {
+            "steps": self.steps,
+            "c2p_fail": list(self.c2p_fail)[1:],
+            "fix_calls": self.fix_calls,
+            "fix_fail": self.fix_fail,
+            "fallback_cells": self.fallback_cells,
+            "newton_iters": self.newton_iters,
+            "newton_hist": list(self.newton_hist),
+            "max_speed": self.max_speed,
+            "tau": self.tau,
+            "t": self.t,
+            "max_residual": self.max_residual,
+        }
+
+
+# Every symbol the header declares: name -> (restype, argtypes)
+_vp, _i64, _i32, _dbl = C.c_void_p, C.c_int64, C.c_int32, C.c_double
+SIGNATURES = {
+    "rhllc_create": (C.c_int, [_vp, _i64, _vp, _i64, _vp, _vp]),
+    "rhllc_destroy": (C.c_int, [_vp]),
+    "rhllc_upload_state": (C.c_int, [_vp, _vp, _i64, _i64, _i64]),
+    "rhllc_download_state": (C.c_int, [_vp, _vp, _i64, _i64, _i64]),
+    "rhllc_step": (C.c_int, [_vp, _dbl, _vp]),
+    "rhllc_run": (C.c_int, [_vp, _i32, _i32]),
+    "rhllc_sync": (C.c_int, [_vp]),
+    "rhllc_set_time": (C.c_int, [_vp, _dbl, _dbl]),
+    "rhllc_get_stats": (C.c_int, [_vp, _vp]),
+    "rhllc_reset_stats": (C.c_int, [_vp]),
+    "rhllc_download_face_flux": (C.c_int, [_vp, _vp]),
+    "rhllc_set_radiation": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "rhllc_time_kernels": (C.c_int, [_vp, _i32, _vp]),
+    "rhllc_info": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "rhllc_eval_flux": (C.c_int, [_i32, _dbl, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "rhllc_eval_c2p": (C.c_int, [_dbl, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "rhllc_eval_p2c": (C.c_int, [_dbl, _i64, _vp, _vp]),
+    "rhllc_eval_pressure_fix": (C.c_int, [_dbl, _i64, _vp, _vp, _vp]),
+    "rhllc_eval_rad_source": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "rhllc_device_query": (C.c_int, [_i32, C.c_char_p, _i32, _vp, _vp]),
+    "rhllc_last_error": (C.c_char_p, []),
+}
+
+
+def declare(lib: C.CDLL) -> C.CDLL:
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+class RhllcError(RuntimeError):
+    pass
+
+
+def _lib():
+    from ._lib import load_hip_lib
+
+    return load_hip_lib()
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise RhllcError(f"rhllc error {rc}: {_lib().rhllc_last_error().decode()}")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Solver:
+    """Owns one device handle (``rhllc_solver``)."""
+
+    def __init__(self, mesh, gamma=5.0 / 3.0, cfl=0.4, h_min=None, precision=PREC_FP64,
+                 guess=GUESS_REFERENCE, reorder=True, device=0, collect_stats=False, out_source=None):
+        L = _lib()
+        p = Params()
+        p.gamma, p.cfl = gamma, cfl
+        p.h_min = mesh.h_min() if h_min is None else h_min
+        p.precision, p.c2p_guess, p.reorder, p.device = precision, guess, int(reorder), device
+        p.collect_stats = int(collect_stats)
+        if out_source is not None:
+            p.has_out_source = 1
+            for k in range(5):
+                p.out_source_W[k] = out_source[0][k]
+                p.out_source_U[k] = out_source[1][k]
+        self.params = p
+        self.n_cells, self.n_faces = mesh.n_cells, mesh.n_faces
+        h = C.c_void_p()
+        check(L.rhllc_create(C.byref(p), mesh.n_cells, mesh.cells.ctypes.data, mesh.n_faces,
+                             mesh.faces.ctypes.data, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _lib().rhllc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_W(self, W):
+        """Primitive state W[N,5]; U = p2c(W) on the device (rhllc::Init)."""
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(self.n_cells, 5)
+        check(_lib().rhllc_upload_state(self.h, W.ctypes.data, 40, 0, -1))
+
+    def upload(self, W, U):
+        rec = np.ascontiguousarray(np.concatenate([np.asarray(W, np.float64), np.asarray(U, np.float64)], axis=1))
+        check(_lib().rhllc_upload_state(self.h, rec.ctypes.data, 80, 0, 40))
+
+    def download(self):
+        rec = np.zeros((self.n_cells, 10))
+        check(_lib().rhllc_download_state(self.h, rec.ctypes.data, 80, 0, 40))
+        return rec[:, :5].copy(), rec[:, 5:].copy()
+
+    def step(self, tau: float) -> float:
+        s = C.c_double()
+        check(_lib().rhllc_step(self.h, tau, C.byref(s)))
+        return s.value
+
+    def run(self, nsteps: int, graph: bool = True):
+        check(_lib().rhllc_run(self.h, nsteps, int(graph)))
+
+    def sync(self):
+        check(_lib().rhllc_sync(self.h))
+
+    def set_time(self, tau: float, t: float = 0.0):
+        check(_lib().rhllc_set_time(self.h, tau, t))
+
+    def stats(self) -> dict:
+        s = Stats()
+        check(_lib().rhllc_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        check(_lib().rhllc_reset_stats(self.h))
+
+    def face_flux(self):
+        F = np.zeros((self.n_faces, 5))
+        check(_lib().rhllc_download_face_flux(self.h, F.ctypes.data))
+        return F
+
+    def set_radiation(self, Ur, Fr, ka, ks):
+        if Ur is None:
+            check(_lib().rhllc_set_radiation(self.h, None, None, None, None))
+            return
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (Ur, Fr, ka, ks)]
+        self._rad = arrs
+        check(_lib().rhllc_set_radiation(self.h, *[a.ctypes.data for a in arrs]))
+
+    def time_kernels(self, nsteps: int):
+        out = np.zeros(4)
+        check(_lib().rhllc_time_kernels(self.h, nsteps, out.ctypes.data))
+        return {"face_ms": out[0], "cell_ms": out[1], "dt_ms": out[2], "step_ms": out[3]}
+
+    def algorithmic_bytes(self) -> int:
+        b = C.c_int64()
+        check(_lib().rhllc_info(self.h, None, None, C.byref(b)))
+        return b.value
+
+
+# ----------------------------------------------------------- pointwise evals
+def eval_flux(UL, UR, WL, WR, nS4, gamma=5.0 / 3.0, precision=PREC_FP64):
+    a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 5) for x in (UL, UR, WL, WR)]
+    nS4 = np.ascontiguousarray(nS4, dtype=np.float64).reshape(-1, 4)
+    n = nS4.shape[0]
+    F = np.zeros((n, 5))
+    sp = np.zeros(n)
+    check(_lib().rhllc_eval_flux(precision, gamma, n, *[x.ctypes.data for x in a], nS4.ctypes.data,
+                                 F.ctypes.data, sp.ctypes.data))
+    return F, sp
+
+
+def eval_c2p(U, gamma=5.0 / 3.0, guess=None, W_init=None):
+    U = np.ascontiguousarray(U, dtype=np.float64).reshape(-1, 5)
+    n = U.shape[0]
+    W = np.zeros((n, 5)) if W_init is None else np.array(W_init, dtype=np.float64, order="C").reshape(n, 5)
+    g = None if guess is None else np.ascontiguousarray(guess, dtype=np.float64)
+    codes = np.zeros(n, np.int32)
+    iters = np.zeros(n, np.int32)
+    check(_lib().rhllc_eval_c2p(gamma, n, U.ctypes.data, _ptr(g), W.ctypes.data, codes.ctypes.data,
+                                iters.ctypes.data))
+    return W, codes, iters
+
+
+def eval_p2c(W, gamma=5.0 / 3.0):
+    W = np.array(W, dtype=np.float64, order="C").reshape(-1, 5)
+    U = np.zeros_like(W)
+    check(_lib().rhllc_eval_p2c(gamma, W.shape[0], W.ctypes.data, U.ctypes.data))
+    return W, U
+
+
+def eval_pressure_fix(U, W, gamma=5.0 / 3.0):
+    U = np.array(U, dtype=np.float64, order="C").reshape(-1, 5)
+    W = np.array(W, dtype=np.float64, order="C").reshape(-1, 5)
+    codes = np.zeros(U.shape[0], np.int32)
+    check(_lib().rhllc_eval_pressure_fix(gamma, U.shape[0], U.ctypes.data, W.ctypes.data, codes.ctypes.data))
+    return U, W, codes
+
+
+def eval_rad_source(W, Ur, Fr, ka, ks):
+    W = np.ascontiguousarray(W, dtype=np.float64).reshape(-1, 5)
+    n = W.shape[0]
+    Ur, ka, ks = (np.ascontiguousarray(x, dtype=np.float64).reshape(n) for x in (Ur, ka, ks))
+    Fr = np.ascontiguousarray(Fr, dtype=np.float64).reshape(n, 3)
+    G = np.zeros((n, 4))
+    check(_lib().rhllc_eval_rad_source(n, W.ctypes.data, Ur.ctypes.data, Fr.ctypes.data, ka.ctypes.data,
+                                       ks.ctypes.data, G.ctypes.data))
+    return G
+
+
+def device_query(device: int = 0):
+    name = C.create_string_buffer(128)
+    cus = C.c_int32()
+    gb = C.c_double()
+    check(_lib().rhllc_device_query(device, name, 128, C.byref(cus), C.byref(gb)))
+    return {"arch": name.value.decode(), "cus": cus.value, "hbm_gb": gb.value}

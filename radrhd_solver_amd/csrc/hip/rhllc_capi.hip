@@ -1,0 +1,659 @@
+// C ABI of librhllc_hip.so (declarations and the reference interfaces each
+// entry point replaces: include/rhllc_hip.h).
+//
+// A handle owns the mesh and state in device memory (SoA, Morton-relabelled
+// when asked), one HIP stream, and optionally a captured hipGraph of the step.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../../include/rhllc_hip.h"
+#include "../mesh/mesh_common.h"
+#include "rhllc_internal.h"
+
+using namespace rhllc_dev;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+int hip_fail(hipError_t e, const char *what) {
+  return fail(RHLLC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                   \
+  do {                                                  \
+    hipError_t e_ = (expr);                             \
+    if (e_ != hipSuccess) return hip_fail(e_, #expr);   \
+  } while (0)
+
+}  // namespace
+
+struct rhllc_solver {
+  rhllc_params prm{};
+  int32_t n_cells = 0, n_faces = 0;
+  bool mixed = false;
+  int variant = kFp64Ref;
+  bool has_state = false;
+  bool rad = false;
+  hipStream_t stream = nullptr;
+  Dev d{};
+  std::vector<void *> allocs;
+  std::vector<int32_t> cell_new_to_old, face_new_to_old;
+  Scalars *host_sc = nullptr;  // pinned staging of the device scalars
+  float4 *rad_FU = nullptr;
+  float2 *rad_k = nullptr;
+  hipGraphExec_t graph = nullptr;
+  hipGraph_t graph_src = nullptr;
+
+  template <typename T>
+  hipError_t alloc(T **p, size_t count) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T) + 64);
+    if (e == hipSuccess) allocs.push_back(q);
+    *p = static_cast<T *>(q);
+    return e;
+  }
+
+  void drop_graph() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (graph_src) (void)hipGraphDestroy(graph_src);
+    graph = nullptr;
+    graph_src = nullptr;
+  }
+
+  ~rhllc_solver() {
+    drop_graph();
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void *p : allocs) (void)hipFree(p);
+    if (rad_FU) (void)hipFree(rad_FU);
+    if (rad_k) (void)hipFree(rad_k);
+    if (host_sc) (void)hipHostFree(host_sc);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  hipError_t enqueue_step() {
+    hipError_t e = launch_face(d, mixed, stream);
+    if (e != hipSuccess) return e;
+    e = launch_cell(d, variant, rad, prm.collect_stats != 0, stream);
+    if (e != hipSuccess) return e;
+    return launch_dt(d, stream);
+  }
+};
+
+extern "C" const char *rhllc_last_error(void) { return g_err.c_str(); }
+
+extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const void *geo_cells, int64_t n_faces,
+                            const void *geo_faces, rhllc_solver **out) {
+  if (!params || !out || !geo_cells || !geo_faces) return fail(RHLLC_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (n_cells < 1 || n_cells >= ((int64_t)1 << 25))
+    return fail(RHLLC_ERR_ARG, "n_cells must be in [1, 2^25) (geo_face_t::id_l is 25 bits)");
+  if (n_faces < 1 || n_faces > 0x7FFFFFFF) return fail(RHLLC_ERR_ARG, "n_faces out of range");
+  const rhllc_params &p = *params;
+  if (!(p.gamma > 1.0)) return fail(RHLLC_ERR_ARG, "gamma must be > 1");
+  if (!(p.cfl > 0.0) || !(p.h_min > 0.0)) return fail(RHLLC_ERR_ARG, "CFL and h_min must be > 0");
+  if (p.precision != RHLLC_PREC_FP64 && p.precision != RHLLC_PREC_MIXED)
+    return fail(RHLLC_ERR_ARG, "unknown precision");
+  int variant;
+  if (p.precision == RHLLC_PREC_FP64) {
+    if (p.c2p_guess == RHLLC_GUESS_REFERENCE)
+      variant = kFp64Ref;
+    else if (p.c2p_guess == RHLLC_GUESS_PREVIOUS)
+      variant = kFp64Prev;
+    else
+      return fail(RHLLC_ERR_ARG, "the FP64 path supports the reference and previous-pressure starts");
+  } else {
+    if (p.c2p_guess == RHLLC_GUESS_PREVIOUS)
+      variant = kMixedPrev;
+    else
+      return fail(RHLLC_ERR_ARG, "the mixed path supports the previous-pressure start");
+  }
+
+  const int32_t N = (int32_t)n_cells, F = (int32_t)n_faces;
+  const uint8_t *gc = static_cast<const uint8_t *>(geo_cells);
+  const uint8_t *gf = static_cast<const uint8_t *>(geo_faces);
+
+  // Decode and validate so every device gather stays in bounds.
+  std::vector<int32_t> fl(F), fr(F);
+  for (int32_t f = 0; f < F; ++f) {
+    const uint8_t *r = gf + (size_t)f * RHLLC_GEO_FACE_BYTES;
+    const int32_t l = (int32_t)rhllc_mesh::face_id_l(r);
+    const int32_t rr = rhllc_mesh::face_id_r(r);
+    if (l >= N) return fail(RHLLC_ERR_ARG, "face " + std::to_string(f) + ": id_l out of range");
+    if (rr >= N) return fail(RHLLC_ERR_ARG, "face " + std::to_string(f) + ": id_r out of range");
+    if (rr < RHLLC_BOUND_OUTER_SURFACE)
+      return fail(RHLLC_ERR_ARG, "face " + std::to_string(f) + ": unknown boundary code " + std::to_string(rr));
+    if (rr == RHLLC_BOUND_OUT_SOURCE && !p.has_out_source)
+      return fail(RHLLC_ERR_ARG, "mesh has e_bound_out_source (-3) faces but no out_source state was given");
+    fl[f] = l;
+    fr[f] = rr;
+  }
+  for (int32_t c = 0; c < N; ++c) {
+    const uint8_t *r = gc + (size_t)c * RHLLC_GEO_CELL_BYTES;
+    int32_t ids[4];
+    rhllc_mesh::cell_ids(r, ids);
+    const uint8_t sign = r[48];
+    for (int j = 0; j < 4; ++j) {
+      if (ids[j] < 0 || ids[j] >= F) return fail(RHLLC_ERR_ARG, "cell " + std::to_string(c) + ": face id out of range");
+      const bool is_left = (sign >> j) & 1;
+      if (is_left ? fl[ids[j]] != c : fr[ids[j]] != c)
+        return fail(RHLLC_ERR_ARG, "cell " + std::to_string(c) + ": slot " + std::to_string(j) +
+                                       " disagrees with its face's id_l/id_r and sign bit");
+    }
+    if (!(rhllc_mesh::cell_V(r) > 0.0)) return fail(RHLLC_ERR_ARG, "cell " + std::to_string(c) + ": V <= 0");
+  }
+
+  std::unique_ptr<rhllc_solver> h(new rhllc_solver());
+  h->prm = p;
+  h->n_cells = N;
+  h->n_faces = F;
+  h->mixed = p.precision == RHLLC_PREC_MIXED;
+  h->variant = variant;
+  HIP_TRY(hipSetDevice(p.device));
+  HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HIP_TRY(hipHostMalloc((void **)&h->host_sc, sizeof(Scalars), hipHostMallocDefault));
+
+  // Relabel (or keep) the numbering.
+  h->cell_new_to_old.resize(N);
+  h->face_new_to_old.resize(F);
+  if (p.reorder) {
+    rhllc_mesh::morton_order_impl(N, geo_cells, F, geo_faces, h->cell_new_to_old.data(), h->face_new_to_old.data());
+  } else {
+    for (int32_t i = 0; i < N; ++i) h->cell_new_to_old[i] = i;
+    for (int32_t i = 0; i < F; ++i) h->face_new_to_old[i] = i;
+  }
+  std::vector<int32_t> cnew(N), fnew(F);
+  for (int32_t i = 0; i < N; ++i) cnew[h->cell_new_to_old[i]] = i;
+  for (int32_t i = 0; i < F; ++i) fnew[h->face_new_to_old[i]] = i;
+
+  std::vector<int2> lr(F);
+  std::vector<double4> nS(h->mixed ? 0 : F);
+  std::vector<float4> nS32(h->mixed ? F : 0);
+  for (int32_t i = 0; i < F; ++i) {
+    const int32_t f = h->face_new_to_old[i];
+    const uint8_t *r = gf + (size_t)f * RHLLC_GEO_FACE_BYTES;
+    double n[3], S;
+    std::memcpy(n, r + 8, 24);
+    std::memcpy(&S, r + 32, 8);
+    const int32_t l = cnew[fl[f]];
+    // Copy-type boundaries (Cone geometry: every code but -3) read the cell itself.
+    const int32_t rr = fr[f] >= 0 ? cnew[fr[f]] : (fr[f] == RHLLC_BOUND_OUT_SOURCE ? -1 : l);
+    lr[i] = make_int2(l, rr);
+    if (h->mixed)
+      nS32[i] = make_float4((float)n[0], (float)n[1], (float)n[2], (float)S);
+    else
+      nS[i] = make_double4(n[0], n[1], n[2], S);
+  }
+  std::vector<int4> cf(N);
+  std::vector<double> V(N);
+  for (int32_t i = 0; i < N; ++i) {
+    const int32_t c = h->cell_new_to_old[i];
+    const uint8_t *r = gc + (size_t)c * RHLLC_GEO_CELL_BYTES;
+    int32_t ids[4], e[4];
+    rhllc_mesh::cell_ids(r, ids);
+    for (int j = 0; j < 4; ++j) e[j] = fnew[ids[j]] | (((r[48] >> j) & 1) ? 0 : (int32_t)0x80000000u);
+    cf[i] = make_int4(e[0], e[1], e[2], e[3]);
+    V[i] = rhllc_mesh::cell_V(r);
+  }
+
+  Dev &d = h->d;
+  d.n_cells = N;
+  d.n_faces = F;
+  d.gamma = p.gamma;
+  d.Gg = p.gamma / (p.gamma - 1.0);
+  d.cfl_hmin = p.cfl * p.h_min;
+  for (int k = 0; k < 5; ++k) d.ghostW[k] = p.out_source_W[k], d.ghostU[k] = p.out_source_U[k];
+
+  int2 *lr_d;
+  int4 *cf_d;
+  double *V_d;
+  HIP_TRY(h->alloc(&lr_d, F));
+  HIP_TRY(h->alloc(&cf_d, N));
+  HIP_TRY(h->alloc(&V_d, N));
+  HIP_TRY(hipMemcpy(lr_d, lr.data(), sizeof(int2) * F, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(cf_d, cf.data(), sizeof(int4) * N, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(V_d, V.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+  d.face_lr = lr_d;
+  d.cell_faces = cf_d;
+  d.cell_V = V_d;
+  if (h->mixed) {
+    float4 *q;
+    HIP_TRY(h->alloc(&q, F));
+    HIP_TRY(hipMemcpy(q, nS32.data(), sizeof(float4) * F, hipMemcpyHostToDevice));
+    d.face_nS32 = q;
+    HIP_TRY(h->alloc(&d.F4f, F));
+    HIP_TRY(h->alloc(&d.FEf, F));
+  } else {
+    double4 *q;
+    HIP_TRY(h->alloc(&q, F));
+    HIP_TRY(hipMemcpy(q, nS.data(), sizeof(double4) * F, hipMemcpyHostToDevice));
+    d.face_nS = q;
+    HIP_TRY(h->alloc(&d.F4, F));
+    HIP_TRY(h->alloc(&d.FE, F));
+  }
+  HIP_TRY(h->alloc(&d.U4, N));
+  HIP_TRY(h->alloc(&d.UE, N));
+  HIP_TRY(h->alloc(&d.W4, N));
+  HIP_TRY(h->alloc(&d.Wp, N));
+  HIP_TRY(h->alloc(&d.W4f, N));
+  HIP_TRY(h->alloc(&d.Wpf, N));
+  HIP_TRY(h->alloc(&d.sc, 1));
+  HIP_TRY(h->alloc(&d.speed_slots, (size_t)kSpeedSlots * kSlotStride));
+  std::memset(h->host_sc, 0, sizeof(Scalars));
+  h->host_sc->tau = d.cfl_hmin / 1.0;
+  h->host_sc->last_speed = 1.0;
+  HIP_TRY(hipMemcpy(d.sc, h->host_sc, sizeof(Scalars), hipMemcpyHostToDevice));
+  HIP_TRY(launch_clear_scalars(d, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  *out = h.release();
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_destroy(rhllc_solver *h) {
+  delete h;
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_upload_state(rhllc_solver *h, const void *recs, int64_t stride, int64_t off_phys,
+                                  int64_t off_conv) {
+  if (!h || !recs) return fail(RHLLC_ERR_ARG, "null argument");
+  if (off_phys < 0 || stride < 40 || off_phys + 40 > stride || (off_conv >= 0 && off_conv + 40 > stride))
+    return fail(RHLLC_ERR_ARG, "bad record stride/offsets");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  const int32_t N = h->n_cells;
+  const uint8_t *base = static_cast<const uint8_t *>(recs);
+  std::vector<double4> w4(N), u4(off_conv >= 0 ? N : 0);
+  std::vector<double> wp(N), ue(off_conv >= 0 ? N : 0);
+  std::vector<float4> w4f(N);
+  std::vector<float> wpf(N);
+  for (int32_t i = 0; i < N; ++i) {
+    const uint8_t *r = base + (size_t)h->cell_new_to_old[i] * (size_t)stride;
+    double w[5];
+    std::memcpy(w, r + off_phys, 40);
+    w4[i] = make_double4(w[0], w[1], w[2], w[3]);
+    wp[i] = w[4];
+    w4f[i] = make_float4((float)w[0], (float)w[1], (float)w[2], (float)w[3]);
+    wpf[i] = (float)w[4];
+    if (off_conv >= 0) {
+      double u[5];
+      std::memcpy(u, r + off_conv, 40);
+      u4[i] = make_double4(u[0], u[1], u[2], u[3]);
+      ue[i] = u[4];
+    }
+  }
+  Dev &d = h->d;
+  HIP_TRY(hipMemcpy(d.W4, w4.data(), sizeof(double4) * N, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d.Wp, wp.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+  if (off_conv >= 0) {
+    HIP_TRY(hipMemcpy(d.U4, u4.data(), sizeof(double4) * N, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.UE, ue.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.W4f, w4f.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.Wpf, wpf.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+  } else {
+    // rhllc::Init (rhllc_init.cpp:190-192): HllcPhysToConv, max_signal_speed = 1,
+    // tau = GetTimeStep.
+    HIP_TRY(launch_p2c_init(d, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(h->host_sc, d.sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    h->host_sc->tau = d.cfl_hmin / 1.0;
+    h->host_sc->t = 0.0;
+    h->host_sc->last_speed = 1.0;
+    HIP_TRY(hipMemcpy(d.sc, h->host_sc, sizeof(Scalars), hipMemcpyHostToDevice));
+  }
+  h->has_state = true;
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_download_state(rhllc_solver *h, void *recs, int64_t stride, int64_t off_phys,
+                                    int64_t off_conv) {
+  if (!h || !recs) return fail(RHLLC_ERR_ARG, "null argument");
+  if (stride < 40 || (off_phys >= 0 && off_phys + 40 > stride) || (off_conv >= 0 && off_conv + 40 > stride))
+    return fail(RHLLC_ERR_ARG, "bad record stride/offsets");
+  if (!h->has_state) return fail(RHLLC_ERR_STATE, "no state uploaded");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const int32_t N = h->n_cells;
+  const Dev &d = h->d;
+  uint8_t *base = static_cast<uint8_t *>(recs);
+  if (off_phys >= 0) {
+    std::vector<double> wp(N);
+    HIP_TRY(hipMemcpy(wp.data(), d.Wp, sizeof(double) * N, hipMemcpyDeviceToHost));
+    if (!h->mixed) {
+      std::vector<double4> w4(N);
+      HIP_TRY(hipMemcpy(w4.data(), d.W4, sizeof(double4) * N, hipMemcpyDeviceToHost));
+      for (int32_t i = 0; i < N; ++i) {
+        const double w[5] = {w4[i].x, w4[i].y, w4[i].z, w4[i].w, wp[i]};
+        std::memcpy(base + (size_t)h->cell_new_to_old[i] * stride + off_phys, w, 40);
+      }
+    } else {
+      std::vector<float4> w4(N);
+      HIP_TRY(hipMemcpy(w4.data(), d.W4f, sizeof(float4) * N, hipMemcpyDeviceToHost));
+      for (int32_t i = 0; i < N; ++i) {
+        const double w[5] = {w4[i].x, w4[i].y, w4[i].z, w4[i].w, wp[i]};
+        std::memcpy(base + (size_t)h->cell_new_to_old[i] * stride + off_phys, w, 40);
+      }
+    }
+  }
+  if (off_conv >= 0) {
+    std::vector<double4> u4(N);
+    std::vector<double> ue(N);
+    HIP_TRY(hipMemcpy(u4.data(), d.U4, sizeof(double4) * N, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ue.data(), d.UE, sizeof(double) * N, hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < N; ++i) {
+      const double u[5] = {u4[i].x, u4[i].y, u4[i].z, u4[i].w, ue[i]};
+      std::memcpy(base + (size_t)h->cell_new_to_old[i] * stride + off_conv, u, 40);
+    }
+  }
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_step(rhllc_solver *h, double tau, double *max_speed) {
+  if (!h) return fail(RHLLC_ERR_ARG, "null handle");
+  if (!h->has_state) return fail(RHLLC_ERR_STATE, "no state uploaded");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  h->host_sc->tau = tau;
+  HIP_TRY(hipMemcpyAsync(&h->d.sc->tau, &h->host_sc->tau, sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(h->enqueue_step());
+  HIP_TRY(hipMemcpyAsync(h->host_sc, h->d.sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const double s = h->host_sc->last_speed;
+  if (max_speed) *max_speed = s;
+  if (!std::isfinite(s)) return fail(RHLLC_ERR_NUMERIC, "non-finite max signal speed");
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_run(rhllc_solver *h, int32_t nsteps, int32_t use_graph) {
+  if (!h) return fail(RHLLC_ERR_ARG, "null handle");
+  if (!h->has_state) return fail(RHLLC_ERR_STATE, "no state uploaded");
+  if (nsteps < 0) return fail(RHLLC_ERR_ARG, "nsteps < 0");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  if (!use_graph) {
+    for (int32_t k = 0; k < nsteps; ++k) HIP_TRY(h->enqueue_step());
+    return RHLLC_OK;
+  }
+  if (!h->graph) {
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = h->enqueue_step();
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+    if (e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return hip_fail(e, "step capture");
+    }
+    if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+    h->graph_src = g;
+    HIP_TRY(hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
+  }
+  for (int32_t k = 0; k < nsteps; ++k) HIP_TRY(hipGraphLaunch(h->graph, h->stream));
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_sync(rhllc_solver *h) {
+  if (!h) return fail(RHLLC_ERR_ARG, "null handle");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_set_time(rhllc_solver *h, double tau, double t) {
+  if (!h) return fail(RHLLC_ERR_ARG, "null handle");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->host_sc->tau = tau;
+  h->host_sc->t = t;
+  HIP_TRY(hipMemcpy(&h->d.sc->tau, &h->host_sc->tau, 2 * sizeof(double), hipMemcpyHostToDevice));
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_get_stats(rhllc_solver *h, rhllc_stats *out) {
+  if (!h || !out) return fail(RHLLC_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  Scalars s;
+  HIP_TRY(hipMemcpy(&s, h->d.sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+  std::memset(out, 0, sizeof(*out));
+  out->steps = (int64_t)s.steps;
+  for (int k = 0; k < 6; ++k) out->c2p_fail[k] = (int64_t)s.c2p_fail[k];
+  out->fix_calls = (int64_t)s.fix_calls;
+  out->fix_fail = (int64_t)s.fix_fail;
+  out->fallback_cells = (int64_t)s.fallback;
+  out->newton_iters = (int64_t)s.newton_iters;
+  for (int k = 0; k < RHLLC_HIST_BINS; ++k) out->newton_hist[k] = (int64_t)s.hist[k];
+  out->max_speed = s.last_speed;
+  out->tau = s.tau;
+  out->t = s.t;
+  double r;
+  std::memcpy(&r, &s.max_residual_bits, 8);
+  out->max_residual = r;
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_reset_stats(rhllc_solver *h) {
+  if (!h) return fail(RHLLC_ERR_ARG, "null handle");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  HIP_TRY(launch_clear_scalars(h->d, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_download_face_flux(rhllc_solver *h, double *F) {
+  if (!h || !F) return fail(RHLLC_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const int32_t nf = h->n_faces;
+  if (!h->mixed) {
+    std::vector<double4> a(nf);
+    std::vector<double> e(nf);
+    HIP_TRY(hipMemcpy(a.data(), h->d.F4, sizeof(double4) * nf, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(e.data(), h->d.FE, sizeof(double) * nf, hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < nf; ++i) {
+      double *o = F + 5 * (size_t)h->face_new_to_old[i];
+      o[0] = a[i].x, o[1] = a[i].y, o[2] = a[i].z, o[3] = a[i].w, o[4] = e[i];
+    }
+  } else {
+    std::vector<float4> a(nf);
+    std::vector<float> e(nf);
+    HIP_TRY(hipMemcpy(a.data(), h->d.F4f, sizeof(float4) * nf, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(e.data(), h->d.FEf, sizeof(float) * nf, hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < nf; ++i) {
+      double *o = F + 5 * (size_t)h->face_new_to_old[i];
+      o[0] = a[i].x, o[1] = a[i].y, o[2] = a[i].z, o[3] = a[i].w, o[4] = e[i];
+    }
+  }
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_set_radiation(rhllc_solver *h, const float *Ur, const float *Fr3, const float *kappa_a,
+                                   const float *kappa_s) {
+  if (!h) return fail(RHLLC_ERR_ARG, "null handle");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->drop_graph();
+  if (!Ur) {
+    h->rad = false;
+    return RHLLC_OK;
+  }
+  if (!Fr3 || !kappa_a || !kappa_s) return fail(RHLLC_ERR_ARG, "null radiation array");
+  const int32_t N = h->n_cells;
+  std::vector<float4> fu(N);
+  std::vector<float2> kk(N);
+  for (int32_t i = 0; i < N; ++i) {
+    const int32_t c = h->cell_new_to_old[i];
+    fu[i] = make_float4(Fr3[3 * (size_t)c], Fr3[3 * (size_t)c + 1], Fr3[3 * (size_t)c + 2], Ur[c]);
+    kk[i] = make_float2(kappa_a[c], kappa_s[c]);
+  }
+  if (!h->rad_FU) HIP_TRY(hipMalloc((void **)&h->rad_FU, sizeof(float4) * N));
+  if (!h->rad_k) HIP_TRY(hipMalloc((void **)&h->rad_k, sizeof(float2) * N));
+  HIP_TRY(hipMemcpy(h->rad_FU, fu.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->rad_k, kk.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
+  h->d.rad_FU = h->rad_FU;
+  h->d.rad_k = h->rad_k;
+  h->rad = true;
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_time_kernels(rhllc_solver *h, int32_t nsteps, double out[4]) {
+  if (!h || !out) return fail(RHLLC_ERR_ARG, "null argument");
+  if (!h->has_state) return fail(RHLLC_ERR_STATE, "no state uploaded");
+  if (nsteps < 1) return fail(RHLLC_ERR_ARG, "nsteps < 1");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  hipEvent_t ev[4];
+  for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+  double acc[4] = {0, 0, 0, 0};
+  int rc = RHLLC_OK;
+  for (int32_t k = 0; k < nsteps && rc == RHLLC_OK; ++k) {
+    hipError_t e = hipEventRecord(ev[0], h->stream);
+    if (e == hipSuccess) e = launch_face(h->d, h->mixed, h->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], h->stream);
+    if (e == hipSuccess) e = launch_cell(h->d, h->variant, h->rad, h->prm.collect_stats != 0, h->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[2], h->stream);
+    if (e == hipSuccess) e = launch_dt(h->d, h->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[3], h->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[3]);
+    float ms[4] = {0, 0, 0, 0};
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[0], ev[0], ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[1], ev[1], ev[2]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[2], ev[2], ev[3]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[3], ev[0], ev[3]);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "kernel timing");
+      break;
+    }
+    for (int i = 0; i < 4; ++i) acc[i] += ms[i];
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  if (rc != RHLLC_OK) return rc;
+  for (int i = 0; i < 4; ++i) out[i] = acc[i] / nsteps;
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_info(rhllc_solver *h, int64_t *n_cells, int64_t *n_faces, int64_t *device_bytes) {
+  if (!h) return fail(RHLLC_ERR_ARG, "null handle");
+  if (n_cells) *n_cells = h->n_cells;
+  if (n_faces) *n_faces = h->n_faces;
+  if (device_bytes) {
+    const int64_t N = h->n_cells, F = h->n_faces;
+    // Algorithmic bytes of one step (both kernels), each array element counted
+    // once per kernel that touches it.
+    *device_bytes = h->mixed ? 161 * N + 64 * F : 233 * N + 120 * F;
+    if (h->rad) *device_bytes += 24 * N;
+  }
+  return RHLLC_OK;
+}
+
+// ------------------------------------------------------------ pointwise evals
+namespace {
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+hipError_t to_dev(DevBuf &b, const void *src, size_t bytes) {
+  hipError_t e = hipMalloc(&b.p, bytes + 64);
+  if (e == hipSuccess && src) e = hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice);
+  return e;
+}
+}  // namespace
+
+extern "C" int rhllc_eval_flux(int32_t precision, double gamma, int64_t n, const double *UL, const double *UR,
+                               const double *WL, const double *WR, const double *nS4, double *F, double *speed) {
+  if (n < 0 || !UL || !UR || !WL || !WR || !nS4 || !F || !speed) return fail(RHLLC_ERR_ARG, "bad argument");
+  DevBuf a, b, c, dd, e, f, g;
+  const size_t s5 = sizeof(double) * 5 * n;
+  HIP_TRY(to_dev(a, UL, s5));
+  HIP_TRY(to_dev(b, UR, s5));
+  HIP_TRY(to_dev(c, WL, s5));
+  HIP_TRY(to_dev(dd, WR, s5));
+  HIP_TRY(to_dev(e, nS4, sizeof(double) * 4 * n));
+  HIP_TRY(to_dev(f, nullptr, s5));
+  HIP_TRY(to_dev(g, nullptr, sizeof(double) * n));
+  HIP_TRY(launch_eval_flux(precision == RHLLC_PREC_MIXED, gamma, n, (double *)a.p, (double *)b.p, (double *)c.p,
+                           (double *)dd.p, (double *)e.p, (double *)f.p, (double *)g.p, nullptr));
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(F, f.p, s5, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(speed, g.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_eval_c2p(double gamma, int64_t n, const double *U, const double *guess, double *W, int32_t *codes,
+                              int32_t *iters) {
+  if (n < 0 || !U || !W || !codes || !iters) return fail(RHLLC_ERR_ARG, "bad argument");
+  DevBuf a, b, c, dd, e;
+  HIP_TRY(to_dev(a, U, sizeof(double) * 5 * n));
+  HIP_TRY(to_dev(c, W, sizeof(double) * 5 * n));
+  if (guess) HIP_TRY(to_dev(b, guess, sizeof(double) * n));
+  HIP_TRY(to_dev(dd, nullptr, sizeof(int32_t) * n));
+  HIP_TRY(to_dev(e, nullptr, sizeof(int32_t) * n));
+  HIP_TRY(launch_eval_c2p(gamma, n, (double *)a.p, (double *)b.p, (double *)c.p, (int32_t *)dd.p, (int32_t *)e.p,
+                          nullptr));
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(W, c.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(codes, dd.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(iters, e.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_eval_p2c(double gamma, int64_t n, double *W, double *U) {
+  if (n < 0 || !W || !U) return fail(RHLLC_ERR_ARG, "bad argument");
+  DevBuf a, b;
+  HIP_TRY(to_dev(a, W, sizeof(double) * 5 * n));
+  HIP_TRY(to_dev(b, nullptr, sizeof(double) * 5 * n));
+  HIP_TRY(launch_eval_p2c(gamma, n, (double *)a.p, (double *)b.p, nullptr));
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(W, a.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(U, b.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost));
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_eval_pressure_fix(double gamma, int64_t n, double *U, double *W, int32_t *codes) {
+  if (n < 0 || !U || !W || !codes) return fail(RHLLC_ERR_ARG, "bad argument");
+  DevBuf a, b, c;
+  HIP_TRY(to_dev(a, U, sizeof(double) * 5 * n));
+  HIP_TRY(to_dev(b, W, sizeof(double) * 5 * n));
+  HIP_TRY(to_dev(c, nullptr, sizeof(int32_t) * n));
+  HIP_TRY(launch_eval_fix(gamma, n, (double *)a.p, (double *)b.p, (int32_t *)c.p, nullptr));
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(U, a.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(W, b.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(codes, c.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_eval_rad_source(int64_t n, const double *W, const double *Ur, const double *Fr3,
+                                     const double *ka, const double *ks, double *G) {
+  if (n < 0 || !W || !Ur || !Fr3 || !ka || !ks || !G) return fail(RHLLC_ERR_ARG, "bad argument");
+  DevBuf a, b, c, dd, e, f;
+  HIP_TRY(to_dev(a, W, sizeof(double) * 5 * n));
+  HIP_TRY(to_dev(b, Ur, sizeof(double) * n));
+  HIP_TRY(to_dev(c, Fr3, sizeof(double) * 3 * n));
+  HIP_TRY(to_dev(dd, ka, sizeof(double) * n));
+  HIP_TRY(to_dev(e, ks, sizeof(double) * n));
+  HIP_TRY(to_dev(f, nullptr, sizeof(double) * 4 * n));
+  HIP_TRY(launch_eval_rad(n, (double *)a.p, (double *)b.p, (double *)c.p, (double *)dd.p, (double *)e.p,
+                          (double *)f.p, nullptr));
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(G, f.p, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_device_query(int32_t device, char *name, int32_t name_len, int32_t *cus, double *hbm_gb) {
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  if (name && name_len > 0) {
+    std::strncpy(name, prop.gcnArchName, (size_t)name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (cus) *cus = prop.multiProcessorCount;
+  if (hbm_gb) *hbm_gb = (double)prop.totalGlobalMem / 1e9;
+  return RHLLC_OK;
+}

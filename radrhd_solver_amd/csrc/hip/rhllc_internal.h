@@ -1,0 +1,85 @@
+// Device layout and kernel launchers of the RHLLC step (shared by the kernel
+// and C-ABI translation units of librhllc_hip.so).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rhllc_dev {
+
+constexpr int kBlock = 256;        // threads per workgroup of the face and cell kernels
+constexpr int kSpeedSlots = 1024;  // per-step max-speed partials (one atomic slot per block group)
+constexpr int kSlotStride = 16;    // doubles between slots: one 128-byte line each
+constexpr int kHistBins = 41;
+
+// Device-resident scalars of the time loop and the step counters.
+struct Scalars {
+  double tau;         // tau of the step about to run
+  double t;           // simulated time
+  double last_speed;  // max signal speed of the last finished step
+  unsigned long long steps;
+  unsigned long long c2p_fail[6];
+  unsigned long long fix_calls, fix_fail, fallback, newton_iters;
+  unsigned long long hist[kHistBins];
+  unsigned long long max_residual_bits;
+};
+
+// Everything a kernel needs, passed by value.  Cells and faces are in the
+// device (Morton) numbering.  SoA: each per-cell / per-face quantity is a
+// 16/32-byte vector array plus a scalar array so that gathers are single
+// wide loads.
+struct Dev {
+  int32_t n_cells, n_faces;
+  double gamma, Gg, cfl_hmin;
+  // faces
+  const int2 *face_lr;      // (left cell, right cell); right < 0: fixed ghost state
+  const double4 *face_nS;   // (nx, ny, nz, S), FP64 path
+  const float4 *face_nS32;  // (nx, ny, nz, S), FP32 path
+  double4 *F4;              // flux (d, mx, my, mz) * S, FP64 path
+  double *FE;               // flux E * S
+  float4 *F4f;              // FP32 path
+  float *FEf;
+  // cells
+  const int4 *cell_faces;  // face index | 0x80000000 when the cell is id_r (sign bit 0)
+  const double *cell_V;
+  double4 *U4;   // (D, mx, my, mz)
+  double *UE;    // E
+  double4 *W4;   // (rho, vx, vy, vz), FP64 path
+  double *Wp;    // p (both paths: the FP64 pressure)
+  float4 *W4f;   // (rho, vx, vy, vz), FP32 copy read by the mixed face kernel
+  float *Wpf;    // p, FP32 copy
+  // radiation source (optional): (Fx, Fy, Fz, Ur) and (kappa_a, kappa_s)
+  const float4 *rad_FU;
+  const float2 *rad_k;
+  // fixed ghost state for boundary code -3
+  double ghostW[5], ghostU[5];
+  // surrogate c2p guess (optional)
+  const void *mlp;
+  Scalars *sc;
+  double *speed_slots;  // kSpeedSlots * kSlotStride
+};
+
+enum Variant : int {
+  kFp64Ref = 0,   // FP64, reference c2p start (bitwise parity)
+  kFp64Prev = 1,  // FP64, previous-pressure start
+  kMixedPrev = 2, // mixed, previous-pressure start
+  kMixedMlp = 3,  // mixed, MLP start
+};
+
+hipError_t launch_p2c_init(const Dev &d, hipStream_t s);
+hipError_t launch_face(const Dev &d, bool mixed, hipStream_t s);
+hipError_t launch_cell(const Dev &d, int variant, bool rad, bool stats, hipStream_t s);
+hipError_t launch_dt(const Dev &d, hipStream_t s);
+hipError_t launch_clear_scalars(const Dev &d, hipStream_t s);
+
+// Pointwise evaluations (n states) for parity tests.
+hipError_t launch_eval_flux(bool fp32, double gamma, int64_t n, const double *UL, const double *UR, const double *WL,
+                            const double *WR, const double *nS4, double *F, double *speed, hipStream_t s);
+hipError_t launch_eval_c2p(double gamma, int64_t n, const double *U, const double *guess, double *W, int32_t *codes,
+                           int32_t *iters, hipStream_t s);
+hipError_t launch_eval_p2c(double gamma, int64_t n, double *W, double *U, hipStream_t s);
+hipError_t launch_eval_fix(double gamma, int64_t n, double *U, double *W, int32_t *codes, hipStream_t s);
+hipError_t launch_eval_rad(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
+                           const double *ks, double *G, hipStream_t s);
+
+}  // namespace rhllc_dev

@@ -1,0 +1,469 @@
+// gfx950 kernels of the RHLLC step.
+//
+// One step = three launches on one stream (solvers/rhllc/src/rhllc_3d_stab.cpp:10-65
+// plus rhllc_utils.cpp:15-23):
+//   k_face  one thread per face: gather left/right states, HLLC flux, write
+//           the face flux, fold the signal speed into per-slot maxima;
+//   k_cell  one thread per cell: gather its 4 face fluxes in slot order with
+//           their signs, U -= tau/V * sum, c2p (Newton, FP64) with the
+//           PhysPressureFix fallback, optional radiation source + second c2p;
+//   k_dt    one workgroup: max over the slots, tau = CFL*h_min/speed, t += tau.
+// No float atomics: face fluxes are written once and read by a fixed-order
+// gather, so the step is deterministic; the speed maximum is order-free.
+#include "rhllc_internal.h"
+#include "rhllc_physics.h"
+
+namespace rhllc_dev {
+
+// Workgroups are dispatched round-robin over the 8 XCDs.  Give each XCD a
+// contiguous range of logical blocks so neighbouring faces/cells (close in the
+// Morton numbering) are gathered through the same L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nblk) {
+  const uint32_t x = bid & 7u, k = bid >> 3;
+  const uint32_t q = nblk >> 3, r = nblk & 7u;
+  return x * q + (x < r ? x : r) + k;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double w = __shfl_xor(v, o, 64);
+    v = v < w ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ void load_U(const Dev &d, int c, State<double> &U) {
+  const double4 a = d.U4[c];
+  U.d = a.x, U.v[0] = a.y, U.v[1] = a.z, U.v[2] = a.w, U.p = d.UE[c];
+}
+__device__ __forceinline__ void load_W(const Dev &d, int c, State<double> &W) {
+  const double4 a = d.W4[c];
+  W.d = a.x, W.v[0] = a.y, W.v[1] = a.z, W.v[2] = a.w, W.p = d.Wp[c];
+}
+__device__ __forceinline__ void load_W32(const Dev &d, int c, State<float> &W) {
+  const float4 a = d.W4f[c];
+  W.d = a.x, W.v[0] = a.y, W.v[1] = a.z, W.v[2] = a.w, W.p = d.Wpf[c];
+}
+
+// Per-block max of non-negative speeds into one of kSpeedSlots slots.
+__device__ __forceinline__ void fold_speed(const Dev &d, double speed) {
+  __shared__ double part[kBlock / 64];
+  speed = wave_max(speed);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) part[wid] = speed;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = part[0];
+#pragma unroll
+    for (int k = 1; k < kBlock / 64; ++k) m = m < part[k] ? part[k] : m;
+    // Non-negative doubles order like their bit patterns.
+    unsigned long long *slot =
+        reinterpret_cast<unsigned long long *>(d.speed_slots + (size_t)(blockIdx.x % kSpeedSlots) * kSlotStride);
+    atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_face(Dev d) {
+  const int i = (int)(xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x);
+  double speed = 0.0;  // rhllc_3d_stab.cpp:18 -- per-thread max starts at 0
+  if (i < d.n_faces) {
+    const int2 lr = d.face_lr[i];
+    if (!MIXED) {
+      const double4 nS = d.face_nS[i];
+      State<double> UL, WL, UR, WR, F;
+      load_U(d, lr.x, UL);
+      load_W(d, lr.x, WL);
+      if (lr.y >= 0) {
+        load_U(d, lr.y, UR);
+        load_W(d, lr.y, WR);
+      } else {
+        UR.d = d.ghostU[0], UR.v[0] = d.ghostU[1], UR.v[1] = d.ghostU[2], UR.v[2] = d.ghostU[3], UR.p = d.ghostU[4];
+        WR.d = d.ghostW[0], WR.v[0] = d.ghostW[1], WR.v[1] = d.ghostW[2], WR.v[2] = d.ghostW[3], WR.p = d.ghostW[4];
+      }
+      const double s = hllc_flux<double>(d.gamma, d.Gg, UL, UR, WL, WR, nS.x, nS.y, nS.z, nS.w, F);
+      speed = smax(speed, s);
+      d.F4[i] = make_double4(F.d, F.v[0], F.v[1], F.v[2]);
+      d.FE[i] = F.p;
+    } else {
+      const float4 nS = d.face_nS32[i];
+      const float G1 = (float)d.gamma, Gg = (float)d.Gg;
+      State<float> UL, WL, UR, WR, F;
+      load_W32(d, lr.x, WL);
+      if (lr.y >= 0) {
+        load_W32(d, lr.y, WR);
+      } else {
+        WR.d = (float)d.ghostW[0], WR.v[0] = (float)d.ghostW[1], WR.v[1] = (float)d.ghostW[2];
+        WR.v[2] = (float)d.ghostW[3], WR.p = (float)d.ghostW[4];
+      }
+      // U is rebuilt from the FP32 primitives instead of being read.
+      p2c<float>(Gg, WL, UL);
+      p2c<float>(Gg, WR, UR);
+      const float s = hllc_flux<float>(G1, Gg, UL, UR, WL, WR, nS.x, nS.y, nS.z, nS.w, F);
+      speed = smax(speed, (double)s);
+      d.F4f[i] = make_float4(F.d, F.v[0], F.v[1], F.v[2]);
+      d.FEf[i] = F.p;
+    }
+  }
+  fold_speed(d, speed);
+}
+
+template <bool MIXED>
+__device__ __forceinline__ void load_flux(const Dev &d, int f, State<double> &F) {
+  if (!MIXED) {
+    const double4 a = d.F4[f];
+    F.d = a.x, F.v[0] = a.y, F.v[1] = a.z, F.v[2] = a.w, F.p = d.FE[f];
+  } else {
+    const float4 a = d.F4f[f];
+    F.d = a.x, F.v[0] = a.y, F.v[1] = a.z, F.v[2] = a.w, F.p = d.FEf[f];
+  }
+}
+
+// Counters of rare events: one atomic per wave that saw the event.
+__device__ __forceinline__ void count_if(unsigned long long *ctr, bool pred) {
+  const unsigned long long m = __ballot(pred);
+  if (m && (threadIdx.x & 63) == __ffsll((long long)m) - 1) atomicAdd(ctr, (unsigned long long)__popcll(m));
+}
+
+template <int VARIANT, bool RAD, bool STATS>
+__global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
+  constexpr bool MIXED = VARIANT >= kMixedPrev;
+  const int c = (int)(xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x);
+  const bool live = c < d.n_cells;
+  int iters = 0, code = 0;
+  bool fell_back = false, fixed = false, fix_failed = false;
+  double resid = 0.0;
+  if (live) {
+    const int4 cf = d.cell_faces[c];
+    const int fidx[4] = {cf.x, cf.y, cf.z, cf.w};
+    State<double> s;
+    s.d = 0.0, s.v[0] = 0.0, s.v[1] = 0.0, s.v[2] = 0.0, s.p = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      State<double> F;
+      load_flux<MIXED>(d, fidx[j] & 0x7FFFFFFF, F);
+      if (fidx[j] >= 0) {
+        s.d += F.d, s.v[0] += F.v[0], s.v[1] += F.v[1], s.v[2] += F.v[2], s.p += F.p;
+      } else {
+        s.d -= F.d, s.v[0] -= F.v[0], s.v[1] -= F.v[1], s.v[2] -= F.v[2], s.p -= F.p;
+      }
+    }
+    const double tau = d.sc->tau;
+    const double k = tau / d.cell_V[c];
+    s.d *= k, s.v[0] *= k, s.v[1] *= k, s.v[2] *= k, s.p *= k;
+    State<double> U, W;
+    load_U(d, c, U);
+    U.d -= s.d, U.v[0] -= s.v[0], U.v[1] -= s.v[1], U.v[2] -= s.v[2], U.p -= s.p;
+    // The reference start ignores the previous W; the other variants read only
+    // the previous FP64 pressure.  c2p writes W only when it succeeds.
+    const double guess = (VARIANT == kFp64Ref) ? -1.0 : d.Wp[c];
+    code = c2p(d.Gg, U, W, guess, iters);
+    if (code && VARIANT != kFp64Ref) {
+      int it2 = 0;
+      fell_back = true;
+      code = c2p(d.Gg, U, W, -1.0, it2);
+      iters += it2;
+    }
+    if (code) {
+      // PhysPressureFix receives the cell's previous phys_val (rhllc_3d_stab.cpp:49-52).
+      if (!MIXED) {
+        load_W(d, c, W);
+      } else {
+        const float4 a = d.W4f[c];
+        W.d = a.x, W.v[0] = a.y, W.v[1] = a.z, W.v[2] = a.w, W.p = d.Wp[c];
+      }
+      fixed = true;
+      fix_failed = pressure_fix(d.Gg, U, W) != 0;
+    }
+    if (RAD) {
+      // radRHD_main.cpp:117-132: U += (ds*tau)*G(W), ds = 1, then c2p (code ignored).
+      const float4 fu = d.rad_FU[c];
+      const float2 kk = d.rad_k[c];
+      const double Fr[3] = {(double)fu.x, (double)fu.y, (double)fu.z};
+      double G[4];
+      rad_source(W, (double)fu.w, Fr, (double)kk.x, (double)kk.y, G);
+      const double ds = 1;
+      U.p += ds * tau * G[0];
+      U.v[0] += ds * tau * G[1];
+      U.v[1] += ds * tau * G[2];
+      U.v[2] += ds * tau * G[3];
+      int it3 = 0;
+      c2p(d.Gg, U, W, MIXED ? W.p : -1.0, it3);
+      iters += it3;
+    }
+    if (STATS) resid = c2p_residual(d.Gg, U, W.p);
+    d.U4[c] = make_double4(U.d, U.v[0], U.v[1], U.v[2]);
+    d.UE[c] = U.p;
+    d.Wp[c] = W.p;
+    if (!MIXED) {
+      d.W4[c] = make_double4(W.d, W.v[0], W.v[1], W.v[2]);
+    } else {
+      d.W4f[c] = make_float4((float)W.d, (float)W.v[0], (float)W.v[1], (float)W.v[2]);
+      d.Wpf[c] = (float)W.p;
+    }
+  }
+  count_if(&d.sc->c2p_fail[1], live && code == 1);
+  count_if(&d.sc->c2p_fail[2], live && code == 2);
+  count_if(&d.sc->c2p_fail[3], live && code == 3);
+  count_if(&d.sc->c2p_fail[4], live && code == 4);
+  count_if(&d.sc->c2p_fail[5], live && code == 5);
+  count_if(&d.sc->fix_calls, fixed);
+  count_if(&d.sc->fix_fail, fix_failed);
+  count_if(&d.sc->fallback, fell_back);
+  if (STATS) {
+    __shared__ unsigned int hist[kHistBins];
+    __shared__ unsigned long long isum[kBlock / 64];
+    __shared__ double rmax[kBlock / 64];
+    for (int b = threadIdx.x; b < kHistBins; b += kBlock) hist[b] = 0;
+    __syncthreads();
+    if (live) atomicAdd(&hist[iters < kHistBins ? iters : kHistBins - 1], 1u);
+    const unsigned long long ws = wave_sum_u64(live ? (unsigned long long)iters : 0ull);
+    const double wr = wave_max(resid);
+    if ((threadIdx.x & 63) == 0) isum[threadIdx.x >> 6] = ws, rmax[threadIdx.x >> 6] = wr;
+    __syncthreads();
+    for (int b = threadIdx.x; b < kHistBins; b += kBlock)
+      if (hist[b]) atomicAdd(&d.sc->hist[b], (unsigned long long)hist[b]);
+    if (threadIdx.x == 0) {
+      unsigned long long t = 0;
+      double r = 0.0;
+      for (int k = 0; k < kBlock / 64; ++k) t += isum[k], r = r < rmax[k] ? rmax[k] : r;
+      atomicAdd(&d.sc->newton_iters, t);
+      atomicMax(&d.sc->max_residual_bits, (unsigned long long)__double_as_longlong(r));
+    }
+  }
+}
+
+// tau of the next step from the max signal speed (rhllc_utils.cpp:15-23):
+// tau = CFL*h_min / speed.  Also clears the slots for the next step.
+__global__ __launch_bounds__(1024) void k_dt(Dev d) {
+  __shared__ double part[1024 / 64];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < kSpeedSlots; i += 1024) {
+    double *slot = d.speed_slots + (size_t)i * kSlotStride;
+    const double v = *slot;
+    m = m < v ? v : m;
+    *slot = 0.0;
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = part[0];
+    for (int k = 1; k < 1024 / 64; ++k) s = s < part[k] ? part[k] : s;
+    Scalars *sc = d.sc;
+    sc->t += sc->tau;
+    sc->last_speed = s;
+    sc->tau = d.cfl_hmin / s;
+    sc->steps += 1;
+  }
+}
+
+// HllcPhysToConv (rhllc_utils.cpp:25-33): U = p2c(W), W clamped in place;
+// also seeds the FP32 primitive copy.
+__global__ __launch_bounds__(kBlock) void k_p2c_init(Dev d) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= d.n_cells) return;
+  State<double> W, U;
+  load_W(d, c, W);
+  p2c<double>(d.Gg, W, U);
+  d.W4[c] = make_double4(W.d, W.v[0], W.v[1], W.v[2]);
+  d.Wp[c] = W.p;
+  d.U4[c] = make_double4(U.d, U.v[0], U.v[1], U.v[2]);
+  d.UE[c] = U.p;
+  d.W4f[c] = make_float4((float)W.d, (float)W.v[0], (float)W.v[1], (float)W.v[2]);
+  d.Wpf[c] = (float)W.p;
+}
+
+// Clears the counters and speed slots with plain stores (graph-safe).
+__global__ void k_clear(Dev d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kSpeedSlots) d.speed_slots[(size_t)i * kSlotStride] = 0.0;
+  if (i == 0) {
+    Scalars *sc = d.sc;
+    sc->steps = 0;
+    for (int k = 0; k < 6; ++k) sc->c2p_fail[k] = 0;
+    sc->fix_calls = sc->fix_fail = sc->fallback = sc->newton_iters = 0;
+    for (int k = 0; k < kHistBins; ++k) sc->hist[k] = 0;
+    sc->max_residual_bits = 0;
+  }
+}
+
+static inline unsigned int blocks_for(int64_t n) { return (unsigned int)((n + kBlock - 1) / kBlock); }
+
+hipError_t launch_p2c_init(const Dev &d, hipStream_t s) {
+  if (d.n_cells == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_p2c_init, dim3(blocks_for(d.n_cells)), dim3(kBlock), 0, s, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_face(const Dev &d, bool mixed, hipStream_t s) {
+  if (d.n_faces == 0) return hipSuccess;
+  if (mixed)
+    hipLaunchKernelGGL(k_face<true>, dim3(blocks_for(d.n_faces)), dim3(kBlock), 0, s, d);
+  else
+    hipLaunchKernelGGL(k_face<false>, dim3(blocks_for(d.n_faces)), dim3(kBlock), 0, s, d);
+  return hipGetLastError();
+}
+
+template <int V>
+static void launch_cell_v(const Dev &d, bool rad, bool stats, hipStream_t s) {
+  const dim3 g(blocks_for(d.n_cells)), b(kBlock);
+  if (rad) {
+    if (stats)
+      hipLaunchKernelGGL((k_cell<V, true, true>), g, b, 0, s, d);
+    else
+      hipLaunchKernelGGL((k_cell<V, true, false>), g, b, 0, s, d);
+  } else {
+    if (stats)
+      hipLaunchKernelGGL((k_cell<V, false, true>), g, b, 0, s, d);
+    else
+      hipLaunchKernelGGL((k_cell<V, false, false>), g, b, 0, s, d);
+  }
+}
+
+hipError_t launch_cell(const Dev &d, int variant, bool rad, bool stats, hipStream_t s) {
+  if (d.n_cells == 0) return hipSuccess;
+  switch (variant) {
+    case kFp64Ref:
+      launch_cell_v<kFp64Ref>(d, rad, stats, s);
+      break;
+    case kFp64Prev:
+      launch_cell_v<kFp64Prev>(d, rad, stats, s);
+      break;
+    case kMixedPrev:
+      launch_cell_v<kMixedPrev>(d, rad, stats, s);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_dt(const Dev &d, hipStream_t s) {
+  hipLaunchKernelGGL(k_dt, dim3(1), dim3(1024), 0, s, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_clear_scalars(const Dev &d, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear, dim3((kSpeedSlots + 255) / 256), dim3(256), 0, s, d);
+  return hipGetLastError();
+}
+
+// ----------------------------------------------------------- pointwise evals
+__device__ __forceinline__ void ld5(const double *p, State<double> &s) {
+  s.d = p[0], s.v[0] = p[1], s.v[1] = p[2], s.v[2] = p[3], s.p = p[4];
+}
+__device__ __forceinline__ void st5(double *p, const State<double> &s) {
+  p[0] = s.d, p[1] = s.v[0], p[2] = s.v[1], p[3] = s.v[2], p[4] = s.p;
+}
+
+template <bool FP32>
+__global__ void k_eval_flux(double gamma, int64_t n, const double *UL, const double *UR, const double *WL,
+                            const double *WR, const double *nS4, double *F, double *speed) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double Gg = gamma / (gamma - 1.0);
+  State<double> ul, ur, wl, wr, f;
+  ld5(UL + 5 * i, ul), ld5(UR + 5 * i, ur), ld5(WL + 5 * i, wl), ld5(WR + 5 * i, wr);
+  if (!FP32) {
+    speed[i] = hllc_flux<double>(gamma, Gg, ul, ur, wl, wr, nS4[4 * i], nS4[4 * i + 1], nS4[4 * i + 2],
+                                 nS4[4 * i + 3], f);
+    st5(F + 5 * i, f);
+  } else {
+    // The mixed face path: U rebuilt from FP32 W.
+    State<float> a, b, ua, ub, ff;
+    a.d = (float)wl.d, a.v[0] = (float)wl.v[0], a.v[1] = (float)wl.v[1], a.v[2] = (float)wl.v[2], a.p = (float)wl.p;
+    b.d = (float)wr.d, b.v[0] = (float)wr.v[0], b.v[1] = (float)wr.v[1], b.v[2] = (float)wr.v[2], b.p = (float)wr.p;
+    p2c<float>((float)Gg, a, ua);
+    p2c<float>((float)Gg, b, ub);
+    speed[i] = hllc_flux<float>((float)gamma, (float)Gg, ua, ub, a, b, (float)nS4[4 * i], (float)nS4[4 * i + 1],
+                                (float)nS4[4 * i + 2], (float)nS4[4 * i + 3], ff);
+    double *o = F + 5 * i;
+    o[0] = ff.d, o[1] = ff.v[0], o[2] = ff.v[1], o[3] = ff.v[2], o[4] = ff.p;
+  }
+}
+
+__global__ void k_eval_c2p(double gamma, int64_t n, const double *U, const double *guess, double *W,
+                           int32_t *codes, int32_t *iters) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  State<double> u, w;
+  ld5(U + 5 * i, u);
+  ld5(W + 5 * i, w);
+  int it = 0;
+  codes[i] = c2p(gamma / (gamma - 1.0), u, w, guess ? guess[i] : -1.0, it);
+  iters[i] = it;
+  st5(W + 5 * i, w);
+}
+
+__global__ void k_eval_p2c(double gamma, int64_t n, double *W, double *U) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  State<double> w, u;
+  ld5(W + 5 * i, w);
+  p2c<double>(gamma / (gamma - 1.0), w, u);
+  st5(W + 5 * i, w);
+  st5(U + 5 * i, u);
+}
+
+__global__ void k_eval_fix(double gamma, int64_t n, double *U, double *W, int32_t *codes) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  State<double> u, w;
+  ld5(U + 5 * i, u);
+  ld5(W + 5 * i, w);
+  codes[i] = pressure_fix(gamma / (gamma - 1.0), u, w);
+  st5(U + 5 * i, u);
+  st5(W + 5 * i, w);
+}
+
+__global__ void k_eval_rad(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
+                           const double *ks, double *G) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  State<double> w;
+  ld5(W + 5 * i, w);
+  rad_source(w, Ur[i], Fr3 + 3 * i, ka[i], ks[i], G + 4 * i);
+}
+
+static inline dim3 eval_grid(int64_t n) { return dim3((unsigned int)((n + 255) / 256)); }
+
+hipError_t launch_eval_flux(bool fp32, double gamma, int64_t n, const double *UL, const double *UR, const double *WL,
+                            const double *WR, const double *nS4, double *F, double *speed, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (fp32)
+    hipLaunchKernelGGL(k_eval_flux<true>, eval_grid(n), dim3(256), 0, s, gamma, n, UL, UR, WL, WR, nS4, F, speed);
+  else
+    hipLaunchKernelGGL(k_eval_flux<false>, eval_grid(n), dim3(256), 0, s, gamma, n, UL, UR, WL, WR, nS4, F, speed);
+  return hipGetLastError();
+}
+hipError_t launch_eval_c2p(double gamma, int64_t n, const double *U, const double *guess, double *W, int32_t *codes,
+                           int32_t *iters, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_eval_c2p, eval_grid(n), dim3(256), 0, s, gamma, n, U, guess, W, codes, iters);
+  return hipGetLastError();
+}
+hipError_t launch_eval_p2c(double gamma, int64_t n, double *W, double *U, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_eval_p2c, eval_grid(n), dim3(256), 0, s, gamma, n, W, U);
+  return hipGetLastError();
+}
+hipError_t launch_eval_fix(double gamma, int64_t n, double *U, double *W, int32_t *codes, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_eval_fix, eval_grid(n), dim3(256), 0, s, gamma, n, U, W, codes);
+  return hipGetLastError();
+}
+hipError_t launch_eval_rad(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
+                           const double *ks, double *G, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_eval_rad, eval_grid(n), dim3(256), 0, s, n, W, Ur, Fr3, ka, ks, G);
+  return hipGetLastError();
+}
+
+}  // namespace rhllc_dev

@@ -1,0 +1,367 @@
+// Pointwise RHLLC physics for gfx950 (device functions).
+//
+// Each function is written from the reference's algorithm and keeps its
+// operation order, so the double instantiation is bitwise equal to the
+// reference compiled with strict IEEE arithmetic (the library is built with
+// -ffp-contract=off; every divide and sqrt is IEEE correctly rounded).  The
+// float instantiation is the mixed-precision face path.
+//
+//   p2c          rhllc::GetConvValueStab   solvers/rhllc/src/rhllc_flux_stab.cpp:10-38
+//   pressure_fix rhllc::PhysPressureFix    solvers/rhllc/src/rhllc_flux_stab.cpp:40-121
+//   c2p          rhllc::GetPhysValueStab   solvers/rhllc/src/rhllc_flux_stab.cpp:123-220
+//   rotation     GetRotationMatrix         lib/geometry/src/linear_alg.cpp:3-42
+//   hllc_flux    rhllc::GetFluxStab        solvers/rhllc/src/rhllc_flux_stab.cpp:231-431
+//
+// EOS is the ideal gas with a runtime adiabatic index (the reference's
+// kGamma1 is a constexpr 5/3, lib/global_consts.h:20-21).
+//
+// Sums of three products follow the reference's Eigen 3.4 evaluation on
+// x86-64 (2-double packets): dot products (a0+a1)+a2; rows 0 and 1 of T*v
+// (a0+a1)+a2, row 2 a0+(a1+a2); rows of T^T*v (a0+a1)+a2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace rhllc_dev {
+
+constexpr int kMaxIter = 40;  // rhllc_flux_stab.cpp:8
+
+template <typename R>
+struct State {
+  R d, v[3], p;  // W = (rho, v, p) or U = (D, m, E)
+};
+
+template <typename R>
+__device__ __forceinline__ R smax(R a, R b) {  // std::max, NaN behaviour included
+  return (a < b) ? b : a;
+}
+template <typename R>
+__device__ __forceinline__ R smin(R a, R b) {  // std::min
+  return (b < a) ? b : a;
+}
+template <typename R>
+__device__ __forceinline__ R dot3(const R *a, const R *b) {
+  return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+template <typename R>
+struct Rot {
+  R t[9];  // row-major, rows (n, t1, t2)
+};
+
+template <typename R>
+__device__ __forceinline__ Rot<R> rotation(R n0, R n1, R n2) {
+  Rot<R> T;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) T.t[i] = R(0);
+  const R eps = R(1e-12);
+  if (fabs(n2 * n2 - R(1)) > eps) {
+    T.t[0] = n0;
+    T.t[1] = n1;
+    T.t[2] = n2;
+    const R sqr = sqrt(R(1) - n2 * n2);
+    const R sqr_inv = R(1) / sqr;
+    T.t[3] = -n1 * sqr_inv;
+    T.t[4] = n0 * sqr_inv;
+    T.t[6] = -n0 * n2 * sqr_inv;
+    T.t[7] = -n1 * n2 * sqr_inv;
+    T.t[8] = sqr;
+  } else if (n2 > R(0)) {
+    T.t[2] = R(1);
+    T.t[4] = R(1);
+    T.t[6] = R(-1);
+  } else {
+    T.t[2] = R(-1);
+    T.t[4] = R(-1);
+    T.t[6] = R(1);
+  }
+  return T;
+}
+
+// r = T v
+template <typename R>
+__device__ __forceinline__ void rot_fwd(const Rot<R> &T, const R *v, R *r) {
+  const R r0 = (T.t[0] * v[0] + T.t[1] * v[1]) + T.t[2] * v[2];
+  const R r1 = (T.t[3] * v[0] + T.t[4] * v[1]) + T.t[5] * v[2];
+  const R r2 = T.t[6] * v[0] + (T.t[7] * v[1] + T.t[8] * v[2]);
+  r[0] = r0, r[1] = r1, r[2] = r2;
+}
+// r = T^T v
+template <typename R>
+__device__ __forceinline__ void rot_back(const Rot<R> &T, const R *v, R *r) {
+  const R r0 = (T.t[0] * v[0] + T.t[3] * v[1]) + T.t[6] * v[2];
+  const R r1 = (T.t[1] * v[0] + T.t[4] * v[1]) + T.t[7] * v[2];
+  const R r2 = (T.t[2] * v[0] + T.t[5] * v[1]) + T.t[8] * v[2];
+  r[0] = r0, r[1] = r1, r[2] = r2;
+}
+
+// Conserved from primitive; clamps |v| >= 1 to 0.9999 and rewrites W.v as the reference does.
+template <typename R>
+__device__ __forceinline__ void p2c(R Gg, State<R> &W, State<R> &U) {
+  const R h = R(1) + Gg * W.p / W.d;
+  R g = dot3(W.v, W.v);
+  if (g >= R(1)) {
+    const R beta_fix = R(0.9999);
+    g = beta_fix / sqrt(g);
+    W.v[0] *= g, W.v[1] *= g, W.v[2] *= g;
+    g = beta_fix * beta_fix;
+  }
+  g = R(1) / (R(1) - g);
+  const R scrh = W.d * h * g;
+  g = sqrt(g);
+  U.d = W.d * g;
+  U.v[0] = scrh * W.v[0], U.v[1] = scrh * W.v[1], U.v[2] = scrh * W.v[2];
+  U.p = scrh - W.p;
+}
+
+// Secant fallback at p = 1e-12 (kMinPressure); rewrites U.E on success.  Returns 0 or 2.
+__device__ __forceinline__ int pressure_fix(double Gg, State<double> &U, State<double> &W) {
+  const double p = 1e-12;
+  const double D = U.d;
+  const double m2 = dot3(U.v, U.v);
+  const double m = sqrt(m2);
+  const double umax = m / D;
+  double u0 = umax;
+  double lor = sqrt(1.0 + u0 * u0);
+  double plor = p * lor;
+  const double alpha = Gg;
+  double Dh = D + plor * alpha;
+  double f0 = m / Dh - u0;
+  double u1 = (-D + sqrt(D * D + 4.0 * m * alpha * p)) / (2.0 * alpha * p);
+  int done = 0;
+  double f1, du;
+  for (int k = 1; k < kMaxIter; k++) {
+    lor = sqrt(1.0 + u1 * u1);
+    plor = p * lor;
+    Dh = D + plor * alpha;
+    f1 = m / Dh - u1;
+    if (done == 1) break;
+    du = (u1 - u0) / (f1 - f0) * f1;
+    u0 = u1;
+    f0 = f1;
+    u1 -= du;
+    u1 = smin(u1, umax);
+    u1 = smax(u1, 0.0);
+    if (fabs(f1) < 1.e-9) done = 1;
+  }
+  if (done != 1) return 2;
+  lor = sqrt(1.0 + u1 * u1);
+  plor = p * lor;
+  Dh = D + plor * alpha;
+  W.d = U.d / lor;
+  W.p = p;
+  U.p = Dh * lor - p;
+  f0 = 1.0 / (Dh * lor);
+  W.v[0] = U.v[0] * f0, W.v[1] = U.v[1] * f0, W.v[2] = U.v[2] * f0;
+  return 0;
+}
+
+// Newton c2p.  guess < 0 selects the reference's initial pressure
+// max(|m| - E, 0, pmin); otherwise the start is max(guess, pmin).
+// Returns 0 or the reference's codes 1..4; 5 = the reference's DIE_IF(lor2 < 1).
+// On failure W is left unchanged.  *iters = Newton iterations done.
+__device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<double> &W, double guess, int &iters) {
+  const double tol = 1.e-11;
+  const double eps2 = 1.e-12;
+  const double D = U.d;
+  const double E = U.p;
+  const double m2 = dot3(U.v, U.v);
+  const double Q = E - sqrt(m2 + D * D);
+  iters = 0;
+  if (Q < 0.0) return 1;
+  const double m = sqrt(m2);
+  double p = smax(m - E, 0.0);
+  const double D_1 = 1.0 / D;
+  const double pmin = sqrt(m2 / (1.0 - eps2)) - E;
+  if (guess >= 0.0) p = guess;
+  p = smax(p, pmin);
+  double alpha, alpha2, lor2, lor, tau, theta, h, dh_dp, dh_dtau, yp = 0.0, dp, dyp;
+  int iter;
+  for (iter = 0; iter < kMaxIter; iter++) {
+    alpha = E + p;
+    alpha2 = alpha * alpha;
+    lor2 = 1.0 - m2 / alpha2;
+    lor2 = 1.0 / lor2;
+    if (lor2 < 1.0) {
+      iters = iter + 1;
+      return 5;
+    }
+    lor = sqrt(lor2);
+    tau = lor * D_1;
+    theta = p * tau;
+    h = 1.0 + Gg * theta;
+    dh_dp = Gg * tau;
+    dh_dtau = Gg * p;
+    yp = D * h * lor - E - p;
+    dyp = D * lor * dh_dp - m2 * lor2 * lor / (alpha2 * alpha) * (lor * dh_dtau + D * h) - 1.0;
+    dp = yp / dyp;
+    p -= dp;
+    if (p < pmin) p = pmin;
+    if (fabs(dp) < tol * E) break;
+  }
+  iters = iter < kMaxIter ? iter + 1 : kMaxIter;
+  if (p < 0.0) return 2;
+  if (isnan(p)) return 4;
+  if (iter >= kMaxIter || fabs(yp / (E + p)) > 1.e-4 || p < (m - E)) return 3;
+  alpha = E + p;
+  alpha2 = alpha * alpha;
+  lor2 = alpha2 / (alpha2 - m2);
+  lor = sqrt(lor2);
+  tau = lor * D_1;
+  W.d = 1.0 / tau;
+  W.p = p;
+  const double den = U.p + p;
+  W.v[0] = U.v[0] / den, W.v[1] = U.v[1] / den, W.v[2] = U.v[2] / den;
+  return 0;
+}
+
+// |f(p)| / (E + p) of the c2p equation at the recovered state (used for the
+// residual check of the mixed path; not part of the reference).
+__device__ __forceinline__ double c2p_residual(double Gg, const State<double> &U, double p) {
+  const double D = U.d, E = U.p;
+  const double m2 = dot3(U.v, U.v);
+  const double alpha = E + p;
+  const double lor = sqrt(1.0 / (1.0 - m2 / (alpha * alpha)));
+  const double h = 1.0 + Gg * p * lor / D;
+  return fabs(D * h * lor - E - p) / alpha;
+}
+
+template <typename R>
+__device__ __forceinline__ void signal_speed(R cs2, R Vn, R V2, R &cmin, R &cmax) {
+  const R Vn2 = Vn * Vn;
+  const R Vt2 = V2 - Vn2;
+  const R sroot = sqrt(cs2 * (R(1) - Vn2 - Vt2 * cs2) * (R(1) - V2));
+  cmax = (Vn * (R(1) - cs2) + sroot) / (R(1) - V2 * cs2);
+  cmin = (Vn * (R(1) - cs2) - sroot) / (R(1) - V2 * cs2);
+}
+
+// HLLC flux through a face with unit normal n (pointing out of the left cell)
+// and area S.  F receives S * flux in the global frame; returns max(lL, lR).
+template <typename R>
+__device__ __forceinline__ R hllc_flux(R G1, R Gg, const State<R> &cUL, const State<R> &cUR, const State<R> &cWL,
+                                       const State<R> &cWR, R n0, R n1, R n2, R S, State<R> &Fout) {
+  const Rot<R> T = rotation(n0, n1, n2);
+  State<R> UL = cUL, UR = cUR, WL = cWL, WR = cWR;
+  rot_fwd(T, cUL.v, UL.v);
+  rot_fwd(T, cUR.v, UR.v);
+  rot_fwd(T, cWL.v, WL.v);
+  rot_fwd(T, cWR.v, WR.v);
+  const R *VL = WL.v, *VR = WR.v;
+  const R dL = WL.d, dR = WR.d, pL = WL.p, pR = WR.p;
+  const R VVL = dot3(VL, VL), VVR = dot3(VR, VR);
+  const R hL = R(1) + Gg * pL / dL;
+  const R hR = R(1) + Gg * pR / dR;
+  const R csL2 = ((G1 * pL) / (dL * hL));
+  const R csR2 = ((G1 * pR) / (dR * hR));
+  R cmaxL, cmaxR, cminL, cminR;
+  signal_speed(csL2, VL[0], VVL, cminL, cmaxL);
+  signal_speed(csR2, VR[0], VVR, cminR, cmaxR);
+  const R lL = smin(cminL, cminR);
+  const R lR = smax(cmaxL, cmaxR);
+  State<R> F;
+  if (lL >= R(0)) {
+    F.d = UL.d * VL[0];
+    F.v[0] = UL.v[0] * VL[0], F.v[1] = UL.v[1] * VL[0], F.v[2] = UL.v[2] * VL[0];
+    F.p = UL.v[0];
+    F.v[0] += pL;
+  } else if (lR <= R(0)) {
+    F.d = UR.d * VR[0];
+    F.v[0] = UR.v[0] * VR[0], F.v[1] = UR.v[1] * VR[0], F.v[2] = UR.v[2] * VR[0];
+    F.p = UR.v[0];
+    F.v[0] += pR;
+  } else {
+    // The star state is taken on the side of the contact speed us; both
+    // sides share one code path with (lam, U, V, p, F) of that side.
+    const R FLd = UL.d * VL[0], FLv0 = UL.v[0] * VL[0], FLv1 = UL.v[1] * VL[0], FLv2 = UL.v[2] * VL[0];
+    const R FLp = UL.v[0];
+    const R FRd = UR.d * VR[0], FRv0 = UR.v[0] * VR[0], FRv1 = UR.v[1] * VR[0], FRv2 = UR.v[2] * VR[0];
+    const R FRp = UR.v[0];
+    const R AL = lL * UL.p - FLp;
+    const R AR = lR * UR.p - FRp;
+    const R BL = lL * UL.v[0] - FLv0 - pL;
+    const R BR = lR * UR.v[0] - FRv0 - pR;
+    const R a = AR * lL - AL * lR;
+    const R b = AL + BL * lR - AR - BR * lL;
+    const R c = BR - BL;
+    const R sgn = b < R(0) ? R(-1) : R(1);  // SIGN, lib/global_def.h:62
+    const R scrh = R(-0.5) * (b + sgn * sqrt(b * b - R(4) * a * c));
+    const R us = c / scrh;
+    const R ps = (AL * us - BL) / (R(1) - us * lL);
+    const bool left = us >= R(0);
+    const R lam = left ? lL : lR;
+    const State<R> &Us = left ? UL : UR;
+    const R Vn = left ? VL[0] : VR[0];
+    const R ph = left ? pL : pR;
+    const R Fd = left ? FLd : FRd, Fv0 = left ? FLv0 : FRv0, Fv1 = left ? FLv1 : FRv1, Fv2 = left ? FLv2 : FRv2;
+    const R Fp = left ? FLp : FRp;
+    const R dif = R(1) / (lam - us);
+    R sd = Us.d * (lam - Vn) * dif;
+    R sv0 = (lam * (Us.p + ps) - Us.v[0]) * us * dif;
+    R sv1 = Us.v[1] * (lam - Vn) * dif;
+    R sv2 = Us.v[2] * (lam - Vn) * dif;
+    R sp = Us.p + (sv0 - Us.v[0]) / lam;
+    sd -= Us.d, sv0 -= Us.v[0], sv1 -= Us.v[1], sv2 -= Us.v[2], sp -= Us.p;
+    sd *= lam, sv0 *= lam, sv1 *= lam, sv2 *= lam, sp *= lam;
+    F.d = Fd + sd;
+    F.v[0] = Fv0 + sv0, F.v[1] = Fv1 + sv1, F.v[2] = Fv2 + sv2;
+    F.p = Fp + sp;
+    F.v[0] += ph;
+  }
+  Fout.d = F.d;
+  Fout.p = F.p;
+  rot_back(T, F.v, Fout.v);
+  Fout.d *= S, Fout.v[0] *= S, Fout.v[1] *= S, Fout.v[2] *= S, Fout.p *= S;
+  return smax(lL, lR);
+}
+
+// Radiation 4-momentum source, solvers/RadRHD/src/radRHD_utils.cpp:8-101 with
+// the M1 Eddington tensor (ISOTROPIC_INTENSITY == 1 branch); the thermal
+// emission term is omitted (q = 0).
+__device__ __forceinline__ double edd_tensor(int i, int j, double U, const double *F) {
+  double flux_module = dot3(F, F);
+  if (flux_module < 1e-150 || U < 1e-150) return i == j ? 0.33333333333333 : 0.0;
+  flux_module = sqrt(flux_module);
+  const double ni = F[i] / flux_module;
+  const double nj = F[j] / flux_module;
+  if (flux_module > U) return ni * nj;
+  double f2 = flux_module / U;
+  f2 = f2 * f2;
+  const double chi_2 = (1.5 + 2.0 * f2) / (5.0 + 2.0 * sqrt(4.0 - 3.0 * f2));
+  double edt = (3.0 * chi_2 - 0.5) * ni * nj;
+  if (i == j) edt += 0.5 - chi_2;
+  return edt;
+}
+
+__device__ __forceinline__ void rad_source(const State<double> &W, double Ur, const double *Fr, double abs_op,
+                                           double scat_op, double *G) {
+  const double tot_op = abs_op + scat_op;
+  const double rho = W.d;
+  double gamma = dot3(W.v, W.v);
+  double u2 = gamma;
+  const double gamma2 = 1.0 / (1.0 - gamma);
+  u2 *= gamma2;
+  gamma = sqrt(gamma2);
+  const double u[3] = {gamma * W.v[0], gamma * W.v[1], gamma * W.v[2]};
+  double uuD = 0.;
+  const double uF = dot3(u, Fr);
+  double uD[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    uD[i] = 0.;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const double D = edd_tensor(j, i, Ur, Fr);
+      uD[i] += u[j] * D;
+      uuD += u[i] * u[j] * D;
+    }
+  }
+  const double rhogamma = rho * gamma;
+  const double q = 0.0;
+  G[0] = q * gamma + rhogamma * (abs_op - scat_op * (u2 + uuD)) * Ur - rho * (abs_op - scat_op * (u2 + gamma2)) * uF;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+    G[i + 1] = q * u[i] - rho * (tot_op * uD[i] + scat_op * u[i] * (gamma2 + uuD)) * Ur +
+               rhogamma * (tot_op * Fr[i] + scat_op * 2.0 * uF * u[i]);
+}
+
+}  // namespace rhllc_dev

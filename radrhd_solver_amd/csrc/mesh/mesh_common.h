@@ -24,7 +24,19 @@ inline uint64_t mix64(uint64_t z) {
 // Uniform double in [0, 1) from the top 53 bits.
 inline double u01(uint64_t r) { return (double)(r >> 11) * (1.0 / 9007199254740992.0); }
 
-bool fail(const std::string &msg);
+inline std::string &err_storage() {
+  static thread_local std::string e;
+  return e;
+}
+inline bool fail(const std::string &msg) {
+  err_storage() = msg;
+  return false;
+}
+
+// Cells sorted by the Morton code of their centres; faces by the new index of
+// their id_l cell, stable in the old face order (counting sort).
+void morton_order_impl(int64_t n_cells, const void *geo_cells, int64_t n_faces, const void *geo_faces,
+                       int32_t *cell_new_to_old, int32_t *face_new_to_old);
 
 inline int num_threads() {
 #ifdef _OPENMP

@@ -1,0 +1,160 @@
+"""GPU parity of the HIP RHLLC step against the oracle (FP64: bitwise).
+
+The oracle is the C restatement of the reference's Hllc3dStab
+(solvers/rhllc/src/rhllc_3d_stab.cpp:10-65), itself pinned bit-for-bit to the
+compiled reference in tests/test_oracle_vs_ref.py and by tests/golden.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from radrhd_solver_amd import capi
+from radrhd_solver_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261202
+
+
+def random_prims(rng, n, vmax=0.999, rho=(-3, 2), p=(-4, 3)):
+    r = 10 ** rng.uniform(*rho, n)
+    pr = 10 ** rng.uniform(*p, n)
+    speed = vmax * rng.uniform(0, 1, n) ** 0.5
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    return np.column_stack([r, d * speed[:, None], pr])
+
+
+def random_faces(rng, n):
+    nn = rng.normal(size=(n, 3))
+    nn /= np.linalg.norm(nn, axis=1)[:, None]
+    nn[:8] = [[0, 0, 1], [0, 0, -1], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]]
+    return np.column_stack([nn, rng.uniform(0.01, 2.0, n)])
+
+
+def test_device_is_gfx950():
+    info = capi.device_query(0)
+    assert info["arch"].startswith("gfx950"), info
+
+
+@pytest.mark.parametrize("gamma", [5.0 / 3.0, 4.0 / 3.0])
+def test_eval_flux_fp64_bitwise(gamma):
+    rng = np.random.default_rng(SEED)
+    n = 20000
+    WL, WR = random_prims(rng, n), random_prims(rng, n)
+    # supersonic left/right-going pairs exercise the lL >= 0 and lR <= 0 branches
+    WL[:500, 1:4] = [0.99, 0, 0]
+    WR[:500, 1:4] = [0.99, 0, 0]
+    WL[500:1000, 1:4] = [-0.99, 0, 0]
+    WR[500:1000, 1:4] = [-0.99, 0, 0]
+    _, UL = O.p2c(WL, gamma)
+    _, UR = O.p2c(WR, gamma)
+    nS = random_faces(rng, n)
+    nS[:1000, :3] = [1, 0, 0]
+    Fo, so = O.flux(UL, UR, WL, WR, nS, gamma)
+    Fg, sg = capi.eval_flux(UL, UR, WL, WR, nS, gamma)
+    assert np.array_equal(Fo, Fg), np.argwhere(Fo != Fg)[:5]
+    assert np.array_equal(so, sg)
+
+
+def test_eval_c2p_p2c_fix_bitwise():
+    rng = np.random.default_rng(SEED + 1)
+    n = 20000
+    W = random_prims(rng, n)
+    Wc_o, U_o = O.p2c(W)
+    Wc_g, U_g = capi.eval_p2c(W)
+    assert np.array_equal(U_o, U_g) and np.array_equal(Wc_o, Wc_g)
+    # perturb energies so some states fail (Q < 0) and go to the fix
+    U = U_o.copy()
+    U[:2000, 4] = np.sqrt(U[:2000, 0] ** 2 + (U[:2000, 1:4] ** 2).sum(1)) * (1 - 1e-3)
+    W0 = np.full((n, 5), -7.0)
+    Wo, co, io = O.c2p(U, W_init=W0)
+    Wg, cg, ig = capi.eval_c2p(U, W_init=W0)
+    assert np.array_equal(co, cg)
+    assert np.array_equal(io, ig)
+    assert np.array_equal(Wo, Wg)
+    assert (co != 0).sum() >= 2000
+    bad = co != 0
+    Uo2, Wo2, fo = O.pressure_fix(U[bad], Wo[bad])
+    Ug2, Wg2, fg = capi.eval_pressure_fix(U[bad], Wg[bad])
+    assert np.array_equal(fo, fg) and np.array_equal(Uo2, Ug2) and np.array_equal(Wo2, Wg2)
+
+
+def _run_both(mesh, W0, cfl, steps, gamma=5.0 / 3.0, graph=True, reorder=True):
+    h = mesh.h_min()
+    st = O.State(mesh, W0, gamma)
+    t_o, taus_o, speeds_o = st.run(cfl, h, steps)
+    s = capi.Solver(mesh, gamma=gamma, cfl=cfl, h_min=h, reorder=reorder)
+    s.upload_W(W0)
+    s.run(steps, graph=graph)
+    s.sync()
+    Wg, Ug = s.download()
+    stats = s.stats()
+    s.close()
+    return st, (t_o, taus_o, speeds_o), (Wg, Ug, stats)
+
+
+def test_step_shock_tube_200_steps_bitwise():
+    """Config 0: relativistic shock tube, 120k jittered/permuted tets, CFL 0.4, 200 steps."""
+    m = M.kuhn_box(200, 10, 10, length=(1.0, 0.05, 0.05), seed=SEED)
+    assert (m.n_cells, m.n_faces) == (120000, 248200)
+    W0 = M.init_state(m, M.INIT_SHOCK_TUBE)
+    st, (t_o, taus, speeds), (Wg, Ug, stats) = _run_both(m, W0, 0.4, 200)
+    assert stats["steps"] == 200
+    assert np.array_equal(st.U, Ug)
+    assert np.array_equal(st.W, Wg)
+    assert stats["max_speed"] == speeds[-1]
+    assert stats["t"] == t_o
+    for k in range(5):
+        ref = np.abs(st.U[:, k]).sum()
+        assert np.abs(st.U[:, k] - Ug[:, k]).sum() <= 1e-12 * ref
+
+
+def test_step_periodic_shear_bitwise_eager_and_graph():
+    """Periodic shear layer (W ~ 10, gamma 4/3) on 12^3 hexes, eager and graph replay agree bitwise."""
+    m = M.kuhn_box(12, 12, 12, origin=(-0.5, -0.5, -0.5), periodic=7, seed=SEED)
+    W0 = M.init_state(m, M.INIT_SHEAR)
+    st, (t_o, _, _), (Wg, Ug, _) = _run_both(m, W0, 0.4, 40, gamma=4.0 / 3.0, graph=False, reorder=False)
+    assert np.array_equal(st.U, Ug) and np.array_equal(st.W, Wg)
+    _, _, (Wg2, Ug2, _) = _run_both(m, W0, 0.4, 40, gamma=4.0 / 3.0, graph=True, reorder=True)
+    assert np.array_equal(Ug, Ug2) and np.array_equal(Wg, Wg2)
+
+
+def test_step_matches_host_tau_interface():
+    """rhllc_step(tau) (the Hllc3dStab drop-in) returns the oracle's max signal speed."""
+    m = M.kuhn_box(16, 4, 4, length=(1.0, 0.25, 0.25), seed=SEED)
+    W0 = M.init_state(m, M.INIT_SHOCK_TUBE)
+    h = m.h_min()
+    st = O.State(m, W0)
+    st.run(0.4, h, 0)
+    s = capi.Solver(m, cfl=0.4, h_min=h)
+    s.upload_W(W0)
+    tau = 0.4 * h
+    for _ in range(10):
+        sp_o = st.step(tau)
+        sp_g = s.step(tau)
+        assert sp_o == sp_g
+        tau = 0.4 * h / sp_o
+    _, Ug = s.download()
+    assert np.array_equal(st.U, Ug)
+    s.close()
+
+
+def test_mixed_precision_blast_l1():
+    """Mixed FP32-flux / FP64-c2p against the FP64 step on a small blast wave."""
+    m = M.kuhn_box(24, 24, 24, seed=SEED)
+    W0 = M.init_state(m, M.INIT_BLAST)
+    h = m.h_min()
+    res = {}
+    for prec, guess in ((capi.PREC_FP64, capi.GUESS_REFERENCE), (capi.PREC_MIXED, capi.GUESS_PREVIOUS)):
+        s = capi.Solver(m, cfl=0.4, h_min=h, precision=prec, guess=guess)
+        s.upload_W(W0)
+        s.run(50)
+        s.sync()
+        res[prec] = s.download()[1]
+        s.close()
+    U64, U32 = res[capi.PREC_FP64], res[capi.PREC_MIXED]
+    V = m.cells["V"]
+    l1 = [float((np.abs(U32[:, k] - U64[:, k]) * V).sum() / (np.abs(U64[:, k]) * V).sum()) for k in range(5)]
+    print("mixed vs fp64 rel L1 per conserved field:", l1)
+    assert max(l1) < 1e-3
