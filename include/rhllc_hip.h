@@ -142,6 +142,12 @@ int rhllc_eval_pressure_fix(double gamma, int64_t n, double *U, double *W, int32
 int rhllc_eval_rad_source(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
                           const double *ks, double *G);
 
+/* Calibration copy: reps launches of a plain vectorised copy kernel over
+ * nbytes (read nbytes + write nbytes per launch); *gbps receives the mean
+ * (read + write) bandwidth.  Used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE
+ * counters and as the measured HBM ceiling.  No counterpart in the reference. */
+int rhllc_copy_probe(int32_t device, int64_t nbytes, int32_t reps, double *gbps);
+
 /* Device name and compute units of the current device (for reports). */
 int rhllc_device_query(int32_t device, char *name, int32_t name_len, int32_t *cus, double *hbm_gb);
 

@@ -84,6 +84,7 @@ SIGNATURES = {
     "rhllc_eval_pressure_fix": (C.c_int, [_dbl, _i64, _vp, _vp, _vp]),
     "rhllc_eval_rad_source": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_device_query": (C.c_int, [_i32, C.c_char_p, _i32, _vp, _vp]),
+    "rhllc_copy_probe": (C.c_int, [_i32, _i64, _i32, _vp]),
     "rhllc_last_error": (C.c_char_p, []),
 }
 
@@ -257,6 +258,13 @@ def eval_rad_source(W, Ur, Fr, ka, ks):
     check(_lib().rhllc_eval_rad_source(n, W.ctypes.data, Ur.ctypes.data, Fr.ctypes.data, ka.ctypes.data,
                                        ks.ctypes.data, G.ctypes.data))
     return G
+
+
+def copy_probe(nbytes: int = 1 << 30, reps: int = 20, device: int = 0) -> float:
+    """Read+write GB/s of a plain copy kernel over nbytes."""
+    g = C.c_double()
+    check(_lib().rhllc_copy_probe(device, nbytes, reps, C.byref(g)))
+    return g.value
 
 
 def device_query(device: int = 0):

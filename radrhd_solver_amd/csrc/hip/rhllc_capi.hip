@@ -646,6 +646,31 @@ extern "C" int rhllc_eval_rad_source(int64_t n, const double *W, const double *U
   return RHLLC_OK;
 }
 
+extern "C" int rhllc_copy_probe(int32_t device, int64_t nbytes, int32_t reps, double *gbps) {
+  if (nbytes < 16 || reps < 1 || !gbps) return fail(RHLLC_ERR_ARG, "bad argument");
+  HIP_TRY(hipSetDevice(device));
+  const int64_t n16 = nbytes / 16;
+  DevBuf a, b;
+  HIP_TRY(to_dev(a, nullptr, (size_t)n16 * 16));
+  HIP_TRY(to_dev(b, nullptr, (size_t)n16 * 16));
+  HIP_TRY(hipMemset(a.p, 0, (size_t)n16 * 16));
+  HIP_TRY(launch_copy(a.p, b.p, n16, nullptr));
+  HIP_TRY(hipDeviceSynchronize());
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, nullptr));
+  for (int32_t r = 0; r < reps; ++r) HIP_TRY(launch_copy(a.p, b.p, n16, nullptr));
+  HIP_TRY(hipEventRecord(e1, nullptr));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *gbps = 2.0 * (double)n16 * 16.0 * reps / (ms * 1e-3) / 1e9;
+  return RHLLC_OK;
+}
+
 extern "C" int rhllc_device_query(int32_t device, char *name, int32_t name_len, int32_t *cus, double *hbm_gb) {
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, device));

@@ -72,6 +72,8 @@ hipError_t launch_cell(const Dev &d, int variant, bool rad, bool stats, hipStrea
 hipError_t launch_dt(const Dev &d, hipStream_t s);
 hipError_t launch_clear_scalars(const Dev &d, hipStream_t s);
 
+hipError_t launch_copy(const void *src, void *dst, int64_t n16, hipStream_t s);
+
 // Pointwise evaluations (n states) for parity tests.
 hipError_t launch_eval_flux(bool fp32, double gamma, int64_t n, const double *UL, const double *UR, const double *WL,
                             const double *WR, const double *nS4, double *F, double *speed, hipStream_t s);

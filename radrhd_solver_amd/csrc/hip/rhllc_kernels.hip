@@ -432,6 +432,17 @@ __global__ void k_eval_rad(int64_t n, const double *W, const double *Ur, const d
   rad_source(w, Ur[i], Fr3 + 3 * i, ka[i], ks[i], G + 4 * i);
 }
 
+// Plain copy, 16 B per thread per iteration (calibration of byte counters).
+__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+hipError_t launch_copy(const void *src, void *dst, int64_t n16, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, s, (const float4 *)src, (float4 *)dst, n16);
+  return hipGetLastError();
+}
+
 static inline dim3 eval_grid(int64_t n) { return dim3((unsigned int)((n + 255) / 256)); }
 
 hipError_t launch_eval_flux(bool fp32, double gamma, int64_t n, const double *UL, const double *UR, const double *WL,

@@ -158,3 +158,21 @@ def test_mixed_precision_blast_l1():
     l1 = [float((np.abs(U32[:, k] - U64[:, k]) * V).sum() / (np.abs(U64[:, k]) * V).sum()) for k in range(5)]
     print("mixed vs fp64 rel L1 per conserved field:", l1)
     assert max(l1) < 1e-3
+
+
+def test_step_radiation_source_bitwise():
+    """Full-form coupling: hydro step, U += tau*G(W) (M1 closure), c2p -- as radRHD_main.cpp:99-132."""
+    m = M.kuhn_box(16, 16, 16, seed=SEED)
+    W0 = M.init_state(m, M.INIT_BLAST)
+    Ur, Fr, ka, ks = M.init_radiation(m.n_cells, seed=SEED)
+    h = m.h_min()
+    st = O.State(m, W0)
+    st.run_rad(0.4, h, 30, Ur, Fr, ka, ks)
+    s = capi.Solver(m, cfl=0.4, h_min=h)
+    s.upload_W(W0)
+    s.set_radiation(Ur, Fr, ka, ks)
+    s.run(30)
+    s.sync()
+    W, U = s.download()
+    s.close()
+    assert np.array_equal(st.U, U) and np.array_equal(st.W, W)
