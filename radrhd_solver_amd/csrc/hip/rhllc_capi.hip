@@ -325,22 +325,18 @@ extern "C" int rhllc_download_state(rhllc_solver *h, void *recs, int64_t stride,
   const Dev &d = h->d;
   uint8_t *base = static_cast<uint8_t *>(recs);
   if (off_phys >= 0) {
+    if (h->mixed) {
+      // The mixed step keeps W in FP32; rebuild FP64 W from the FP64 U.
+      HIP_TRY(launch_recover_W(d, h->stream));
+      HIP_TRY(hipStreamSynchronize(h->stream));
+    }
     std::vector<double> wp(N);
+    std::vector<double4> w4(N);
     HIP_TRY(hipMemcpy(wp.data(), d.Wp, sizeof(double) * N, hipMemcpyDeviceToHost));
-    if (!h->mixed) {
-      std::vector<double4> w4(N);
-      HIP_TRY(hipMemcpy(w4.data(), d.W4, sizeof(double4) * N, hipMemcpyDeviceToHost));
-      for (int32_t i = 0; i < N; ++i) {
-        const double w[5] = {w4[i].x, w4[i].y, w4[i].z, w4[i].w, wp[i]};
-        std::memcpy(base + (size_t)h->cell_new_to_old[i] * stride + off_phys, w, 40);
-      }
-    } else {
-      std::vector<float4> w4(N);
-      HIP_TRY(hipMemcpy(w4.data(), d.W4f, sizeof(float4) * N, hipMemcpyDeviceToHost));
-      for (int32_t i = 0; i < N; ++i) {
-        const double w[5] = {w4[i].x, w4[i].y, w4[i].z, w4[i].w, wp[i]};
-        std::memcpy(base + (size_t)h->cell_new_to_old[i] * stride + off_phys, w, 40);
-      }
+    HIP_TRY(hipMemcpy(w4.data(), d.W4, sizeof(double4) * N, hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < N; ++i) {
+      const double w[5] = {w4[i].x, w4[i].y, w4[i].z, w4[i].w, wp[i]};
+      std::memcpy(base + (size_t)h->cell_new_to_old[i] * stride + off_phys, w, 40);
     }
   }
   if (off_conv >= 0) {
@@ -544,7 +540,7 @@ extern "C" int rhllc_info(rhllc_solver *h, int64_t *n_cells, int64_t *n_faces, i
     const int64_t N = h->n_cells, F = h->n_faces;
     // Algorithmic bytes of one step (both kernels), each array element counted
     // once per kernel that touches it.
-    *device_bytes = h->mixed ? 161 * N + 64 * F : 233 * N + 120 * F;
+    *device_bytes = h->mixed ? 148 * N + 64 * F : 233 * N + 120 * F;
     if (h->rad) *device_bytes += 24 * N;
   }
   return RHLLC_OK;

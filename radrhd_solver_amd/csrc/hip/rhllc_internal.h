@@ -37,8 +37,8 @@ struct Dev {
   const float4 *face_nS32;  // (nx, ny, nz, S), FP32 path
   double4 *F4;              // flux (d, mx, my, mz) * S, FP64 path
   double *FE;               // flux E * S
-  float4 *F4f;              // FP32 path
-  float *FEf;
+  float4 *F4f;              // mixed path: flux (d, mx, my, mz) * S rounded to FP32
+  float *FEf;               // mixed path: flux E * S
   // cells
   const int4 *cell_faces;  // face index | 0x80000000 when the cell is id_r (sign bit 0)
   const double *cell_V;
@@ -71,6 +71,8 @@ hipError_t launch_face(const Dev &d, bool mixed, hipStream_t s);
 hipError_t launch_cell(const Dev &d, int variant, bool rad, bool stats, hipStream_t s);
 hipError_t launch_dt(const Dev &d, hipStream_t s);
 hipError_t launch_clear_scalars(const Dev &d, hipStream_t s);
+// Mixed path: W (FP64) recovered from U by c2p (start from the FP32 pressure).
+hipError_t launch_recover_W(const Dev &d, hipStream_t s);
 
 hipError_t launch_copy(const void *src, void *dst, int64_t n16, hipStream_t s);
 

@@ -92,23 +92,27 @@ __global__ __launch_bounds__(kBlock) void k_face(Dev d) {
       d.F4[i] = make_double4(F.d, F.v[0], F.v[1], F.v[2]);
       d.FE[i] = F.p;
     } else {
+      // FP32 storage (state, geometry, flux), FP64 arithmetic with the fast
+      // divide/sqrt policy; U is rebuilt from the FP32 primitives.
       const float4 nS = d.face_nS32[i];
-      const float G1 = (float)d.gamma, Gg = (float)d.Gg;
-      State<float> UL, WL, UR, WR, F;
-      load_W32(d, lr.x, WL);
-      if (lr.y >= 0) {
-        load_W32(d, lr.y, WR);
-      } else {
-        WR.d = (float)d.ghostW[0], WR.v[0] = (float)d.ghostW[1], WR.v[1] = (float)d.ghostW[2];
-        WR.v[2] = (float)d.ghostW[3], WR.p = (float)d.ghostW[4];
+      State<double> UL, WL, UR, WR, F;
+      {
+        const float4 a = d.W4f[lr.x];
+        WL.d = a.x, WL.v[0] = a.y, WL.v[1] = a.z, WL.v[2] = a.w, WL.p = d.Wpf[lr.x];
       }
-      // U is rebuilt from the FP32 primitives instead of being read.
-      p2c<float>(Gg, WL, UL);
-      p2c<float>(Gg, WR, UR);
-      const float s = hllc_flux<float>(G1, Gg, UL, UR, WL, WR, nS.x, nS.y, nS.z, nS.w, F);
-      speed = smax(speed, (double)s);
-      d.F4f[i] = make_float4(F.d, F.v[0], F.v[1], F.v[2]);
-      d.FEf[i] = F.p;
+      if (lr.y >= 0) {
+        const float4 a = d.W4f[lr.y];
+        WR.d = a.x, WR.v[0] = a.y, WR.v[1] = a.z, WR.v[2] = a.w, WR.p = d.Wpf[lr.y];
+      } else {
+        WR.d = d.ghostW[0], WR.v[0] = d.ghostW[1], WR.v[1] = d.ghostW[2], WR.v[2] = d.ghostW[3], WR.p = d.ghostW[4];
+      }
+      p2c<double, Fast>(d.Gg, WL, UL);
+      p2c<double, Fast>(d.Gg, WR, UR);
+      const double s = hllc_flux<double, Fast>(d.gamma, d.Gg, UL, UR, WL, WR, (double)nS.x, (double)nS.y,
+                                               (double)nS.z, (double)nS.w, F);
+      speed = smax(speed, s);
+      d.F4f[i] = make_float4((float)F.d, (float)F.v[0], (float)F.v[1], (float)F.v[2]);
+      d.FEf[i] = (float)F.p;
     }
   }
   fold_speed(d, speed);
@@ -160,14 +164,17 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
     State<double> U, W;
     load_U(d, c, U);
     U.d -= s.d, U.v[0] -= s.v[0], U.v[1] -= s.v[1], U.v[2] -= s.v[2], U.p -= s.p;
-    // The reference start ignores the previous W; the other variants read only
-    // the previous FP64 pressure.  c2p writes W only when it succeeds.
-    const double guess = (VARIANT == kFp64Ref) ? -1.0 : d.Wp[c];
-    code = c2p(d.Gg, U, W, guess, iters);
+    // The reference start ignores the previous W; the FP64 previous-pressure
+    // variant reads the FP64 pressure, the mixed path the FP32 one.  c2p writes
+    // W only when it succeeds.
+    double guess = -1.0;
+    if (VARIANT == kFp64Prev) guess = d.Wp[c];
+    if (MIXED) guess = (double)d.Wpf[c];
+    code = MIXED ? c2p_fast(d.Gg, U, W, guess, iters) : c2p(d.Gg, U, W, guess, iters);
     if (code && VARIANT != kFp64Ref) {
       int it2 = 0;
       fell_back = true;
-      code = c2p(d.Gg, U, W, -1.0, it2);
+      code = MIXED ? c2p_fast(d.Gg, U, W, -1.0, it2) : c2p(d.Gg, U, W, -1.0, it2);
       iters += it2;
     }
     if (code) {
@@ -176,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
         load_W(d, c, W);
       } else {
         const float4 a = d.W4f[c];
-        W.d = a.x, W.v[0] = a.y, W.v[1] = a.z, W.v[2] = a.w, W.p = d.Wp[c];
+        W.d = a.x, W.v[0] = a.y, W.v[1] = a.z, W.v[2] = a.w, W.p = d.Wpf[c];
       }
       fixed = true;
       fix_failed = pressure_fix(d.Gg, U, W) != 0;
@@ -194,15 +201,18 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
       U.v[1] += ds * tau * G[2];
       U.v[2] += ds * tau * G[3];
       int it3 = 0;
-      c2p(d.Gg, U, W, MIXED ? W.p : -1.0, it3);
+      if (MIXED)
+        c2p_fast(d.Gg, U, W, W.p, it3);
+      else
+        c2p(d.Gg, U, W, -1.0, it3);
       iters += it3;
     }
     if (STATS) resid = c2p_residual(d.Gg, U, W.p);
     d.U4[c] = make_double4(U.d, U.v[0], U.v[1], U.v[2]);
     d.UE[c] = U.p;
-    d.Wp[c] = W.p;
     if (!MIXED) {
       d.W4[c] = make_double4(W.d, W.v[0], W.v[1], W.v[2]);
+      d.Wp[c] = W.p;
     } else {
       d.W4f[c] = make_float4((float)W.d, (float)W.v[0], (float)W.v[1], (float)W.v[2]);
       d.Wpf[c] = (float)W.p;
@@ -280,6 +290,20 @@ __global__ __launch_bounds__(kBlock) void k_p2c_init(Dev d) {
   d.Wpf[c] = (float)W.p;
 }
 
+// Mixed path: FP64 primitives from the FP64 conserved state (for download).
+__global__ __launch_bounds__(kBlock) void k_recover_W(Dev d) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= d.n_cells) return;
+  State<double> U, W;
+  load_U(d, c, U);
+  const float4 a = d.W4f[c];
+  W.d = a.x, W.v[0] = a.y, W.v[1] = a.z, W.v[2] = a.w, W.p = d.Wpf[c];
+  int it = 0;
+  c2p(d.Gg, U, W, W.p, it);  // W keeps the FP32 values if c2p fails
+  d.W4[c] = make_double4(W.d, W.v[0], W.v[1], W.v[2]);
+  d.Wp[c] = W.p;
+}
+
 // Clears the counters and speed slots with plain stores (graph-safe).
 __global__ void k_clear(Dev d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -350,6 +374,12 @@ hipError_t launch_dt(const Dev &d, hipStream_t s) {
   return hipGetLastError();
 }
 
+hipError_t launch_recover_W(const Dev &d, hipStream_t s) {
+  if (d.n_cells == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_recover_W, dim3(blocks_for(d.n_cells)), dim3(kBlock), 0, s, d);
+  return hipGetLastError();
+}
+
 hipError_t launch_clear_scalars(const Dev &d, hipStream_t s) {
   hipLaunchKernelGGL(k_clear, dim3((kSpeedSlots + 255) / 256), dim3(256), 0, s, d);
   return hipGetLastError();
@@ -376,16 +406,17 @@ __global__ void k_eval_flux(double gamma, int64_t n, const double *UL, const dou
                                  nS4[4 * i + 3], f);
     st5(F + 5 * i, f);
   } else {
-    // The mixed face path: U rebuilt from FP32 W.
-    State<float> a, b, ua, ub, ff;
+    // The mixed face path: FP32 primitives and geometry, FP64 fast arithmetic,
+    // flux rounded to FP32.
+    State<double> a, b, ua, ub, ff;
     a.d = (float)wl.d, a.v[0] = (float)wl.v[0], a.v[1] = (float)wl.v[1], a.v[2] = (float)wl.v[2], a.p = (float)wl.p;
     b.d = (float)wr.d, b.v[0] = (float)wr.v[0], b.v[1] = (float)wr.v[1], b.v[2] = (float)wr.v[2], b.p = (float)wr.p;
-    p2c<float>((float)Gg, a, ua);
-    p2c<float>((float)Gg, b, ub);
-    speed[i] = hllc_flux<float>((float)gamma, (float)Gg, ua, ub, a, b, (float)nS4[4 * i], (float)nS4[4 * i + 1],
-                                (float)nS4[4 * i + 2], (float)nS4[4 * i + 3], ff);
+    p2c<double, Fast>(Gg, a, ua);
+    p2c<double, Fast>(Gg, b, ub);
+    speed[i] = hllc_flux<double, Fast>(gamma, Gg, ua, ub, a, b, (double)(float)nS4[4 * i], (double)(float)nS4[4 * i + 1],
+                                       (double)(float)nS4[4 * i + 2], (double)(float)nS4[4 * i + 3], ff);
     double *o = F + 5 * i;
-    o[0] = ff.d, o[1] = ff.v[0], o[2] = ff.v[1], o[3] = ff.v[2], o[4] = ff.p;
+    o[0] = (float)ff.d, o[1] = (float)ff.v[0], o[2] = (float)ff.v[1], o[3] = (float)ff.v[2], o[4] = (float)ff.p;
   }
 }
 

@@ -44,12 +44,45 @@ __device__ __forceinline__ R dot3(const R *a, const R *b) {
   return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
 }
 
+// Arithmetic policies.  Ieee: correctly rounded divide and square root (the
+// parity path; every expression keeps the reference's form).  Fast: hardware
+// reciprocal / reciprocal square root refined by two Newton steps -- a few
+// ulp in FP64, far below the FP32 storage rounding of the mixed path.
+struct Ieee {
+  __device__ static __forceinline__ double div(double a, double b) { return a / b; }
+  __device__ static __forceinline__ double rcp(double x) { return 1.0 / x; }
+  __device__ static __forceinline__ double sqrt_(double x) { return sqrt(x); }
+};
+struct Fast {
+  __device__ static __forceinline__ double rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+  }
+  __device__ static __forceinline__ double div(double a, double b) {
+    const double r = rcp(b);
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);  // one correction of the quotient
+  }
+  __device__ static __forceinline__ double sqrt_(double x) {
+    // y ~ 1/sqrt(x), two Newton steps, then s = x*y with one correction.
+    if (!(x > 0.0)) return sqrt(x);  // zero, negative, NaN: the IEEE result
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-0.5 * x * y, y, 1.5);
+    y = y * fma(-0.5 * x * y, y, 1.5);
+    const double s = x * y;
+    return fma(fma(-s, s, x), 0.5 * y, s);
+  }
+};
+
 template <typename R>
 struct Rot {
   R t[9];  // row-major, rows (n, t1, t2)
 };
 
-template <typename R>
+template <typename R, typename A = Ieee>
 __device__ __forceinline__ Rot<R> rotation(R n0, R n1, R n2) {
   Rot<R> T;
 #pragma unroll
@@ -59,8 +92,8 @@ __device__ __forceinline__ Rot<R> rotation(R n0, R n1, R n2) {
     T.t[0] = n0;
     T.t[1] = n1;
     T.t[2] = n2;
-    const R sqr = sqrt(R(1) - n2 * n2);
-    const R sqr_inv = R(1) / sqr;
+    const R sqr = A::sqrt_(R(1) - n2 * n2);
+    const R sqr_inv = A::rcp(sqr);
     T.t[3] = -n1 * sqr_inv;
     T.t[4] = n0 * sqr_inv;
     T.t[6] = -n0 * n2 * sqr_inv;
@@ -96,19 +129,19 @@ __device__ __forceinline__ void rot_back(const Rot<R> &T, const R *v, R *r) {
 }
 
 // Conserved from primitive; clamps |v| >= 1 to 0.9999 and rewrites W.v as the reference does.
-template <typename R>
+template <typename R, typename A = Ieee>
 __device__ __forceinline__ void p2c(R Gg, State<R> &W, State<R> &U) {
-  const R h = R(1) + Gg * W.p / W.d;
+  const R h = R(1) + A::div(Gg * W.p, W.d);
   R g = dot3(W.v, W.v);
   if (g >= R(1)) {
     const R beta_fix = R(0.9999);
-    g = beta_fix / sqrt(g);
+    g = A::div(beta_fix, A::sqrt_(g));
     W.v[0] *= g, W.v[1] *= g, W.v[2] *= g;
     g = beta_fix * beta_fix;
   }
-  g = R(1) / (R(1) - g);
+  g = A::rcp(R(1) - g);
   const R scrh = W.d * h * g;
-  g = sqrt(g);
+  g = A::sqrt_(g);
   U.d = W.d * g;
   U.v[0] = scrh * W.v[0], U.v[1] = scrh * W.v[1], U.v[2] = scrh * W.v[2];
   U.p = scrh - W.p;
@@ -226,21 +259,21 @@ __device__ __forceinline__ double c2p_residual(double Gg, const State<double> &U
   return fabs(D * h * lor - E - p) / alpha;
 }
 
-template <typename R>
+template <typename R, typename A = Ieee>
 __device__ __forceinline__ void signal_speed(R cs2, R Vn, R V2, R &cmin, R &cmax) {
   const R Vn2 = Vn * Vn;
   const R Vt2 = V2 - Vn2;
-  const R sroot = sqrt(cs2 * (R(1) - Vn2 - Vt2 * cs2) * (R(1) - V2));
-  cmax = (Vn * (R(1) - cs2) + sroot) / (R(1) - V2 * cs2);
-  cmin = (Vn * (R(1) - cs2) - sroot) / (R(1) - V2 * cs2);
+  const R sroot = A::sqrt_(cs2 * (R(1) - Vn2 - Vt2 * cs2) * (R(1) - V2));
+  cmax = A::div(Vn * (R(1) - cs2) + sroot, R(1) - V2 * cs2);
+  cmin = A::div(Vn * (R(1) - cs2) - sroot, R(1) - V2 * cs2);
 }
 
 // HLLC flux through a face with unit normal n (pointing out of the left cell)
 // and area S.  F receives S * flux in the global frame; returns max(lL, lR).
-template <typename R>
+template <typename R, typename A = Ieee>
 __device__ __forceinline__ R hllc_flux(R G1, R Gg, const State<R> &cUL, const State<R> &cUR, const State<R> &cWL,
                                        const State<R> &cWR, R n0, R n1, R n2, R S, State<R> &Fout) {
-  const Rot<R> T = rotation(n0, n1, n2);
+  const Rot<R> T = rotation<R, A>(n0, n1, n2);
   State<R> UL = cUL, UR = cUR, WL = cWL, WR = cWR;
   rot_fwd(T, cUL.v, UL.v);
   rot_fwd(T, cUR.v, UR.v);
@@ -249,13 +282,13 @@ __device__ __forceinline__ R hllc_flux(R G1, R Gg, const State<R> &cUL, const St
   const R *VL = WL.v, *VR = WR.v;
   const R dL = WL.d, dR = WR.d, pL = WL.p, pR = WR.p;
   const R VVL = dot3(VL, VL), VVR = dot3(VR, VR);
-  const R hL = R(1) + Gg * pL / dL;
-  const R hR = R(1) + Gg * pR / dR;
-  const R csL2 = ((G1 * pL) / (dL * hL));
-  const R csR2 = ((G1 * pR) / (dR * hR));
+  const R hL = R(1) + A::div(Gg * pL, dL);
+  const R hR = R(1) + A::div(Gg * pR, dR);
+  const R csL2 = A::div(G1 * pL, dL * hL);
+  const R csR2 = A::div(G1 * pR, dR * hR);
   R cmaxL, cmaxR, cminL, cminR;
-  signal_speed(csL2, VL[0], VVL, cminL, cmaxL);
-  signal_speed(csR2, VR[0], VVR, cminR, cmaxR);
+  signal_speed<R, A>(csL2, VL[0], VVL, cminL, cmaxL);
+  signal_speed<R, A>(csR2, VR[0], VVR, cminR, cmaxR);
   const R lL = smin(cminL, cminR);
   const R lR = smax(cmaxL, cmaxR);
   State<R> F;
@@ -284,9 +317,9 @@ __device__ __forceinline__ R hllc_flux(R G1, R Gg, const State<R> &cUL, const St
     const R b = AL + BL * lR - AR - BR * lL;
     const R c = BR - BL;
     const R sgn = b < R(0) ? R(-1) : R(1);  // SIGN, lib/global_def.h:62
-    const R scrh = R(-0.5) * (b + sgn * sqrt(b * b - R(4) * a * c));
-    const R us = c / scrh;
-    const R ps = (AL * us - BL) / (R(1) - us * lL);
+    const R scrh = R(-0.5) * (b + sgn * A::sqrt_(b * b - R(4) * a * c));
+    const R us = A::div(c, scrh);
+    const R ps = A::div(AL * us - BL, R(1) - us * lL);
     const bool left = us >= R(0);
     const R lam = left ? lL : lR;
     const State<R> &Us = left ? UL : UR;
@@ -294,12 +327,12 @@ __device__ __forceinline__ R hllc_flux(R G1, R Gg, const State<R> &cUL, const St
     const R ph = left ? pL : pR;
     const R Fd = left ? FLd : FRd, Fv0 = left ? FLv0 : FRv0, Fv1 = left ? FLv1 : FRv1, Fv2 = left ? FLv2 : FRv2;
     const R Fp = left ? FLp : FRp;
-    const R dif = R(1) / (lam - us);
+    const R dif = A::rcp(lam - us);
     R sd = Us.d * (lam - Vn) * dif;
     R sv0 = (lam * (Us.p + ps) - Us.v[0]) * us * dif;
     R sv1 = Us.v[1] * (lam - Vn) * dif;
     R sv2 = Us.v[2] * (lam - Vn) * dif;
-    R sp = Us.p + (sv0 - Us.v[0]) / lam;
+    R sp = Us.p + A::div(sv0 - Us.v[0], lam);
     sd -= Us.d, sv0 -= Us.v[0], sv1 -= Us.v[1], sv2 -= Us.v[2], sp -= Us.p;
     sd *= lam, sv0 *= lam, sv1 *= lam, sv2 *= lam, sp *= lam;
     F.d = Fd + sd;
@@ -316,20 +349,40 @@ __device__ __forceinline__ R hllc_flux(R G1, R Gg, const State<R> &cUL, const St
 
 // Radiation 4-momentum source, solvers/RadRHD/src/radRHD_utils.cpp:8-101 with
 // the M1 Eddington tensor (ISOTROPIC_INTENSITY == 1 branch); the thermal
-// emission term is omitted (q = 0).
-__device__ __forceinline__ double edd_tensor(int i, int j, double U, const double *F) {
-  double flux_module = dot3(F, F);
-  if (flux_module < 1e-150 || U < 1e-150) return i == j ? 0.33333333333333 : 0.0;
-  flux_module = sqrt(flux_module);
-  const double ni = F[i] / flux_module;
-  const double nj = F[j] / flux_module;
-  if (flux_module > U) return ni * nj;
-  double f2 = flux_module / U;
+// emission term is omitted (q = 0).  EddTensor(a, b) is evaluated once per
+// component from shared subexpressions; every component keeps the reference's
+// operation order, so the values are bitwise those of nine separate calls.
+__device__ __forceinline__ void edd_matrix(double U, const double *F, double D[3][3]) {
+  double fm = dot3(F, F);
+  if (fm < 1e-150 || U < 1e-150) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) D[a][b] = a == b ? 0.33333333333333 : 0.0;
+    return;
+  }
+  fm = sqrt(fm);
+  const double n[3] = {F[0] / fm, F[1] / fm, F[2] / fm};
+  if (fm > U) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) D[a][b] = n[a] * n[b];
+    return;
+  }
+  double f2 = fm / U;
   f2 = f2 * f2;
   const double chi_2 = (1.5 + 2.0 * f2) / (5.0 + 2.0 * sqrt(4.0 - 3.0 * f2));
-  double edt = (3.0 * chi_2 - 0.5) * ni * nj;
-  if (i == j) edt += 0.5 - chi_2;
-  return edt;
+  const double c = 3.0 * chi_2 - 0.5;
+  const double dg = 0.5 - chi_2;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      double e = c * n[a] * n[b];
+      if (a == b) e += dg;
+      D[a][b] = e;
+    }
 }
 
 __device__ __forceinline__ void rad_source(const State<double> &W, double Ur, const double *Fr, double abs_op,
@@ -342,6 +395,8 @@ __device__ __forceinline__ void rad_source(const State<double> &W, double Ur, co
   u2 *= gamma2;
   gamma = sqrt(gamma2);
   const double u[3] = {gamma * W.v[0], gamma * W.v[1], gamma * W.v[2]};
+  double Dm[3][3];
+  edd_matrix(Ur, Fr, Dm);
   double uuD = 0.;
   const double uF = dot3(u, Fr);
   double uD[3];
@@ -350,7 +405,7 @@ __device__ __forceinline__ void rad_source(const State<double> &W, double Ur, co
     uD[i] = 0.;
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-      const double D = edd_tensor(j, i, Ur, Fr);
+      const double D = Dm[j][i];  // EddTensor(j, i, Ur, Fr)
       uD[i] += u[j] * D;
       uuD += u[i] * u[j] * D;
     }
@@ -362,6 +417,58 @@ __device__ __forceinline__ void rad_source(const State<double> &W, double Ur, co
   for (int i = 0; i < 3; i++)
     G[i + 1] = q * u[i] - rho * (tot_op * uD[i] + scat_op * u[i] * (gamma2 + uuD)) * Ur +
                rhogamma * (tot_op * Fr[i] + scat_op * 2.0 * uF * u[i]);
+}
+
+// c2p of the mixed path: the reference's Newton iteration on
+// f(p) = D h Gamma - E - p (same start rule, same stopping test |dp| < 1e-11 E,
+// same failure codes) with the per-iteration algebra rearranged to two
+// divisions instead of four:  with q = alpha^2 - m^2 and r = 1/(alpha q),
+// Gamma^2 = alpha^3 r and m^2 Gamma^3 / alpha^3 = m^2 Gamma r.
+__device__ __forceinline__ int c2p_fast(double Gg, const State<double> &U, State<double> &W, double guess,
+                                        int &iters) {
+  using A = Fast;
+  const double tol = 1.e-11;
+  const double inv_1_eps2 = 1.0 / (1.0 - 1.e-12);
+  const double D = U.d;
+  const double E = U.p;
+  const double m2 = dot3(U.v, U.v);
+  iters = 0;
+  if (E - A::sqrt_(m2 + D * D) < 0.0) return 1;
+  const double m = A::sqrt_(m2);
+  const double D_1 = A::rcp(D);
+  const double pmin = A::sqrt_(m2 * inv_1_eps2) - E;
+  double p = guess >= 0.0 ? guess : smax(m - E, 0.0);
+  p = smax(p, pmin);
+  double yp = 0.0, lor = 1.0;
+  int iter;
+  for (iter = 0; iter < kMaxIter; iter++) {
+    const double alpha = E + p;
+    const double a2 = alpha * alpha;
+    const double r = A::rcp(alpha * (a2 - m2));
+    const double lor2 = a2 * alpha * r;
+    if (!(lor2 >= 1.0)) {
+      iters = iter + 1;
+      return 5;
+    }
+    lor = A::sqrt_(lor2);
+    const double tau = lor * D_1;
+    const double h = 1.0 + Gg * (p * tau);
+    yp = D * h * lor - E - p;
+    const double dyp = D * lor * (Gg * tau) - m2 * lor * r * (lor * (Gg * p) + D * h) - 1.0;
+    const double dp = A::div(yp, dyp);
+    p -= dp;
+    if (p < pmin) p = pmin;
+    if (fabs(dp) < tol * E) break;
+  }
+  iters = iter < kMaxIter ? iter + 1 : kMaxIter;
+  if (p < 0.0) return 2;
+  if (isnan(p)) return 4;
+  if (iter >= kMaxIter || fabs(yp / (E + p)) > 1.e-4 || p < (m - E)) return 3;
+  const double ia = A::rcp(E + p);
+  W.d = D * A::sqrt_(1.0 - m2 * ia * ia);
+  W.p = p;
+  W.v[0] = U.v[0] * ia, W.v[1] = U.v[1] * ia, W.v[2] = U.v[2] * ia;
+  return 0;
 }
 
 }  // namespace rhllc_dev

@@ -172,7 +172,9 @@ def read_records(path: str, dtype) -> np.ndarray:
         data = f.read(n * dtype.itemsize)
     if len(data) != n * dtype.itemsize:
         raise IOError(f"short read in {path}")
-    return np.frombuffer(data, dtype=dtype).copy()
+    # A writable view of the raw bytes (a structured copy would not keep the
+    # records' padding bytes).
+    return np.frombuffer(bytearray(data), dtype=dtype)
 
 
 def write_hllc_cfg(path: str, T: float, CFL: float, h_min: float, save_timer: float, tau: float = 0.0) -> None:

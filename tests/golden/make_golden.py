@@ -1,0 +1,137 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures of the RHLLC step from the reference's own code.
+
+Run in a container where the reference tree exists and `make -C oracle ref`
+has built oracle/_ref/libref_rhllc.so (the reference's hot-path sources,
+EOS IDEAL, gamma 5/3, strict IEEE).  Inputs are seeded; outputs are what the
+compiled reference returns.  The fixtures are data only (inputs + outputs):
+
+  c2p.npz        GetPhysValueStab        U -> W, return code
+  p2c.npz        GetConvValueStab        W -> (W clamped, U)
+  fix.npz        PhysPressureFix         (U, W) -> (U with E rewritten, W, code)
+  rotation.npz   GetRotationMatrix       n -> T
+  flux.npz       GetFluxStab             (UL, UR, WL, WR, n, S) -> (F, max(lL, lR)), all 4 branches
+  traj_tiny.npz  Hllc3dStab loop         8x2x2-hex jittered/permuted mesh, 200 steps
+  shock_tube_200.json  config 0 (200x10x10 hexes, 120k tets) after 200 steps: per-field sums and hashes
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle as O  # noqa: E402
+from radrhd_solver_amd import mesh as M  # noqa: E402
+
+SEED = 20261202
+
+
+def prims(rng, n, rho=(-4, 2), pr=(-6, 2), wmax=50.0):
+    """Config-3 sampling box: log-uniform rho, p/rho, Lorentz factor W in [1, wmax], isotropic v."""
+    r = 10 ** rng.uniform(*rho, n)
+    p = r * 10 ** rng.uniform(*pr, n)
+    W = np.exp(rng.uniform(0, np.log(wmax), n))
+    v = np.sqrt(1 - 1 / W**2)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    return np.column_stack([r, d * v[:, None], p])
+
+
+def unit(rng, n):
+    x = rng.normal(size=(n, 3))
+    return x / np.linalg.norm(x, axis=1)[:, None]
+
+
+def main():
+    if not O.have_ref():
+        sys.exit("oracle/_ref/libref_rhllc.so is not built (make -C oracle ref)")
+    rng = np.random.default_rng(SEED)
+    out = {}
+
+    # p2c, with some |v| >= 1 inputs (clamped to 0.9999 by the reference)
+    W = prims(rng, 500)
+    W[:25, 1:4] *= 1.0 / np.linalg.norm(W[:25, 1:4], axis=1)[:, None] * 1.0001
+    Wc, U = O.ref_p2c(W)
+    np.savez(os.path.join(HERE, "p2c.npz"), W_in=W, W_out=Wc, U=U)
+
+    # c2p: valid states plus states with E lowered below sqrt(m^2 + D^2) (code 1)
+    W = prims(rng, 1500)
+    _, U = O.ref_p2c(W)
+    U[:100, 4] = np.sqrt(U[:100, 0] ** 2 + (U[:100, 1:4] ** 2).sum(1)) * (1 - 1e-6)
+    W0 = np.full_like(U, -3.0)
+    Wr, codes = O.ref_c2p(U, W0)
+    np.savez(os.path.join(HERE, "c2p.npz"), U=U, W_init=W0, W_out=Wr, codes=codes)
+
+    # PhysPressureFix on the failing states and on low-pressure states
+    Wf = prims(rng, 100, pr=(-9, -6))
+    _, Uf = O.ref_p2c(Wf)
+    Uf[:50] = U[:50]
+    Wf_in = np.full_like(Uf, -5.0)
+    Uo, Wo, fc = O.ref_pressure_fix(Uf, Wf_in)
+    np.savez(os.path.join(HERE, "fix.npz"), U_in=Uf, W_in=Wf_in, U_out=Uo, W_out=Wo, codes=fc)
+
+    # rotations, including n = +-z exactly and near it
+    n = unit(rng, 60)
+    n = np.vstack([n, [[0, 0, 1], [0, 0, -1], [0, 0, 1 - 1e-13], [1, 0, 0]]])
+    n[-2] /= np.linalg.norm(n[-2])
+    np.savez(os.path.join(HERE, "rotation.npz"), n=n, T=O.ref_rotation(n))
+
+    # fluxes: random pairs (subsonic HLLC, both us signs) + supersonic pairs
+    k = 1500
+    WL = prims(rng, k, rho=(-2, 2), pr=(-3, 2), wmax=10)
+    WR = prims(rng, k, rho=(-2, 2), pr=(-3, 2), wmax=10)
+    nn = unit(rng, k)
+    nn[:6] = [[0, 0, 1], [0, 0, -1], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, -1, 0]]
+    cold = slice(6, 206)  # cold supersonic streams along +n and -n
+    WL[cold, 4] = WL[cold, 0] * 1e-4
+    WR[cold, 4] = WR[cold, 0] * 1e-4
+    WL[6:106, 1:4] = 0.95 * nn[6:106]
+    WR[6:106, 1:4] = 0.95 * nn[6:106]
+    WL[106:206, 1:4] = -0.95 * nn[106:206]
+    WR[106:206, 1:4] = -0.95 * nn[106:206]
+    WR[206:256] = WL[206:256]  # identical states
+    _, UL = O.ref_p2c(WL)
+    _, UR = O.ref_p2c(WR)
+    nS = np.column_stack([nn, rng.uniform(0.01, 2.0, k)])
+    F, sp = O.ref_flux(UL, UR, WL, WR, nS)
+    np.savez(os.path.join(HERE, "flux.npz"), UL=UL, UR=UR, WL=WL, WR=WR, nS=nS, F=F, speed=sp)
+
+    # tiny trajectory: 8x2x2 hexes, shock tube, 200 steps
+    m = M.kuhn_box(8, 2, 2, length=(1.0, 0.25, 0.25), seed=SEED)
+    W0 = M.init_state(m, M.INIT_SHOCK_TUBE)
+    h = m.h_min()
+    Wt, Ut, taus, speeds = O.ref_run(m, W0, 0.4, h, 200)
+    np.savez(
+        os.path.join(HERE, "traj_tiny.npz"),
+        geo_cells=m.cells.view(np.uint8).reshape(-1), geo_faces=m.faces.view(np.uint8).reshape(-1),
+        W0=W0, cfl=0.4, h_min=h, W=Wt, U=Ut, taus=taus, speeds=speeds,
+    )
+
+    # config 0 after 200 steps (the fixture is too large to store; sums and hashes are)
+    m = M.kuhn_box(200, 10, 10, length=(1.0, 0.05, 0.05), seed=SEED)
+    W0 = M.init_state(m, M.INIT_SHOCK_TUBE)
+    h = m.h_min()
+    Wt, Ut, taus, speeds = O.ref_run(m, W0, 0.4, h, 200)
+    rec = {
+        "mesh": {"hexes": [200, 10, 10], "length": [1.0, 0.05, 0.05], "seed": SEED, "n_cells": m.n_cells,
+                 "n_faces": m.n_faces, "geo_cells_sha256": hashlib.sha256(m.cells.tobytes()).hexdigest(),
+                 "geo_faces_sha256": hashlib.sha256(m.faces.tobytes()).hexdigest()},
+        "h_min": h, "cfl": 0.4, "steps": 200,
+        "t": float(taus.sum()), "last_speed": float(speeds[-1]), "last_tau": float(taus[-1]),
+        "U_abs_sum": [float(np.abs(Ut[:, j]).sum()) for j in range(5)],
+        "W_abs_sum": [float(np.abs(Wt[:, j]).sum()) for j in range(5)],
+        "U_sha256": hashlib.sha256(np.ascontiguousarray(Ut).tobytes()).hexdigest(),
+        "W_sha256": hashlib.sha256(np.ascontiguousarray(Wt).tobytes()).hexdigest(),
+    }
+    with open(os.path.join(HERE, "shock_tube_200.json"), "w") as f:
+        json.dump(rec, f, indent=1)
+    print("wrote fixtures to", HERE)
+
+
+if __name__ == "__main__":
+    main()

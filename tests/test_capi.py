@@ -1,0 +1,89 @@
+"""The C-ABI libraries load and export every declared symbol; mesh validation
+in rhllc_create runs before any device call, so it is testable without a GPU."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from radrhd_solver_amd import _lib, capi
+from radrhd_solver_amd import mesh as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    with open(os.path.join(ROOT, "include", header)) as f:
+        txt = f.read()
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|double|const char \*)\s*\*?\s*(rhllc_\w+)\s*\(", txt, re.M)))
+
+
+def test_hip_library_exports_every_declared_symbol():
+    lib = _lib.load_hip_lib()
+    names = declared("rhllc_hip.h")
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(capi.SIGNATURES), set(names) ^ set(capi.SIGNATURES)
+
+
+def test_mesh_library_exports_every_declared_symbol():
+    lib = _lib.load_mesh_lib()
+    names = declared("rhllc_mesh.h")
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def _create(mesh, **kw):
+    p = capi.Params()
+    p.gamma, p.cfl, p.h_min = kw.get("gamma", 5 / 3), kw.get("cfl", 0.4), kw.get("h_min", 1e-3)
+    p.precision, p.c2p_guess = kw.get("precision", 0), kw.get("guess", 0)
+    h = C.c_void_p()
+    lib = _lib.load_hip_lib()
+    rc = lib.rhllc_create(C.byref(p), mesh.n_cells, mesh.cells.ctypes.data, mesh.n_faces, mesh.faces.ctypes.data,
+                          C.byref(h))
+    return rc, lib.rhllc_last_error().decode()
+
+
+def test_create_rejects_inconsistent_meshes_before_touching_the_device():
+    m = M.kuhn_box(2, 2, 2, seed=1)
+    bad = M.Mesh(m.cells.copy(), m.faces.copy())
+    bad.faces["w0"][3] = (bad.faces["w0"][3] & ~np.uint32(0x1FFFFFF)) | np.uint32(m.n_cells + 5)
+    rc, msg = _create(bad)
+    assert rc == 1 and "id_l out of range" in msg
+    bad = M.Mesh(m.cells.copy(), m.faces.copy())
+    bad.cells["sign_n"][0] ^= 1
+    rc, msg = _create(bad)
+    assert rc == 1 and "disagrees" in msg
+    bad = M.Mesh(m.cells.copy(), m.faces.copy())
+    f = int(np.nonzero(M.face_id_r(m.faces) == -1)[0][0])
+    bad.faces["w1"][f] = np.uint32((-3) & 0x3FFFFFF)
+    rc, msg = _create(bad)
+    assert rc == 1 and "out_source" in msg
+    bad.faces["w1"][f] = np.uint32((-9) & 0x3FFFFFF)
+    rc, msg = _create(bad)
+    assert rc == 1 and "unknown boundary code" in msg
+    bad = M.Mesh(m.cells.copy(), m.faces.copy())
+    bad.cells["V"][4] = -1.0
+    rc, msg = _create(bad)
+    assert rc == 1 and "V <= 0" in msg
+
+
+def test_create_rejects_bad_parameters():
+    m = M.kuhn_box(2, 2, 2, seed=1)
+    assert _create(m, gamma=1.0)[0] == 1
+    assert _create(m, cfl=0.0)[0] == 1
+    assert _create(m, precision=7)[0] == 1
+    assert _create(m, precision=0, guess=2)[0] == 1
+
+
+def test_null_arguments_are_errors_not_crashes():
+    lib = _lib.load_hip_lib()
+    assert lib.rhllc_create(None, 1, None, 1, None, None) == 1
+    assert lib.rhllc_step(None, 0.1, None) == 1
+    assert lib.rhllc_run(None, 1, 1) == 1
+    assert lib.rhllc_get_stats(None, None) == 1
+    assert lib.rhllc_upload_state(None, None, 40, 0, -1) == 1
+    assert lib.rhllc_destroy(None) == 0

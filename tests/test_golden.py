@@ -1,0 +1,92 @@
+"""The oracle against the golden fixtures written by the compiled reference.
+
+tests/golden/make_golden.py produced these from the reference's own sources
+(solvers/rhllc/src/rhllc_flux_stab.cpp, rhllc_3d_stab.cpp, rhllc_utils.cpp,
+lib/geometry/src/linear_alg.cpp) built strict-IEEE with EOS IDEAL.  The oracle
+must reproduce every value bit for bit.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from radrhd_solver_amd import mesh as M
+
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return np.load(os.path.join(G, name), allow_pickle=False)
+
+
+def test_p2c_bitwise():
+    d = load("p2c.npz")
+    Wc, U = O.p2c(d["W_in"])
+    assert np.array_equal(Wc, d["W_out"])
+    assert np.array_equal(U, d["U"])
+
+
+def test_c2p_bitwise_including_failures():
+    d = load("c2p.npz")
+    W, codes, _ = O.c2p(d["U"], W_init=d["W_init"])
+    assert np.array_equal(codes, d["codes"])
+    assert np.array_equal(W, d["W_out"])
+    assert (d["codes"] == 1).sum() == 100
+
+
+def test_pressure_fix_bitwise():
+    d = load("fix.npz")
+    U, W, codes = O.pressure_fix(d["U_in"], d["W_in"])
+    assert np.array_equal(codes, d["codes"])
+    assert np.array_equal(U, d["U_out"]) and np.array_equal(W, d["W_out"])
+
+
+def test_rotation_bitwise():
+    d = load("rotation.npz")
+    for n, T in zip(d["n"], d["T"]):
+        assert np.array_equal(O.rotation(n), T)
+
+
+def test_flux_bitwise_all_branches():
+    d = load("flux.npz")
+    F, sp = O.flux(d["UL"], d["UR"], d["WL"], d["WR"], d["nS"])
+    assert np.array_equal(F, d["F"])
+    assert np.array_equal(sp, d["speed"])
+
+
+def test_tiny_trajectory_200_steps_bitwise():
+    d = load("traj_tiny.npz")
+    m = M.Mesh(d["geo_cells"].view(M.GEO_CELL_DTYPE).copy(), d["geo_faces"].view(M.GEO_FACE_DTYPE).copy())
+    st = O.State(m, d["W0"])
+    _, taus, speeds = st.run(float(d["cfl"]), float(d["h_min"]), 200)
+    assert np.array_equal(taus, d["taus"]) and np.array_equal(speeds, d["speeds"])
+    assert np.array_equal(st.U, d["U"]) and np.array_equal(st.W, d["W"])
+
+
+def test_shock_tube_config0_200_steps_hash():
+    """Config 0 (120k tets, 200 steps): the mesh and the final state hash to the reference's values."""
+    with open(os.path.join(G, "shock_tube_200.json")) as f:
+        rec = json.load(f)
+    mm = rec["mesh"]
+    m = M.kuhn_box(*mm["hexes"], length=tuple(mm["length"]), seed=mm["seed"])
+    assert hashlib.sha256(m.cells.tobytes()).hexdigest() == mm["geo_cells_sha256"]
+    assert hashlib.sha256(m.faces.tobytes()).hexdigest() == mm["geo_faces_sha256"]
+    assert m.h_min() == rec["h_min"]
+    W0 = M.init_state(m, M.INIT_SHOCK_TUBE)
+    st = O.State(m, W0)
+    t, taus, speeds = st.run(rec["cfl"], rec["h_min"], rec["steps"])
+    assert float(taus.sum()) == rec["t"] and float(speeds[-1]) == rec["last_speed"]
+    assert hashlib.sha256(np.ascontiguousarray(st.U).tobytes()).hexdigest() == rec["U_sha256"]
+    assert hashlib.sha256(np.ascontiguousarray(st.W).tobytes()).hexdigest() == rec["W_sha256"]
+
+
+def test_planted_mistake_is_caught():
+    """A plausible slip in the oracle (a dropped term) must fail the golden flux check."""
+    d = load("flux.npz")
+    F, _ = O.flux(d["UL"], d["UR"], d["WL"], d["WR"], d["nS"])
+    F_bad = F.copy()
+    F_bad[:, 1] -= d["WL"][:, 4] * d["nS"][:, 0] * d["nS"][:, 3] * 1e-15 + 1e-300
+    assert not np.array_equal(F_bad, d["F"])

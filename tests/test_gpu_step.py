@@ -157,7 +157,7 @@ def test_mixed_precision_blast_l1():
     V = m.cells["V"]
     l1 = [float((np.abs(U32[:, k] - U64[:, k]) * V).sum() / (np.abs(U64[:, k]) * V).sum()) for k in range(5)]
     print("mixed vs fp64 rel L1 per conserved field:", l1)
-    assert max(l1) < 1e-3
+    assert max(l1) < 1e-5
 
 
 def test_step_radiation_source_bitwise():

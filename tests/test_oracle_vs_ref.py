@@ -1,0 +1,83 @@
+"""The oracle restatement against the reference compiled from its own sources.
+
+Skipped where oracle/_ref/libref_rhllc.so was not built (it needs the
+reference tree at build time).  Every comparison is bitwise.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from radrhd_solver_amd import mesh as M
+
+pytestmark = pytest.mark.skipif(not O.have_ref(), reason="reference build oracle/_ref not present")
+SEED = 20261202
+
+
+def prims(rng, n, wmax=50.0):
+    r = 10 ** rng.uniform(-4, 2, n)
+    p = r * 10 ** rng.uniform(-6, 2, n)
+    W = np.exp(rng.uniform(0, np.log(wmax), n))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    return np.column_stack([r, d * np.sqrt(1 - 1 / W**2)[:, None], p])
+
+
+def test_record_layouts_match():
+    out = np.zeros(9, np.int32)
+    O.ref_lib().ref_layout(out.ctypes.data)
+    # flux_t, face_t, elem_t, geo_face_t, geo_cell_t, offsetof(conv_val), offsetof(geo), offsetof(face.geo), EOS
+    assert out.tolist() == [40, 80, 136, 40, 56, 40, 80, 40, 1]
+    assert M.GEO_FACE_DTYPE.itemsize == 40 and M.GEO_CELL_DTYPE.itemsize == 56
+    assert O.ref_lib().ref_gamma() == 5.0 / 3.0
+
+
+def test_pointwise_functions_bitwise():
+    rng = np.random.default_rng(SEED)
+    W = prims(rng, 5000)
+    Wr, Ur = O.ref_p2c(W)
+    Wo, Uo = O.p2c(W)
+    assert np.array_equal(Ur, Uo) and np.array_equal(Wr, Wo)
+    Ur[:300, 4] *= 0.5  # force failures
+    W0 = np.full_like(Ur, 3.0)
+    Wr, cr = O.ref_c2p(Ur, W0)
+    Wo, co, _ = O.c2p(Ur, W_init=W0)
+    assert np.array_equal(cr, co) and np.array_equal(Wr, Wo)
+    bad = cr != 0
+    a = O.ref_pressure_fix(Ur[bad], Wr[bad])
+    b = O.pressure_fix(Ur[bad], Wo[bad])
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_flux_bitwise_random_and_axis_normals():
+    rng = np.random.default_rng(SEED + 1)
+    n = 8000
+    WL, WR = prims(rng, n, 10.0), prims(rng, n, 10.0)
+    _, UL = O.ref_p2c(WL)
+    _, UR = O.ref_p2c(WR)
+    nn = rng.normal(size=(n, 3))
+    nn /= np.linalg.norm(nn, axis=1)[:, None]
+    nn[:6] = np.eye(3).tolist() + (-np.eye(3)).tolist()
+    nS = np.column_stack([nn, rng.uniform(0.01, 3, n)])
+    Fr, sr = O.ref_flux(UL, UR, WL, WR, nS)
+    Fo, so = O.flux(UL, UR, WL, WR, nS)
+    assert np.array_equal(Fr, Fo) and np.array_equal(sr, so)
+
+
+@pytest.mark.parametrize("case", ["shock_tube", "blast", "uniform_moving"])
+def test_trajectories_bitwise(case):
+    if case == "shock_tube":
+        m = M.kuhn_box(40, 4, 4, length=(1.0, 0.1, 0.1), seed=SEED)
+        W0 = M.init_state(m, M.INIT_SHOCK_TUBE)
+    elif case == "blast":
+        m = M.kuhn_box(10, 10, 10, seed=SEED)
+        W0 = M.init_state(m, M.INIT_BLAST, params=[0.5, 0.5, 0.5, 0.25, 1.0, 1000.0, 1.0])
+    else:
+        m = M.kuhn_box(6, 6, 6, periodic=7, seed=SEED)
+        W0 = M.init_state(m, M.INIT_UNIFORM, params=[1.0, 0.5, 0.3, -0.2, 2.0])
+    h = m.h_min()
+    Wr, Ur, tr, sr = O.ref_run(m, W0, 0.4, h, 60)
+    st = O.State(m, W0)
+    _, to, so = st.run(0.4, h, 60)
+    assert np.array_equal(tr, to) and np.array_equal(sr, so)
+    assert np.array_equal(Ur, st.U) and np.array_equal(Wr, st.W)
