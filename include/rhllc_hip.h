@@ -45,8 +45,8 @@ extern "C" {
 /* Initial pressure of the c2p Newton iteration. */
 #define RHLLC_GUESS_REFERENCE 0 /* max(|m|-E, 0, pmin) as rhllc_flux_stab.cpp:139-151 */
 #define RHLLC_GUESS_PREVIOUS 1  /* the previous step's pressure */
-#define RHLLC_GUESS_MLP 2       /* the MLP surrogate; previous pressure, then the reference
-                                   start, then PhysPressureFix as fallbacks */
+#define RHLLC_GUESS_MLP 2       /* the MLP surrogate (rhllc_set_mlp); the previous pressure,
+                                   then the reference start, then PhysPressureFix as fallbacks */
 
 typedef struct rhllc_params {
   double gamma;       /* adiabatic index of the ideal-gas EOS (reference: 5/3 constexpr) */
@@ -147,6 +147,48 @@ int rhllc_eval_rad_source(int64_t n, const double *W, const double *Ur, const do
  * (read + write) bandwidth.  Used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE
  * counters and as the measured HBM ceiling.  No counterpart in the reference. */
 int rhllc_copy_probe(int32_t device, int64_t nbytes, int32_t reps, double *gbps);
+
+/* ---- c2p surrogate (no counterpart in the reference) --------------------
+ * A 3-64-64-1 SiLU MLP maps (log D, log1p(|m|/D), log(tau/D)), tau = E - D,
+ * to log p.  The model is RHLLC_MLP_MODEL floats: the parameters W1[64][3]
+ * b1[64] W2[64][64] b2[64] W3[64] b3 followed by the normalisation
+ * mu_x[3] sd_x[3] mu_y sd_y. */
+#define RHLLC_MLP_PARAMS 4481
+#define RHLLC_MLP_MODEL 4489
+
+typedef struct rhllc_mlp_config {
+  int32_t device;
+  double gamma;          /* EOS index of the training states */
+  int32_t steps;         /* Adam steps */
+  int32_t batch;         /* samples per step, multiple of 256 */
+  double lr;             /* peak learning rate (cosine schedule) */
+  uint64_t seed;
+  int32_t norm_samples;  /* samples used for the feature/target normalisation */
+} rhllc_mlp_config;
+
+/* Train on the device with the library's own forward/backward kernels (BF16
+ * MFMA, FP32 accumulation, FP32 master weights, Adam).  Training states are
+ * sampled log-uniformly (rho in [1e-4, 1e2], p/rho in [1e-6, 1e2], Lorentz
+ * factor in [1, 50], isotropic velocity), converted by the FP64 p2c and
+ * labelled by the FP64 c2p.  loss_hist (may be NULL) receives the mean
+ * squared error of the normalised target per step. */
+int rhllc_mlp_train(const rhllc_mlp_config *cfg, float *model_out, float *loss_hist);
+
+/* Pressure predicted by the model for n conserved states (U: n x 5). */
+int rhllc_mlp_eval(int32_t device, const float *model, int64_t n, const double *U, double *p_out);
+
+/* Gradient of sum_i ok_i (f(x_i) - y_i)^2 / n over one batch of normalised
+ * inputs X (n x 3), targets y and masks ok (n a multiple of 256), summed over
+ * waves in order; *loss receives sum_i ok_i (f(x_i) - y_i)^2. */
+int rhllc_mlp_grad(int32_t device, const float *model, int32_t n, const float *X, const float *y, const float *ok,
+                   float *grad, float *loss);
+
+/* Runs one v_mfma_f32_16x16x16_bf16 on known operands; out[64*4] = the
+ * accumulator registers per lane, to check the lane layout the kernels use. */
+int rhllc_mlp_selftest(int32_t device, float *out);
+
+/* Installs a model for RHLLC_GUESS_MLP (mixed path). */
+int rhllc_set_mlp(rhllc_solver *h, const float *model);
 
 /* Device name and compute units of the current device (for reports). */
 int rhllc_device_query(int32_t device, char *name, int32_t name_len, int32_t *cus, double *hbm_gb);

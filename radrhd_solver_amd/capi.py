@@ -14,6 +14,21 @@ GUESS_REFERENCE, GUESS_PREVIOUS, GUESS_MLP = 0, 1, 2
 HIST_BINS = 41
 
 
+MLP_PARAMS, MLP_MODEL = 4481, 4489
+
+
+class MlpConfig(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32),
+        ("gamma", C.c_double),
+        ("steps", C.c_int32),
+        ("batch", C.c_int32),
+        ("lr", C.c_double),
+        ("seed", C.c_uint64),
+        ("norm_samples", C.c_int32),
+    ]
+
+
 class Params(C.Structure):
     _fields_ = [
         ("gamma", C.c_double),
@@ -85,6 +100,11 @@ SIGNATURES = {
     "rhllc_eval_rad_source": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_device_query": (C.c_int, [_i32, C.c_char_p, _i32, _vp, _vp]),
     "rhllc_copy_probe": (C.c_int, [_i32, _i64, _i32, _vp]),
+    "rhllc_mlp_train": (C.c_int, [_vp, _vp, _vp]),
+    "rhllc_mlp_eval": (C.c_int, [_i32, _vp, _i64, _vp, _vp]),
+    "rhllc_mlp_grad": (C.c_int, [_i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "rhllc_mlp_selftest": (C.c_int, [_i32, _vp]),
+    "rhllc_set_mlp": (C.c_int, [_vp, _vp]),
     "rhllc_last_error": (C.c_char_p, []),
 }
 
@@ -199,6 +219,11 @@ class Solver:
         self._rad = arrs
         check(_lib().rhllc_set_radiation(self.h, *[a.ctypes.data for a in arrs]))
 
+    def set_mlp(self, model):
+        model = np.ascontiguousarray(model, dtype=np.float32)
+        assert model.size == MLP_MODEL
+        check(_lib().rhllc_set_mlp(self.h, model.ctypes.data))
+
     def time_kernels(self, nsteps: int):
         out = np.zeros(4)
         check(_lib().rhllc_time_kernels(self.h, nsteps, out.ctypes.data))
@@ -258,6 +283,41 @@ def eval_rad_source(W, Ur, Fr, ka, ks):
     check(_lib().rhllc_eval_rad_source(n, W.ctypes.data, Ur.ctypes.data, Fr.ctypes.data, ka.ctypes.data,
                                        ks.ctypes.data, G.ctypes.data))
     return G
+
+
+def mlp_train(steps=20000, batch=65536, lr=2e-3, gamma=5.0 / 3.0, seed=20261202, norm_samples=1 << 20, device=0):
+    """Trains the c2p surrogate on the device; returns (model[4489] float32, loss per step)."""
+    cfg = MlpConfig(device, gamma, steps, batch, lr, seed, norm_samples)
+    model = np.zeros(MLP_MODEL, np.float32)
+    hist = np.zeros(steps, np.float32)
+    check(_lib().rhllc_mlp_train(C.byref(cfg), model.ctypes.data, hist.ctypes.data))
+    return model, hist
+
+
+def mlp_eval(model, U, device=0):
+    model = np.ascontiguousarray(model, dtype=np.float32)
+    U = np.ascontiguousarray(U, dtype=np.float64).reshape(-1, 5)
+    p = np.zeros(U.shape[0])
+    check(_lib().rhllc_mlp_eval(device, model.ctypes.data, U.shape[0], U.ctypes.data, p.ctypes.data))
+    return p
+
+
+def mlp_grad(model, X, y, ok, device=0):
+    model = np.ascontiguousarray(model, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 3)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    ok = np.ascontiguousarray(ok, dtype=np.float32)
+    g = np.zeros(MLP_PARAMS, np.float32)
+    loss = C.c_float()
+    check(_lib().rhllc_mlp_grad(device, model.ctypes.data, X.shape[0], X.ctypes.data, y.ctypes.data, ok.ctypes.data,
+                                g.ctypes.data, C.byref(loss)))
+    return g, loss.value
+
+
+def mlp_selftest(device=0):
+    out = np.zeros(256, np.float32)
+    check(_lib().rhllc_mlp_selftest(device, out.ctypes.data))
+    return out.reshape(64, 4)
 
 
 def copy_probe(nbytes: int = 1 << 30, reps: int = 20, device: int = 0) -> float:

@@ -1,0 +1,119 @@
+// MLP surrogate of the c2p pressure: (log D, log1p(|m|/D), log(tau/D)) -> log p,
+// 3 -> 64 -> 64 -> 1 with SiLU, no counterpart in the reference.
+//
+// Parameter vector (FP32, kParams floats):
+//   W1[64][3] b1[64] W2[64][64] b2[64] W3[64] b3[1]     (weights stored [out][in])
+// followed by the feature/target normalisation mu_x[3] sd_x[3] mu_y sd_y.
+//
+// Wave-level layout.  A wave evaluates 64 samples with the 64x64 layer on
+// v_mfma_f32_16x16x16_bf16 (FP32 accumulate).  For the 16x16x16 instruction
+// the operand/result lane maps are
+//   A[i][k]: lane i + 16*(k/4), element k%4
+//   B[k][j]: lane j + 16*(k/4), element k%4
+//   C[i][j]: lane j + 16*(i/4), element i%4
+// (checked on the device by rhllc_mlp_selftest).  The forward product is
+// Z2^T = W2 * H1^T, so lane l holds, for samples s_t = 16 t + l%16 (t = 0..3),
+// the hidden units 16 q + 4 (l/16) + e of both layers -- layer 1 is computed
+// directly in that layout and no transpose is needed until dW2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rhllc_mlp {
+
+constexpr int kH = 64;
+constexpr int kIn = 3;
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kH * kIn;  // 192
+constexpr int kOffW2 = kOffB1 + kH;        // 256
+constexpr int kOffB2 = kOffW2 + kH * kH;   // 4352
+constexpr int kOffW3 = kOffB2 + kH;        // 4416
+constexpr int kOffB3 = kOffW3 + kH;        // 4480
+constexpr int kParams = kOffB3 + 1;        // 4481
+constexpr int kOffNorm = kParams;          // mu_x[3], sd_x[3], mu_y, sd_y
+constexpr int kModel = kParams + 8;        // 4489
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short f2bf(float x) {  // round to nearest even
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((uint32_t)(uint16_t)b) << 16); }
+
+__device__ __forceinline__ float silu(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+// d silu / dz = s (1 + z (1 - s)), s = sigmoid(z)
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+// Features of a conserved state (FP64 in, FP32 out), normalised.
+__device__ __forceinline__ void features(const float *norm, double D, double m, double E, float x[3]) {
+  const double tau = E - D;
+  const double r0 = log(D);
+  const double r1 = log1p(m / D);
+  const double r2 = log(tau > 1e-300 * D ? tau / D : 1e-300);
+  x[0] = (float)((r0 - norm[0]) / norm[3]);
+  x[1] = (float)((r1 - norm[1]) / norm[4]);
+  x[2] = (float)((r2 - norm[2]) / norm[5]);
+}
+
+// Forward pass of one wave over 64 samples.  x[t][f] holds the normalised
+// features of sample 16 t + lane%16 (the caller fills them, e.g. by shuffles).
+// Returns, in every lane, the normalised outputs of its 4 samples s_t.
+// W2b: W2 as bf16 row-major [out][in]; P: FP32 parameters.
+__device__ __forceinline__ void forward_wave(const float *__restrict__ P, const short *__restrict__ W2b,
+                                             const float x[4][3], float out[4]) {
+  const int l = threadIdx.x & 63;
+  const int g = l >> 4, c = l & 15;
+  // Layer 1 in the B-operand layout of H1^T: hidden 16 q + 4 g + e of sample t.
+  bf16x4 h1[4][4];  // [q][t]
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int h = 16 * q + 4 * g + e;
+      const float w0 = P[kOffW1 + 3 * h], w1 = P[kOffW1 + 3 * h + 1], w2 = P[kOffW1 + 3 * h + 2], bb = P[kOffB1 + h];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const float z = ((w0 * x[t][0] + w1 * x[t][1]) + w2 * x[t][2]) + bb;
+        h1[q][t][e] = f2bf(silu(z));
+      }
+    }
+  }
+  // Z2^T[h2][s] = sum_h1 W2[h2][h1] H1^T[h1][s]: tiles (u over h2, t over samples).
+  float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    bf16x4 a[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      a[q] = *reinterpret_cast<const bf16x4 *>(W2b + (16 * u + c) * kH + 16 * q + 4 * g);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[q], h1[q][t], acc, 0, 0, 0);
+      // lane holds Z2[s = 16 t + c][h2 = 16 u + 4 g + r]
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int h2 = 16 * u + 4 * g + r;
+        part[t] += P[kOffW3 + h2] * silu(acc[r] + P[kOffB2 + h2]);
+      }
+    }
+  }
+  // Sum the 4 lane groups g (hidden units) of each sample.
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    float v = part[t];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    out[t] = v + P[kOffB3];
+  }
+}
+
+}  // namespace rhllc_mlp

@@ -13,6 +13,7 @@
 
 #include "../../../include/rhllc_hip.h"
 #include "../mesh/mesh_common.h"
+#include "mlp_internal.h"
 #include "rhllc_internal.h"
 
 using namespace rhllc_dev;
@@ -37,6 +38,10 @@ int hip_fail(hipError_t e, const char *what) {
 
 }  // namespace
 
+namespace rhllc_capi_detail {
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace rhllc_capi_detail
+
 struct rhllc_solver {
   rhllc_params prm{};
   int32_t n_cells = 0, n_faces = 0;
@@ -53,6 +58,8 @@ struct rhllc_solver {
   float2 *rad_k = nullptr;
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_src = nullptr;
+  float *mlp_P = nullptr;
+  short *mlp_W2b = nullptr;
 
   template <typename T>
   hipError_t alloc(T **p, size_t count) {
@@ -76,9 +83,13 @@ struct rhllc_solver {
     for (void *p : allocs) (void)hipFree(p);
     if (rad_FU) (void)hipFree(rad_FU);
     if (rad_k) (void)hipFree(rad_k);
+    if (mlp_P) (void)hipFree(mlp_P);
+    if (mlp_W2b) (void)hipFree(mlp_W2b);
     if (host_sc) (void)hipHostFree(host_sc);
     if (stream) (void)hipStreamDestroy(stream);
   }
+
+  bool ready() const { return has_state && (variant != kMixedMlp || mlp_P != nullptr); }
 
   hipError_t enqueue_step() {
     hipError_t e = launch_face(d, mixed, stream);
@@ -114,8 +125,10 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
   } else {
     if (p.c2p_guess == RHLLC_GUESS_PREVIOUS)
       variant = kMixedPrev;
+    else if (p.c2p_guess == RHLLC_GUESS_MLP)
+      variant = kMixedMlp;
     else
-      return fail(RHLLC_ERR_ARG, "the mixed path supports the previous-pressure start");
+      return fail(RHLLC_ERR_ARG, "the mixed path supports the previous-pressure and MLP starts");
   }
 
   const int32_t N = (int32_t)n_cells, F = (int32_t)n_faces;
@@ -354,7 +367,7 @@ extern "C" int rhllc_download_state(rhllc_solver *h, void *recs, int64_t stride,
 
 extern "C" int rhllc_step(rhllc_solver *h, double tau, double *max_speed) {
   if (!h) return fail(RHLLC_ERR_ARG, "null handle");
-  if (!h->has_state) return fail(RHLLC_ERR_STATE, "no state uploaded");
+  if (!h->ready()) return fail(RHLLC_ERR_STATE, "no state uploaded (or no MLP model set for RHLLC_GUESS_MLP)");
   HIP_TRY(hipSetDevice(h->prm.device));
   h->host_sc->tau = tau;
   HIP_TRY(hipMemcpyAsync(&h->d.sc->tau, &h->host_sc->tau, sizeof(double), hipMemcpyHostToDevice, h->stream));
@@ -369,7 +382,7 @@ extern "C" int rhllc_step(rhllc_solver *h, double tau, double *max_speed) {
 
 extern "C" int rhllc_run(rhllc_solver *h, int32_t nsteps, int32_t use_graph) {
   if (!h) return fail(RHLLC_ERR_ARG, "null handle");
-  if (!h->has_state) return fail(RHLLC_ERR_STATE, "no state uploaded");
+  if (!h->ready()) return fail(RHLLC_ERR_STATE, "no state uploaded (or no MLP model set for RHLLC_GUESS_MLP)");
   if (nsteps < 0) return fail(RHLLC_ERR_ARG, "nsteps < 0");
   HIP_TRY(hipSetDevice(h->prm.device));
   if (!use_graph) {
@@ -497,9 +510,24 @@ extern "C" int rhllc_set_radiation(rhllc_solver *h, const float *Ur, const float
   return RHLLC_OK;
 }
 
+extern "C" int rhllc_set_mlp(rhllc_solver *h, const float *model) {
+  if (!h || !model) return fail(RHLLC_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->drop_graph();
+  if (!h->mlp_P) HIP_TRY(hipMalloc((void **)&h->mlp_P, sizeof(float) * RHLLC_MLP_MODEL));
+  if (!h->mlp_W2b) HIP_TRY(hipMalloc((void **)&h->mlp_W2b, sizeof(short) * 64 * 64 * 2));
+  HIP_TRY(hipMemcpy(h->mlp_P, model, sizeof(float) * RHLLC_MLP_MODEL, hipMemcpyHostToDevice));
+  HIP_TRY(rhllc_mlp::launch_pack(h->mlp_P, h->mlp_W2b, h->mlp_W2b + 64 * 64, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->d.mlpP = h->mlp_P;
+  h->d.mlpW2b = h->mlp_W2b;
+  return RHLLC_OK;
+}
+
 extern "C" int rhllc_time_kernels(rhllc_solver *h, int32_t nsteps, double out[4]) {
   if (!h || !out) return fail(RHLLC_ERR_ARG, "null argument");
-  if (!h->has_state) return fail(RHLLC_ERR_STATE, "no state uploaded");
+  if (!h->ready()) return fail(RHLLC_ERR_STATE, "no state uploaded (or no MLP model set)");
   if (nsteps < 1) return fail(RHLLC_ERR_ARG, "nsteps < 1");
   HIP_TRY(hipSetDevice(h->prm.device));
   hipEvent_t ev[4];

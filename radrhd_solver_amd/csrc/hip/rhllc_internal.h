@@ -53,8 +53,9 @@ struct Dev {
   const float2 *rad_k;
   // fixed ghost state for boundary code -3
   double ghostW[5], ghostU[5];
-  // surrogate c2p guess (optional)
-  const void *mlp;
+  // surrogate c2p guess (RHLLC_GUESS_MLP): FP32 model and bf16 W2
+  const float *mlpP;
+  const short *mlpW2b;
   Scalars *sc;
   double *speed_slots;  // kSpeedSlots * kSlotStride
 };
@@ -63,7 +64,7 @@ enum Variant : int {
   kFp64Ref = 0,   // FP64, reference c2p start (bitwise parity)
   kFp64Prev = 1,  // FP64, previous-pressure start
   kMixedPrev = 2, // mixed, previous-pressure start
-  kMixedMlp = 3,  // mixed, MLP start
+  kMixedMlp = 3,  // mixed, MLP start (previous pressure, then reference start as fallbacks)
 };
 
 hipError_t launch_p2c_init(const Dev &d, hipStream_t s);

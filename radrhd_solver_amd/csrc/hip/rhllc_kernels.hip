@@ -11,6 +11,7 @@
 // No float atomics: face fluxes are written once and read by a fixed-order
 // gather, so the step is deterministic; the speed maximum is order-free.
 #include "rhllc_internal.h"
+#include "mlp.h"
 #include "rhllc_physics.h"
 
 namespace rhllc_dev {
@@ -135,6 +136,24 @@ __device__ __forceinline__ void count_if(unsigned long long *ctr, bool pred) {
   if (m && (threadIdx.x & 63) == __ffsll((long long)m) - 1) atomicAdd(ctr, (unsigned long long)__popcll(m));
 }
 
+// Surrogate start: every lane of the wave takes part (MFMA), dead lanes with
+// a dummy state; returns the predicted pressure of this lane's cell.
+__device__ __forceinline__ double mlp_guess(const Dev &d, bool live, const State<double> &U) {
+  using namespace rhllc_mlp;
+  const float *norm = d.mlpP + kOffNorm;
+  float xl[3] = {0.f, 0.f, 0.f};
+  if (live) features(norm, U.d, sqrt(dot3(U.v, U.v)), U.p, xl);
+  const int l = threadIdx.x & 63;
+  float x[4][3];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int f = 0; f < 3; ++f) x[t][f] = __shfl(xl[f], 16 * t + (l & 15), 64);
+  float out[4];
+  forward_wave(d.mlpP, d.mlpW2b, x, out);
+  return exp((double)out[l >> 4] * norm[7] + norm[6]);
+}
+
 template <int VARIANT, bool RAD, bool STATS>
 __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
   constexpr bool MIXED = VARIANT >= kMixedPrev;
@@ -143,6 +162,8 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
   int iters = 0, code = 0;
   bool fell_back = false, fixed = false, fix_failed = false;
   double resid = 0.0;
+  const double tau = d.sc->tau;
+  State<double> U, W;
   if (live) {
     const int4 cf = d.cell_faces[c];
     const int fidx[4] = {cf.x, cf.y, cf.z, cf.w};
@@ -158,19 +179,27 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
         s.d -= F.d, s.v[0] -= F.v[0], s.v[1] -= F.v[1], s.v[2] -= F.v[2], s.p -= F.p;
       }
     }
-    const double tau = d.sc->tau;
     const double k = tau / d.cell_V[c];
     s.d *= k, s.v[0] *= k, s.v[1] *= k, s.v[2] *= k, s.p *= k;
-    State<double> U, W;
     load_U(d, c, U);
     U.d -= s.d, U.v[0] -= s.v[0], U.v[1] -= s.v[1], U.v[2] -= s.v[2], U.p -= s.p;
-    // The reference start ignores the previous W; the FP64 previous-pressure
-    // variant reads the FP64 pressure, the mixed path the FP32 one.  c2p writes
-    // W only when it succeeds.
-    double guess = -1.0;
-    if (VARIANT == kFp64Prev) guess = d.Wp[c];
-    if (MIXED) guess = (double)d.Wpf[c];
+  }
+  // Start of the Newton iteration.  The reference start ignores the previous
+  // W; the FP64 previous-pressure variant reads the FP64 pressure, the mixed
+  // path the FP32 one; the surrogate evaluates the MLP on the updated U.
+  double guess = -1.0;
+  if (VARIANT == kFp64Prev && live) guess = d.Wp[c];
+  if (VARIANT == kMixedPrev && live) guess = (double)d.Wpf[c];
+  if (VARIANT == kMixedMlp) guess = mlp_guess(d, live, U);
+  if (live) {
+    // c2p writes W only when it succeeds.
     code = MIXED ? c2p_fast(d.Gg, U, W, guess, iters) : c2p(d.Gg, U, W, guess, iters);
+    if (code && VARIANT == kMixedMlp) {
+      int it2 = 0;
+      fell_back = true;
+      code = c2p_fast(d.Gg, U, W, (double)d.Wpf[c], it2);
+      iters += it2;
+    }
     if (code && VARIANT != kFp64Ref) {
       int it2 = 0;
       fell_back = true;
@@ -362,6 +391,9 @@ hipError_t launch_cell(const Dev &d, int variant, bool rad, bool stats, hipStrea
       break;
     case kMixedPrev:
       launch_cell_v<kMixedPrev>(d, rad, stats, s);
+      break;
+    case kMixedMlp:
+      launch_cell_v<kMixedMlp>(d, rad, stats, s);
       break;
     default:
       return hipErrorInvalidValue;
