@@ -37,6 +37,8 @@ extern "C" {
 #define RHLLC_ERR_HIP 2     /* HIP runtime failure (no device, out of memory, launch) */
 #define RHLLC_ERR_STATE 3   /* call out of order (e.g. step before upload) */
 #define RHLLC_ERR_NUMERIC 4 /* non-finite max signal speed */
+#define RHLLC_ERR_C2P 5     /* rhllc_step: a PhysPressureFix failed in this step (the reference
+                               aborts there, rhllc_3d_stab.cpp:49-52); the state was updated */
 
 /* Arithmetic of the step. */
 #define RHLLC_PREC_FP64 0  /* all FP64, bitwise equal to the reference compiled strict */

@@ -55,9 +55,27 @@ def build_hip() -> str:
     return HIP_LIB
 
 
+SHIM_DIR = os.path.join(PKG_DIR, "shim")
+SOLVER_EXE = os.path.join(PKG_DIR, "bin", "rhllc_solver")
+DROPIN_LIB = os.path.join(PKG_DIR, "librhllc_dropin.so")
+
+
+def build_shim() -> str:
+    """The drop-in C++ interface (librhllc_dropin.so) and the rhllc_solver executable."""
+    os.makedirs(os.path.dirname(SOLVER_EXE), exist_ok=True)
+    inc = ["-I", os.path.join(os.path.dirname(PKG_DIR), "include")]
+    link = ["-L", PKG_DIR, "-lrhllc_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."]
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wno-invalid-offsetof", "-o", DROPIN_LIB,
+          os.path.join(SHIM_DIR, "rhllc_dropin.cpp")] + inc + link)
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-o", SOLVER_EXE, os.path.join(SHIM_DIR, "rhllc_solver_main.cpp")]
+         + inc + ["-L", PKG_DIR, "-lrhllc_dropin", "-lrhllc_hip", "-Wl,-rpath,$ORIGIN/.."])
+    return SOLVER_EXE
+
+
 def build_all() -> None:
     build_mesh()
     build_hip()
+    build_shim()
 
 
 if __name__ == "__main__":

@@ -369,6 +369,7 @@ extern "C" int rhllc_step(rhllc_solver *h, double tau, double *max_speed) {
   if (!h) return fail(RHLLC_ERR_ARG, "null handle");
   if (!h->ready()) return fail(RHLLC_ERR_STATE, "no state uploaded (or no MLP model set for RHLLC_GUESS_MLP)");
   HIP_TRY(hipSetDevice(h->prm.device));
+  const unsigned long long fix_fail_before = h->host_sc->fix_fail;
   h->host_sc->tau = tau;
   HIP_TRY(hipMemcpyAsync(&h->d.sc->tau, &h->host_sc->tau, sizeof(double), hipMemcpyHostToDevice, h->stream));
   HIP_TRY(h->enqueue_step());
@@ -377,6 +378,9 @@ extern "C" int rhllc_step(rhllc_solver *h, double tau, double *max_speed) {
   const double s = h->host_sc->last_speed;
   if (max_speed) *max_speed = s;
   if (!std::isfinite(s)) return fail(RHLLC_ERR_NUMERIC, "non-finite max signal speed");
+  if (h->host_sc->fix_fail > fix_fail_before)
+    return fail(RHLLC_ERR_C2P, std::to_string(h->host_sc->fix_fail - fix_fail_before) +
+                                   " cells failed PhysPressureFix in this step");
   return RHLLC_OK;
 }
 

@@ -1,0 +1,360 @@
+// Drop-in RHLLC interface on top of librhllc_hip.so (see rhllc_dropin.h).
+#include "rhllc_dropin.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+
+#include "../../include/rhllc_hip.h"
+#include "json_min.h"
+
+hllc_value_t _hllc_cfg;
+solve_mode_t _solve_mode;
+global_files_t glb_files;
+
+namespace {
+
+[[noreturn]] void die(const char *what, const std::string &detail = "") {
+  // DIE_IF of lib/dbgdef.h:50-65 logs the condition and aborts.
+  std::fprintf(stderr, "DIE: %s %s\n", what, detail.c_str());
+  std::fflush(stderr);
+  std::abort();
+}
+
+struct DeviceGrid {
+  rhllc_solver *h = nullptr;
+  const elem_t *cells_ptr = nullptr;
+  size_t n_cells = 0, n_faces = 0;
+  bool resident = false;
+  bool uploaded = false;
+  void release() {
+    if (h) rhllc_destroy(h);
+    h = nullptr;
+    cells_ptr = nullptr;
+    uploaded = false;
+  }
+};
+DeviceGrid g_dev;
+
+int precision_code(const std::string &s) { return s == "mixed" ? RHLLC_PREC_MIXED : RHLLC_PREC_FP64; }
+int guess_code(const std::string &s) {
+  if (s == "previous") return RHLLC_GUESS_PREVIOUS;
+  if (s == "mlp") return RHLLC_GUESS_MLP;
+  return RHLLC_GUESS_REFERENCE;
+}
+
+void ensure_device(grid_t &grid) {
+  if (g_dev.h && g_dev.cells_ptr == grid.cells.data() && g_dev.n_cells == grid.cells.size() &&
+      g_dev.n_faces == grid.faces.size())
+    return;
+  g_dev.release();
+  const size_t N = grid.cells.size(), F = grid.faces.size();
+  std::vector<uint8_t> gc(56 * N), gf(40 * F);
+  for (size_t i = 0; i < N; ++i) std::memcpy(&gc[56 * i], &grid.cells[i].geo, 56);
+  for (size_t i = 0; i < F; ++i) std::memcpy(&gf[40 * i], &grid.faces[i].geo, 40);
+  rhllc_params p;
+  std::memset(&p, 0, sizeof(p));
+  p.gamma = _hllc_cfg.gamma;
+  p.cfl = _hllc_cfg.CFL;
+  p.h_min = _hllc_cfg.h_min;
+  p.precision = precision_code(_hllc_cfg.precision);
+  p.c2p_guess = guess_code(_hllc_cfg.c2p_start);
+  p.reorder = 1;
+  p.device = 0;
+  if (rhllc_create(&p, (int64_t)N, gc.data(), (int64_t)F, gf.data(), &g_dev.h) != RHLLC_OK)
+    die("rhllc_create", rhllc_last_error());
+  if (p.c2p_guess == RHLLC_GUESS_MLP) {
+    std::vector<float> model(RHLLC_MLP_MODEL);
+    std::FILE *f = std::fopen(_hllc_cfg.mlp_model.c_str(), "rb");
+    if (!f || std::fread(model.data(), sizeof(float), model.size(), f) != model.size()) die("mlp_model", _hllc_cfg.mlp_model);
+    std::fclose(f);
+    if (rhllc_set_mlp(g_dev.h, model.data()) != RHLLC_OK) die("rhllc_set_mlp", rhllc_last_error());
+  }
+  g_dev.cells_ptr = grid.cells.data();
+  g_dev.n_cells = N;
+  g_dev.n_faces = F;
+}
+
+void download(grid_t &grid) {
+  if (rhllc_download_state(g_dev.h, grid.cells.data(), sizeof(elem_t), 0, 40) != RHLLC_OK)
+    die("rhllc_download_state", rhllc_last_error());
+  std::vector<double> F(5 * grid.faces.size());
+  if (rhllc_download_face_flux(g_dev.h, F.data()) != RHLLC_OK) die("rhllc_download_face_flux", rhllc_last_error());
+  for (size_t i = 0; i < grid.faces.size(); ++i) {
+    flux_t &f = grid.faces[i].f;
+    f.d = F[5 * i], f.v = Vector3(F[5 * i + 1], F[5 * i + 2], F[5 * i + 3]), f.p = F[5 * i + 4];
+  }
+}
+
+template <typename T>
+int read_records(const std::string &name, std::vector<T> &data, size_t rec_bytes, size_t offset) {
+  std::FILE *f = std::fopen(name.c_str(), "rb");
+  if (!f) {
+    std::fprintf(stderr, "Error open file %s\n", name.c_str());
+    return e_completion_fail;
+  }
+  int n = 0;
+  if (std::fread(&n, sizeof(int), 1, f) != 1 || n < 0) {
+    std::fclose(f);
+    return e_completion_fail;
+  }
+  data.resize(n);
+  std::vector<uint8_t> rec(rec_bytes);
+  for (int i = 0; i < n; ++i) {
+    if (std::fread(rec.data(), rec_bytes, 1, f) != 1) {
+      std::fclose(f);
+      return e_completion_fail;
+    }
+    std::memcpy(reinterpret_cast<uint8_t *>(&data[i]) + offset, rec.data(), rec_bytes);
+  }
+  std::fclose(f);
+  return e_completion_success;
+}
+
+// lib/global_consts.h:34-82 and lib/physics/include/gas_state.h:38-43
+const double kM_hydrogen = 1.6735575 * 1e-24, k_boltzmann = 1.3807 * 1e-16, kDist = 1e13, kC_Light = 3 * 1e10,
+             kGravity = 6.6726 * 1e-8, kEv = 11604525.0061657;
+const double kMass = (kC_Light * kC_Light * kDist) / kGravity;
+const double kTime = kDist / kC_Light;
+const double kDensity = kMass / (kDist * kDist * kDist);
+const double kPressure = kMass / (kDist * kTime * kTime);
+double GetPressure(double rho, double tem) {
+  const double g_idealGasConst = (kM_hydrogen / k_boltzmann);
+  const double g_idealGasConst_inv = kDensity / g_idealGasConst;
+  return (g_idealGasConst_inv / kPressure) * tem * rho;
+}
+
+}  // namespace
+
+namespace rhllc {
+
+Type max_signal_speed = 1;  // rhllc_utils.cpp:13
+
+void SetDeviceResident(bool resident) { g_dev.resident = resident; }
+void ReleaseDevice() { g_dev.release(); }
+void SyncGridFromDevice(grid_t &grid) {
+  if (g_dev.h && g_dev.uploaded) download(grid);
+}
+
+// rhllc_3d_stab.cpp:10-65 on the device.
+void Hllc3dStab(const Type tau, grid_t &grid) {
+  ensure_device(grid);
+  if (!g_dev.resident || !g_dev.uploaded) {
+    if (rhllc_upload_state(g_dev.h, grid.cells.data(), sizeof(elem_t), 0, 40) != RHLLC_OK)
+      die("rhllc_upload_state", rhllc_last_error());
+    g_dev.uploaded = true;
+  }
+  double speed = 0;
+  const int rc = rhllc_step(g_dev.h, tau, &speed);
+  if (rc == RHLLC_ERR_C2P) die("PhysPressureFix(el.conv_val, el.phys_val)", rhllc_last_error());
+  if (rc != RHLLC_OK) die("Hllc3dStab", rhllc_last_error());
+  max_signal_speed = speed;
+  if (!g_dev.resident) download(grid);
+}
+
+// rhllc_utils.cpp:15-23
+Type GetTimeStep(const hllc_value_t &hllc_set, const std::vector<elem_t> &cells) {
+  (void)cells;
+  const Type t = hllc_set.CFL * hllc_set.h_min / max_signal_speed;
+  if (t < 0) die("t < 0");
+  return t;
+}
+
+// rhllc_utils.cpp:25-33: GetConvValueStab per cell (on the device, same arithmetic).
+void HllcPhysToConv(std::vector<elem_t> &cells) {
+  const size_t n = cells.size();
+  std::vector<double> W(5 * n), U(5 * n);
+  for (size_t i = 0; i < n; ++i) std::memcpy(&W[5 * i], &cells[i].phys_val, 40);
+  if (rhllc_eval_p2c(_hllc_cfg.gamma, (int64_t)n, W.data(), U.data()) != RHLLC_OK)
+    die("HllcPhysToConv", rhllc_last_error());
+  for (size_t i = 0; i < n; ++i) {
+    std::memcpy(static_cast<void *>(&cells[i].phys_val), &W[5 * i], 40);
+    std::memcpy(static_cast<void *>(&cells[i].conv_val), &U[5 * i], 40);
+  }
+}
+
+// rhllc_init.cpp:180-194.  Without an init file the reference's default for
+// GEOMETRY_TYPE Cone (rhllc_init.cpp:129-132) is used; like the reference it
+// requires <base_address>centers.bin to exist.
+int Init(std::string &file_init_value, std::vector<elem_t> &cells) {
+  if (files_sys::bin::ReadHllcInit(file_init_value, cells) != e_completion_success) {
+    std::FILE *f = std::fopen((glb_files.base_address + "centers.bin").c_str(), "rb");
+    if (!f) {
+      std::fprintf(stderr, "Default rhllc value not set\n");
+      return e_completion_fail;
+    }
+    std::fclose(f);
+    for (auto &el : cells) {
+      el.phys_val.d = kM_hydrogen * 1e14 / kDensity;
+      el.phys_val.p = GetPressure(el.phys_val.d, kEv);
+      el.phys_val.v = Vector3(0, 0, 0);
+    }
+  }
+  HllcPhysToConv(cells);
+  max_signal_speed = 1;
+  _hllc_cfg.tau = GetTimeStep(_hllc_cfg, cells);
+  return e_completion_success;
+}
+
+// rhllc_main.cpp:15-62.  The state stays on the device between steps and is
+// brought back only to write a solution.
+int RunRhllcModule() {
+  grid_t grid;
+  uint32_t err = 0;
+  err |= files_sys::bin::ReadGridGeo(glb_files.name_file_geometry_faces, grid.faces);
+  err |= files_sys::bin::ReadGridGeo(glb_files.name_file_geometry_cells, grid.cells);
+  if (err) {
+    std::fprintf(stderr, "Error reading \n");
+    return e_completion_fail;
+  }
+  grid.size = (IdType)grid.cells.size();
+  if (rhllc::Init(glb_files.hllc_init_value, grid.cells)) die("rhllc::Init(glb_files.hllc_init_value, grid.cells)");
+  Type t = 0.0;
+  Type cur_timer = 0;
+  int res_count = _solve_mode.start_point;
+  _hllc_cfg.tau = GetTimeStep(_hllc_cfg, grid.cells);
+  files_sys::bin::WriteSolution(glb_files.solve_address + std::to_string(res_count++), grid);
+  const bool was_resident = g_dev.resident;
+  SetDeviceResident(true);
+  while (t < _hllc_cfg.T) {
+    Hllc3dStab(_hllc_cfg.tau, grid);
+    t += _hllc_cfg.tau;
+    cur_timer += _hllc_cfg.tau;
+    if (cur_timer >= _hllc_cfg.save_timer) {
+      SyncGridFromDevice(grid);
+      if (files_sys::bin::WriteSolution(glb_files.solve_address + std::to_string(res_count++), grid) !=
+          e_completion_success)
+        die("WriteSolution");
+      cur_timer = 0;
+    }
+    _hllc_cfg.tau = GetTimeStep(_hllc_cfg, grid.cells);
+  }
+  SyncGridFromDevice(grid);
+  files_sys::bin::WriteSolution(glb_files.solve_address + std::to_string(res_count++), grid);
+  SetDeviceResident(was_resident);
+  ReleaseDevice();
+  return e_completion_success;
+}
+
+}  // namespace rhllc
+
+namespace files_sys {
+namespace json {
+
+namespace {
+int load(const std::string &file, std::map<std::string, jsonmin::Value> &kv) {
+  std::ifstream in(file);
+  if (!in) {
+    std::fprintf(stderr, "Error open file %s\n", file.c_str());
+    return e_completion_fail;
+  }
+  std::stringstream ss;
+  ss << in.rdbuf();
+  std::string err;
+  if (!jsonmin::parse_object(ss.str(), kv, err)) {
+    std::fprintf(stderr, "Error : %s: %s\n", file.c_str(), err.c_str());
+    return e_completion_fail;
+  }
+  return e_completion_success;
+}
+// j.at(name) throws on a missing key in the reference (json_struct.h:33,62,167).
+template <typename T>
+bool need(const std::map<std::string, jsonmin::Value> &kv, const char *name, T &out, const std::string &file) {
+  auto it = kv.find(name);
+  if (it == kv.end()) {
+    std::fprintf(stderr, "Error : key '%s' missing in %s\n", name, file.c_str());
+    return false;
+  }
+  return jsonmin::get(it->second, out);
+}
+template <typename T>
+void opt(const std::map<std::string, jsonmin::Value> &kv, const char *name, T &out) {
+  auto it = kv.find(name);
+  if (it != kv.end()) jsonmin::get(it->second, out);
+}
+}  // namespace
+
+int Read(const std::string &file, global_files_t &st) {
+  std::map<std::string, jsonmin::Value> kv;
+  if (load(file, kv)) return e_completion_fail;
+  bool ok = need(kv, "name_file_vtk", st.name_file_vtk, file) &&
+            need(kv, "name_file_sphere_direction", st.name_file_sphere_direction, file) &&
+            need(kv, "base_address", st.base_address, file) &&
+            need(kv, "illum_geo_address", st.illum_geo_address, file) &&
+            need(kv, "graph_address", st.graph_address, file) && need(kv, "trace_address", st.trace_address, file) &&
+            need(kv, "solve_address", st.solve_address, file) && need(kv, "add_dir_address", st.add_dir_address, file) &&
+            need(kv, "tab_func_address", st.tab_func_address, file) &&
+            need(kv, "name_file_hllc_set", st.name_file_hllc_set, file) &&
+            need(kv, "hllc_init_value", st.hllc_init_value, file) &&
+            need(kv, "solve_configuration", st.solve_configuration, file);
+  return ok ? e_completion_success : e_completion_fail;
+}
+
+int Read(const std::string &file, solve_mode_t &st) {
+  std::map<std::string, jsonmin::Value> kv;
+  if (load(file, kv)) return e_completion_fail;
+  bool ok = need(kv, "class_vtk", st.class_vtk, file) && need(kv, "max_number_of_iter", st.max_number_of_iter, file) &&
+            need(kv, "accuracy", st.accuracy, file) && need(kv, "start_point", st.start_point, file);
+  return ok ? e_completion_success : e_completion_fail;
+}
+
+int Read(const std::string &file, hllc_value_t &st) {
+  std::map<std::string, jsonmin::Value> kv;
+  if (load(file, kv)) return e_completion_fail;
+  bool ok = need(kv, "T", st.T, file) && need(kv, "CFL", st.CFL, file) && need(kv, "h_min", st.h_min, file) &&
+            need(kv, "save_timer", st.save_timer, file) && need(kv, "tau", st.tau, file);
+  opt(kv, "gamma", st.gamma);
+  opt(kv, "precision", st.precision);
+  opt(kv, "c2p_start", st.c2p_start);
+  opt(kv, "mlp_model", st.mlp_model);
+  return ok ? e_completion_success : e_completion_fail;
+}
+
+// lib/files_sys/include/reader_json.h:63-89
+int ReadStartSettings(const std::string &file_set, global_files_t &glb, solve_mode_t *solve_mode,
+                      hllc_value_t *hllc_conf) {
+  if (Read(file_set, glb)) return e_completion_fail;
+  glb.name_file_settings = file_set;
+  glb.Build();
+  if (solve_mode != nullptr && Read(glb.solve_configuration, *solve_mode)) return e_completion_fail;
+  if (hllc_conf != nullptr && Read(glb.name_file_hllc_set, *hllc_conf)) return e_completion_fail;
+  return e_completion_success;
+}
+
+}  // namespace json
+
+namespace bin {
+
+int ReadGridGeo(const std::string &name_file, std::vector<face_t> &faces) {
+  return read_records(name_file, faces, 40, offsetof(face_t, geo));
+}
+int ReadGridGeo(const std::string &name_file, std::vector<elem_t> &cells) {
+  return read_records(name_file, cells, 56, offsetof(elem_t, geo));
+}
+int ReadHllcInit(const std::string &file_init_value, std::vector<elem_t> &cells) {
+  return read_records(file_init_value, cells, 40, offsetof(elem_t, phys_val));
+}
+
+// lib/files_sys/src/writer_bin.cpp:104-112 (WRITE_FILE_ELEM of writer_bin.h:24-35)
+int WriteSolution(const std::string &main_dir, const grid_t &grid) {
+  auto write = [&](const std::string &name, size_t off, size_t bytes) {
+    std::FILE *f = std::fopen((main_dir + name).c_str(), "wb");
+    if (!f) return false;
+    const int n = (int)grid.cells.size();
+    bool ok = std::fwrite(&n, sizeof(int), 1, f) == 1;
+    for (const auto &el : grid.cells) ok = ok && std::fwrite(reinterpret_cast<const uint8_t *>(&el) + off, bytes, 1, f) == 1;
+    return (std::fclose(f) == 0) && ok;
+  };
+  const size_t pv = offsetof(elem_t, phys_val);
+  bool ok = write("density.bin", pv + offsetof(flux_t, d), sizeof(Type)) &&
+            write("pressure.bin", pv + offsetof(flux_t, p), sizeof(Type)) &&
+            write("velocity.bin", pv + offsetof(flux_t, v), sizeof(Vector3));
+  return ok ? e_completion_success : e_completion_fail;
+}
+
+}  // namespace bin
+}  // namespace files_sys

@@ -1,0 +1,52 @@
+"""The drop-in executable (rhllc_solver, the reference's src/RHD/rhllc_solver.cpp)
+on a generated case: configs, geometry and init files in the reference's formats;
+the written solution must equal the oracle's state after the same time loop."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from radrhd_solver_amd import mesh as M
+from radrhd_solver_amd.build import SOLVER_EXE
+
+pytestmark = pytest.mark.gpu
+SEED = 20261202
+
+
+def _oracle_loop(m, W0, cfl, h, T):
+    """rhllc_main.cpp:34-56: while t < T {step(tau); t += tau; tau = CFL h / speed}."""
+    st = O.State(m, W0)
+    st.run(cfl, h, 0)
+    t, tau, n = 0.0, cfl * h / 1.0, 0
+    while t < T:
+        speed = st.step(tau)
+        t += tau
+        tau = cfl * h / speed
+        n += 1
+    return st, n
+
+
+def _read(path, width):
+    with open(path, "rb") as f:
+        n = int(np.frombuffer(f.read(4), np.int32)[0])
+        return np.frombuffer(f.read(8 * n * width), np.float64).reshape(n, width) if width > 1 else np.frombuffer(f.read(8 * n), np.float64)
+
+
+def test_rhllc_solver_executable_matches_oracle(tmp_path):
+    m = M.kuhn_box(24, 3, 3, length=(1.0, 0.125, 0.125), seed=SEED)
+    W0 = M.init_state(m, M.INIT_SHOCK_TUBE)
+    h = m.h_min()
+    T = 60 * 0.4 * h
+    dirs = M.write_case(str(tmp_path), m, W0, T=T, CFL=0.4, save_timer=1e9)
+    st, nsteps = _oracle_loop(m, W0, 0.4, h, T)
+    assert nsteps > 20
+    r = subprocess.run([SOLVER_EXE, str(tmp_path / "directories_cfg.json")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    # Solve0 = initial state, Solve1 = final state (save_timer larger than T)
+    base = dirs["solve_address"] + "1"
+    rho, p, v = _read(base + "density.bin", 1), _read(base + "pressure.bin", 1), _read(base + "velocity.bin", 3)
+    assert np.array_equal(rho, st.W[:, 0]) and np.array_equal(p, st.W[:, 4]) and np.array_equal(v, st.W[:, 1:4])
+    rho0 = _read(dirs["solve_address"] + "0density.bin", 1)
+    assert np.array_equal(rho0, W0[:, 0])
