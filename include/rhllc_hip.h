@@ -133,12 +133,13 @@ int rhllc_time_kernels(rhllc_solver *h, int32_t nsteps, double out[4]);
 int rhllc_info(rhllc_solver *h, int64_t *n_cells, int64_t *n_faces, int64_t *device_bytes);
 
 /* Pointwise device evaluations over host arrays (n states, flux_t layout),
- * for per-function parity checks.  precision selects the FP64 or FP32 path
- * of the flux; nS4 holds (nx, ny, nz, S) per face. */
+ * for per-function parity checks.  precision selects the FP64 (reference
+ * arithmetic) or the mixed path's arithmetic of the flux and of c2p; nS4
+ * holds (nx, ny, nz, S) per face. */
 int rhllc_eval_flux(int32_t precision, double gamma, int64_t n, const double *UL, const double *UR,
                     const double *WL, const double *WR, const double *nS4, double *F, double *speed);
-int rhllc_eval_c2p(double gamma, int64_t n, const double *U, const double *guess, double *W, int32_t *codes,
-                   int32_t *iters);
+int rhllc_eval_c2p(int32_t precision, double gamma, int64_t n, const double *U, const double *guess, double *W,
+                   int32_t *codes, int32_t *iters);
 int rhllc_eval_p2c(double gamma, int64_t n, double *W, double *U);
 int rhllc_eval_pressure_fix(double gamma, int64_t n, double *U, double *W, int32_t *codes);
 int rhllc_eval_rad_source(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,

@@ -94,7 +94,7 @@ SIGNATURES = {
     "rhllc_time_kernels": (C.c_int, [_vp, _i32, _vp]),
     "rhllc_info": (C.c_int, [_vp, _vp, _vp, _vp]),
     "rhllc_eval_flux": (C.c_int, [_i32, _dbl, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "rhllc_eval_c2p": (C.c_int, [_dbl, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "rhllc_eval_c2p": (C.c_int, [_i32, _dbl, _i64, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_eval_p2c": (C.c_int, [_dbl, _i64, _vp, _vp]),
     "rhllc_eval_pressure_fix": (C.c_int, [_dbl, _i64, _vp, _vp, _vp]),
     "rhllc_eval_rad_source": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -247,14 +247,14 @@ def eval_flux(UL, UR, WL, WR, nS4, gamma=5.0 / 3.0, precision=PREC_FP64):
     return F, sp
 
 
-def eval_c2p(U, gamma=5.0 / 3.0, guess=None, W_init=None):
+def eval_c2p(U, gamma=5.0 / 3.0, guess=None, W_init=None, precision=PREC_FP64):
     U = np.ascontiguousarray(U, dtype=np.float64).reshape(-1, 5)
     n = U.shape[0]
     W = np.zeros((n, 5)) if W_init is None else np.array(W_init, dtype=np.float64, order="C").reshape(n, 5)
     g = None if guess is None else np.ascontiguousarray(guess, dtype=np.float64)
     codes = np.zeros(n, np.int32)
     iters = np.zeros(n, np.int32)
-    check(_lib().rhllc_eval_c2p(gamma, n, U.ctypes.data, _ptr(g), W.ctypes.data, codes.ctypes.data,
+    check(_lib().rhllc_eval_c2p(precision, gamma, n, U.ctypes.data, _ptr(g), W.ctypes.data, codes.ctypes.data,
                                 iters.ctypes.data))
     return W, codes, iters
 

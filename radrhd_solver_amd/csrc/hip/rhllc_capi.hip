@@ -613,16 +613,17 @@ extern "C" int rhllc_eval_flux(int32_t precision, double gamma, int64_t n, const
   return RHLLC_OK;
 }
 
-extern "C" int rhllc_eval_c2p(double gamma, int64_t n, const double *U, const double *guess, double *W, int32_t *codes,
-                              int32_t *iters) {
+extern "C" int rhllc_eval_c2p(int32_t precision, double gamma, int64_t n, const double *U, const double *guess,
+                              double *W, int32_t *codes, int32_t *iters) {
   if (n < 0 || !U || !W || !codes || !iters) return fail(RHLLC_ERR_ARG, "bad argument");
+  if (precision != RHLLC_PREC_FP64 && precision != RHLLC_PREC_MIXED) return fail(RHLLC_ERR_ARG, "bad precision");
   DevBuf a, b, c, dd, e;
   HIP_TRY(to_dev(a, U, sizeof(double) * 5 * n));
   HIP_TRY(to_dev(c, W, sizeof(double) * 5 * n));
   if (guess) HIP_TRY(to_dev(b, guess, sizeof(double) * n));
   HIP_TRY(to_dev(dd, nullptr, sizeof(int32_t) * n));
   HIP_TRY(to_dev(e, nullptr, sizeof(int32_t) * n));
-  HIP_TRY(launch_eval_c2p(gamma, n, (double *)a.p, (double *)b.p, (double *)c.p, (int32_t *)dd.p, (int32_t *)e.p,
+  HIP_TRY(launch_eval_c2p(precision == RHLLC_PREC_MIXED, gamma, n, (double *)a.p, (double *)b.p, (double *)c.p, (int32_t *)dd.p, (int32_t *)e.p,
                           nullptr));
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(W, c.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost));

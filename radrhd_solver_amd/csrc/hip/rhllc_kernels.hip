@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kBlock) void k_face(Dev d) {
       // FP32 storage (state, geometry, flux), FP64 arithmetic with the fast
       // divide/sqrt policy; U is rebuilt from the FP32 primitives.
       const float4 nS = d.face_nS32[i];
-      State<double> UL, WL, UR, WR, F;
+      State<double> WL, WR, F;
       {
         const float4 a = d.W4f[lr.x];
         WL.d = a.x, WL.v[0] = a.y, WL.v[1] = a.z, WL.v[2] = a.w, WL.p = d.Wpf[lr.x];
@@ -107,10 +107,7 @@ __global__ __launch_bounds__(kBlock) void k_face(Dev d) {
       } else {
         WR.d = d.ghostW[0], WR.v[0] = d.ghostW[1], WR.v[1] = d.ghostW[2], WR.v[2] = d.ghostW[3], WR.p = d.ghostW[4];
       }
-      p2c<double, Fast>(d.Gg, WL, UL);
-      p2c<double, Fast>(d.Gg, WR, UR);
-      const double s = hllc_flux<double, Fast>(d.gamma, d.Gg, UL, UR, WL, WR, (double)nS.x, (double)nS.y,
-                                               (double)nS.z, (double)nS.w, F);
+      const double s = mixed_face_flux(d.gamma, d.Gg, WL, WR, nS.x, nS.y, nS.z, nS.w, F);
       speed = smax(speed, s);
       d.F4f[i] = make_float4((float)F.d, (float)F.v[0], (float)F.v[1], (float)F.v[2]);
       d.FEf[i] = (float)F.p;
@@ -193,17 +190,17 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
   if (VARIANT == kMixedMlp) guess = mlp_guess(d, live, U);
   if (live) {
     // c2p writes W only when it succeeds.
-    code = MIXED ? c2p_fast(d.Gg, U, W, guess, iters) : c2p(d.Gg, U, W, guess, iters);
+    code = MIXED ? c2p<Fast>(d.Gg, U, W, guess, iters) : c2p(d.Gg, U, W, guess, iters);
     if (code && VARIANT == kMixedMlp) {
       int it2 = 0;
       fell_back = true;
-      code = c2p_fast(d.Gg, U, W, (double)d.Wpf[c], it2);
+      code = c2p<Fast>(d.Gg, U, W, (double)d.Wpf[c], it2);
       iters += it2;
     }
     if (code && VARIANT != kFp64Ref) {
       int it2 = 0;
       fell_back = true;
-      code = MIXED ? c2p_fast(d.Gg, U, W, -1.0, it2) : c2p(d.Gg, U, W, -1.0, it2);
+      code = MIXED ? c2p<Fast>(d.Gg, U, W, -1.0, it2) : c2p(d.Gg, U, W, -1.0, it2);
       iters += it2;
     }
     if (code) {
@@ -231,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
       U.v[2] += ds * tau * G[3];
       int it3 = 0;
       if (MIXED)
-        c2p_fast(d.Gg, U, W, W.p, it3);
+        c2p<Fast>(d.Gg, U, W, W.p, it3);
       else
         c2p(d.Gg, U, W, -1.0, it3);
       iters += it3;
@@ -440,19 +437,17 @@ __global__ void k_eval_flux(double gamma, int64_t n, const double *UL, const dou
   } else {
     // The mixed face path: FP32 primitives and geometry, FP64 fast arithmetic,
     // flux rounded to FP32.
-    State<double> a, b, ua, ub, ff;
+    State<double> a, b, ff;
     a.d = (float)wl.d, a.v[0] = (float)wl.v[0], a.v[1] = (float)wl.v[1], a.v[2] = (float)wl.v[2], a.p = (float)wl.p;
     b.d = (float)wr.d, b.v[0] = (float)wr.v[0], b.v[1] = (float)wr.v[1], b.v[2] = (float)wr.v[2], b.p = (float)wr.p;
-    p2c<double, Fast>(Gg, a, ua);
-    p2c<double, Fast>(Gg, b, ub);
-    speed[i] = hllc_flux<double, Fast>(gamma, Gg, ua, ub, a, b, (double)(float)nS4[4 * i], (double)(float)nS4[4 * i + 1],
-                                       (double)(float)nS4[4 * i + 2], (double)(float)nS4[4 * i + 3], ff);
+    speed[i] = mixed_face_flux(gamma, Gg, a, b, (float)nS4[4 * i], (float)nS4[4 * i + 1], (float)nS4[4 * i + 2],
+                               (float)nS4[4 * i + 3], ff);
     double *o = F + 5 * i;
     o[0] = (float)ff.d, o[1] = (float)ff.v[0], o[2] = (float)ff.v[1], o[3] = (float)ff.v[2], o[4] = (float)ff.p;
   }
 }
 
-__global__ void k_eval_c2p(double gamma, int64_t n, const double *U, const double *guess, double *W,
+__global__ void k_eval_c2p(bool mixed, double gamma, int64_t n, const double *U, const double *guess, double *W,
                            int32_t *codes, int32_t *iters) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -460,7 +455,8 @@ __global__ void k_eval_c2p(double gamma, int64_t n, const double *U, const doubl
   ld5(U + 5 * i, u);
   ld5(W + 5 * i, w);
   int it = 0;
-  codes[i] = c2p(gamma / (gamma - 1.0), u, w, guess ? guess[i] : -1.0, it);
+  const double Gg = gamma / (gamma - 1.0), g0 = guess ? guess[i] : -1.0;
+  codes[i] = mixed ? c2p<Fast>(Gg, u, w, g0, it) : c2p(Gg, u, w, g0, it);
   iters[i] = it;
   st5(W + 5 * i, w);
 }
@@ -517,10 +513,10 @@ hipError_t launch_eval_flux(bool fp32, double gamma, int64_t n, const double *UL
     hipLaunchKernelGGL(k_eval_flux<false>, eval_grid(n), dim3(256), 0, s, gamma, n, UL, UR, WL, WR, nS4, F, speed);
   return hipGetLastError();
 }
-hipError_t launch_eval_c2p(double gamma, int64_t n, const double *U, const double *guess, double *W, int32_t *codes,
-                           int32_t *iters, hipStream_t s) {
+hipError_t launch_eval_c2p(bool mixed, double gamma, int64_t n, const double *U, const double *guess, double *W,
+                           int32_t *codes, int32_t *iters, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_eval_c2p, eval_grid(n), dim3(256), 0, s, gamma, n, U, guess, W, codes, iters);
+  hipLaunchKernelGGL(k_eval_c2p, eval_grid(n), dim3(256), 0, s, mixed, gamma, n, U, guess, W, codes, iters);
   return hipGetLastError();
 }
 hipError_t launch_eval_p2c(double gamma, int64_t n, double *W, double *U, hipStream_t s) {

@@ -75,6 +75,11 @@ struct Fast {
     const double s = x * y;
     return fma(fma(-s, s, x), 0.5 * y, s);
   }
+  __device__ static __forceinline__ double rsqrt(double x) {  // x > 0
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-0.5 * x * y, y, 1.5);
+    return y * fma(-0.5 * x * y, y, 1.5);
+  }
 };
 
 template <typename R>
@@ -193,19 +198,25 @@ __device__ __forceinline__ int pressure_fix(double Gg, State<double> &U, State<d
 // max(|m| - E, 0, pmin); otherwise the start is max(guess, pmin).
 // Returns 0 or the reference's codes 1..4; 5 = the reference's DIE_IF(lor2 < 1).
 // On failure W is left unchanged.  *iters = Newton iterations done.
+// A = Ieee is the reference's arithmetic; the mixed path runs the same
+// operation sequence with the Fast divide/sqrt (a few ulp per operation).
+// The per-iteration form 1 / (1 - m^2/alpha^2) is kept on purpose: it is
+// exactly 1 for m = 0, where rearranged forms round to 1 - ulp half the time
+// and would trip the lor2 < 1 test on gas at rest.
+template <typename A = Ieee>
 __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<double> &W, double guess, int &iters) {
   const double tol = 1.e-11;
   const double eps2 = 1.e-12;
   const double D = U.d;
   const double E = U.p;
   const double m2 = dot3(U.v, U.v);
-  const double Q = E - sqrt(m2 + D * D);
+  const double Q = E - A::sqrt_(m2 + D * D);
   iters = 0;
   if (Q < 0.0) return 1;
-  const double m = sqrt(m2);
+  const double m = A::sqrt_(m2);
   double p = smax(m - E, 0.0);
-  const double D_1 = 1.0 / D;
-  const double pmin = sqrt(m2 / (1.0 - eps2)) - E;
+  const double D_1 = A::rcp(D);
+  const double pmin = A::sqrt_(A::div(m2, 1.0 - eps2)) - E;
   if (guess >= 0.0) p = guess;
   p = smax(p, pmin);
   double alpha, alpha2, lor2, lor, tau, theta, h, dh_dp, dh_dtau, yp = 0.0, dp, dyp;
@@ -213,21 +224,21 @@ __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<doub
   for (iter = 0; iter < kMaxIter; iter++) {
     alpha = E + p;
     alpha2 = alpha * alpha;
-    lor2 = 1.0 - m2 / alpha2;
-    lor2 = 1.0 / lor2;
+    lor2 = 1.0 - A::div(m2, alpha2);
+    lor2 = A::rcp(lor2);
     if (lor2 < 1.0) {
       iters = iter + 1;
       return 5;
     }
-    lor = sqrt(lor2);
+    lor = A::sqrt_(lor2);
     tau = lor * D_1;
     theta = p * tau;
     h = 1.0 + Gg * theta;
     dh_dp = Gg * tau;
     dh_dtau = Gg * p;
     yp = D * h * lor - E - p;
-    dyp = D * lor * dh_dp - m2 * lor2 * lor / (alpha2 * alpha) * (lor * dh_dtau + D * h) - 1.0;
-    dp = yp / dyp;
+    dyp = D * lor * dh_dp - A::div(m2 * lor2 * lor, alpha2 * alpha) * (lor * dh_dtau + D * h) - 1.0;
+    dp = A::div(yp, dyp);
     p -= dp;
     if (p < pmin) p = pmin;
     if (fabs(dp) < tol * E) break;
@@ -235,16 +246,16 @@ __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<doub
   iters = iter < kMaxIter ? iter + 1 : kMaxIter;
   if (p < 0.0) return 2;
   if (isnan(p)) return 4;
-  if (iter >= kMaxIter || fabs(yp / (E + p)) > 1.e-4 || p < (m - E)) return 3;
+  if (iter >= kMaxIter || fabs(A::div(yp, E + p)) > 1.e-4 || p < (m - E)) return 3;
   alpha = E + p;
   alpha2 = alpha * alpha;
-  lor2 = alpha2 / (alpha2 - m2);
-  lor = sqrt(lor2);
+  lor2 = A::div(alpha2, alpha2 - m2);
+  lor = A::sqrt_(lor2);
   tau = lor * D_1;
-  W.d = 1.0 / tau;
+  W.d = A::rcp(tau);
   W.p = p;
   const double den = U.p + p;
-  W.v[0] = U.v[0] / den, W.v[1] = U.v[1] / den, W.v[2] = U.v[2] / den;
+  W.v[0] = A::div(U.v[0], den), W.v[1] = A::div(U.v[1], den), W.v[2] = A::div(U.v[2], den);
   return 0;
 }
 
@@ -347,6 +358,23 @@ __device__ __forceinline__ R hllc_flux(R G1, R Gg, const State<R> &cUL, const St
   return smax(lL, lR);
 }
 
+// Face flux of the mixed path: FP32 primitives and geometry in, FP64
+// arithmetic with the Fast policy.  U is rebuilt from the FP32 primitives.
+// The FP32 normal is renormalised first: rounding leaves |n|^2 = 1 +- 1e-7,
+// and for a normal within ~1e-3 rad of the z axis the rotation's second row,
+// scaled by 1/sqrt(1 - n2^2), would then be off unit length by percents --
+// enough to push the rotated |v| of a fast flow past 1.
+__device__ __forceinline__ double mixed_face_flux(double gamma, double Gg, State<double> &WL, State<double> &WR,
+                                                  float nx, float ny, float nz, float S, State<double> &F) {
+  double n0 = nx, n1 = ny, n2 = nz;
+  const double r = Fast::rsqrt((n0 * n0 + n1 * n1) + n2 * n2);
+  n0 *= r, n1 *= r, n2 *= r;
+  State<double> UL, UR;
+  p2c<double, Fast>(Gg, WL, UL);
+  p2c<double, Fast>(Gg, WR, UR);
+  return hllc_flux<double, Fast>(gamma, Gg, UL, UR, WL, WR, n0, n1, n2, (double)S, F);
+}
+
 // Radiation 4-momentum source, solvers/RadRHD/src/radRHD_utils.cpp:8-101 with
 // the M1 Eddington tensor (ISOTROPIC_INTENSITY == 1 branch); the thermal
 // emission term is omitted (q = 0).  EddTensor(a, b) is evaluated once per
@@ -417,58 +445,6 @@ __device__ __forceinline__ void rad_source(const State<double> &W, double Ur, co
   for (int i = 0; i < 3; i++)
     G[i + 1] = q * u[i] - rho * (tot_op * uD[i] + scat_op * u[i] * (gamma2 + uuD)) * Ur +
                rhogamma * (tot_op * Fr[i] + scat_op * 2.0 * uF * u[i]);
-}
-
-// c2p of the mixed path: the reference's Newton iteration on
-// f(p) = D h Gamma - E - p (same start rule, same stopping test |dp| < 1e-11 E,
-// same failure codes) with the per-iteration algebra rearranged to two
-// divisions instead of four:  with q = alpha^2 - m^2 and r = 1/(alpha q),
-// Gamma^2 = alpha^3 r and m^2 Gamma^3 / alpha^3 = m^2 Gamma r.
-__device__ __forceinline__ int c2p_fast(double Gg, const State<double> &U, State<double> &W, double guess,
-                                        int &iters) {
-  using A = Fast;
-  const double tol = 1.e-11;
-  const double inv_1_eps2 = 1.0 / (1.0 - 1.e-12);
-  const double D = U.d;
-  const double E = U.p;
-  const double m2 = dot3(U.v, U.v);
-  iters = 0;
-  if (E - A::sqrt_(m2 + D * D) < 0.0) return 1;
-  const double m = A::sqrt_(m2);
-  const double D_1 = A::rcp(D);
-  const double pmin = A::sqrt_(m2 * inv_1_eps2) - E;
-  double p = guess >= 0.0 ? guess : smax(m - E, 0.0);
-  p = smax(p, pmin);
-  double yp = 0.0, lor = 1.0;
-  int iter;
-  for (iter = 0; iter < kMaxIter; iter++) {
-    const double alpha = E + p;
-    const double a2 = alpha * alpha;
-    const double r = A::rcp(alpha * (a2 - m2));
-    const double lor2 = a2 * alpha * r;
-    if (!(lor2 >= 1.0)) {
-      iters = iter + 1;
-      return 5;
-    }
-    lor = A::sqrt_(lor2);
-    const double tau = lor * D_1;
-    const double h = 1.0 + Gg * (p * tau);
-    yp = D * h * lor - E - p;
-    const double dyp = D * lor * (Gg * tau) - m2 * lor * r * (lor * (Gg * p) + D * h) - 1.0;
-    const double dp = A::div(yp, dyp);
-    p -= dp;
-    if (p < pmin) p = pmin;
-    if (fabs(dp) < tol * E) break;
-  }
-  iters = iter < kMaxIter ? iter + 1 : kMaxIter;
-  if (p < 0.0) return 2;
-  if (isnan(p)) return 4;
-  if (iter >= kMaxIter || fabs(yp / (E + p)) > 1.e-4 || p < (m - E)) return 3;
-  const double ia = A::rcp(E + p);
-  W.d = D * A::sqrt_(1.0 - m2 * ia * ia);
-  W.p = p;
-  W.v[0] = U.v[0] * ia, W.v[1] = U.v[1] * ia, W.v[2] = U.v[2] * ia;
-  return 0;
 }
 
 }  // namespace rhllc_dev
