@@ -15,6 +15,7 @@
 #include "../mesh/mesh_common.h"
 #include "mlp_internal.h"
 #include "rhllc_internal.h"
+#include "flux_ext.h"
 
 using namespace rhllc_dev;
 
@@ -244,7 +245,7 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
     HIP_TRY(hipMemcpy(q, nS32.data(), sizeof(float4) * F, hipMemcpyHostToDevice));
     d.face_nS32 = q;
     HIP_TRY(h->alloc(&d.F4f, F));
-    HIP_TRY(h->alloc(&d.FEf, F));
+    HIP_TRY(h->alloc(&d.FXf, F));
   } else {
     double4 *q;
     HIP_TRY(h->alloc(&q, F));
@@ -474,13 +475,10 @@ extern "C" int rhllc_download_face_flux(rhllc_solver *h, double *F) {
     }
   } else {
     std::vector<float4> a(nf);
-    std::vector<float> e(nf);
+    std::vector<uint2> e(nf);
     HIP_TRY(hipMemcpy(a.data(), h->d.F4f, sizeof(float4) * nf, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(e.data(), h->d.FEf, sizeof(float) * nf, hipMemcpyDeviceToHost));
-    for (int32_t i = 0; i < nf; ++i) {
-      double *o = F + 5 * (size_t)h->face_new_to_old[i];
-      o[0] = a[i].x, o[1] = a[i].y, o[2] = a[i].z, o[3] = a[i].w, o[4] = e[i];
-    }
+    HIP_TRY(hipMemcpy(e.data(), h->d.FXf, sizeof(uint2) * nf, hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < nf; ++i) rhllc_ext::unpack(a[i], e[i], F + 5 * (size_t)h->face_new_to_old[i]);
   }
   return RHLLC_OK;
 }
@@ -572,7 +570,7 @@ extern "C" int rhllc_info(rhllc_solver *h, int64_t *n_cells, int64_t *n_faces, i
     const int64_t N = h->n_cells, F = h->n_faces;
     // Algorithmic bytes of one step (both kernels), each array element counted
     // once per kernel that touches it.
-    *device_bytes = h->mixed ? 148 * N + 64 * F : 233 * N + 120 * F;
+    *device_bytes = h->mixed ? 148 * N + 72 * F : 233 * N + 120 * F;
     if (h->rad) *device_bytes += 24 * N;
   }
   return RHLLC_OK;

@@ -37,8 +37,8 @@ struct Dev {
   const float4 *face_nS32;  // (nx, ny, nz, S), FP32 path
   double4 *F4;              // flux (d, mx, my, mz) * S, FP64 path
   double *FE;               // flux E * S
-  float4 *F4f;              // mixed path: flux (d, mx, my, mz) * S rounded to FP32
-  float *FEf;               // mixed path: flux E * S
+  float4 *F4f;              // mixed path: flux (d, mx, my, mz) * S, FP32 parts (flux_ext.h)
+  uint2 *FXf;               // mixed path: bits of flux E * S, extension bytes
   // cells
   const int4 *cell_faces;  // face index | 0x80000000 when the cell is id_r (sign bit 0)
   const double *cell_V;

@@ -11,6 +11,7 @@
 // No float atomics: face fluxes are written once and read by a fixed-order
 // gather, so the step is deterministic; the speed maximum is order-free.
 #include "rhllc_internal.h"
+#include "flux_ext.h"
 #include "mlp.h"
 #include "rhllc_physics.h"
 
@@ -109,8 +110,9 @@ __global__ __launch_bounds__(kBlock) void k_face(Dev d) {
       }
       const double s = mixed_face_flux(d.gamma, d.Gg, WL, WR, nS.x, nS.y, nS.z, nS.w, F);
       speed = smax(speed, s);
-      d.F4f[i] = make_float4((float)F.d, (float)F.v[0], (float)F.v[1], (float)F.v[2]);
-      d.FEf[i] = (float)F.p;
+      const rhllc_ext::Packed pk = rhllc_ext::pack(F.d, F.v[0], F.v[1], F.v[2], F.p);
+      d.F4f[i] = pk.a;
+      d.FXf[i] = pk.b;
     }
   }
   fold_speed(d, speed);
@@ -122,8 +124,9 @@ __device__ __forceinline__ void load_flux(const Dev &d, int f, State<double> &F)
     const double4 a = d.F4[f];
     F.d = a.x, F.v[0] = a.y, F.v[1] = a.z, F.v[2] = a.w, F.p = d.FE[f];
   } else {
-    const float4 a = d.F4f[f];
-    F.d = a.x, F.v[0] = a.y, F.v[1] = a.z, F.v[2] = a.w, F.p = d.FEf[f];
+    double o[5];
+    rhllc_ext::unpack(d.F4f[f], d.FXf[f], o);
+    F.d = o[0], F.v[0] = o[1], F.v[1] = o[2], F.v[2] = o[3], F.p = o[4];
   }
 }
 
@@ -442,8 +445,8 @@ __global__ void k_eval_flux(double gamma, int64_t n, const double *UL, const dou
     b.d = (float)wr.d, b.v[0] = (float)wr.v[0], b.v[1] = (float)wr.v[1], b.v[2] = (float)wr.v[2], b.p = (float)wr.p;
     speed[i] = mixed_face_flux(gamma, Gg, a, b, (float)nS4[4 * i], (float)nS4[4 * i + 1], (float)nS4[4 * i + 2],
                                (float)nS4[4 * i + 3], ff);
-    double *o = F + 5 * i;
-    o[0] = (float)ff.d, o[1] = (float)ff.v[0], o[2] = (float)ff.v[1], o[3] = (float)ff.v[2], o[4] = (float)ff.p;
+    const rhllc_ext::Packed pk = rhllc_ext::pack(ff.d, ff.v[0], ff.v[1], ff.v[2], ff.p);
+    rhllc_ext::unpack(pk.a, pk.b, F + 5 * i);
   }
 }
 

@@ -358,21 +358,104 @@ __device__ __forceinline__ R hllc_flux(R G1, R Gg, const State<R> &cUL, const St
   return smax(lL, lR);
 }
 
+// ---------------------------------------------------------------- mixed path
 // Face flux of the mixed path: FP32 primitives and geometry in, FP64
-// arithmetic with the Fast policy.  U is rebuilt from the FP32 primitives.
+// arithmetic with the Fast policy.  Same HLLC as hllc_flux (same signal
+// speeds, same quadratic for the contact speed, same side selection), written
+// without the rotation matrix: in the face frame every tangential flux
+// component is its tangential momentum times one factor c_t, so the global
+// momentum flux is c_t m + (F_n - c_t m_n) n, with F_n the normal component.
+// Only v.n and m.n of each side are needed.  U is rebuilt from the FP32
+// primitives (GetConvValueStab, with its |v| >= 1 clamp).
+struct MixedSide {
+  double d, p, v[3];        // primitives (v after the clamp)
+  double D, E, s;           // conserved D, E and s = rho h W^2 (m = s v)
+  double vn, mn;            // v.n, m.n
+  double cmin, cmax;        // signal speeds
+};
+
+__device__ __forceinline__ void mixed_side(double G1, double Gg, const State<double> &W, const double *n,
+                                           MixedSide &o) {
+  using A = Fast;
+  o.d = W.d, o.p = W.p;
+  o.v[0] = W.v[0], o.v[1] = W.v[1], o.v[2] = W.v[2];
+  double V2 = dot3(o.v, o.v);
+  if (V2 >= 1.0) {
+    const double g = A::div(0.9999, A::sqrt_(V2));
+    o.v[0] *= g, o.v[1] *= g, o.v[2] *= g;
+    V2 = 0.9999 * 0.9999;
+  }
+  const double h = 1.0 + A::div(Gg * W.p, W.d);
+  const double lor2 = A::rcp(1.0 - V2);
+  o.s = W.d * h * lor2;
+  o.D = W.d * A::sqrt_(lor2);
+  o.E = o.s - W.p;
+  o.vn = dot3(o.v, n);
+  o.mn = o.s * o.vn;
+  const double cs2 = A::div(G1 * W.p, W.d * h);
+  const double vn2 = o.vn * o.vn;
+  const double sroot = A::sqrt_(cs2 * (1.0 - vn2 - (V2 - vn2) * cs2) * (1.0 - V2));
+  const double r = A::rcp(1.0 - V2 * cs2);
+  const double a = o.vn * (1.0 - cs2);
+  o.cmax = (a + sroot) * r;
+  o.cmin = (a - sroot) * r;
+}
+
 // The FP32 normal is renormalised first: rounding leaves |n|^2 = 1 +- 1e-7,
-// and for a normal within ~1e-3 rad of the z axis the rotation's second row,
-// scaled by 1/sqrt(1 - n2^2), would then be off unit length by percents --
-// enough to push the rotated |v| of a fast flow past 1.
-__device__ __forceinline__ double mixed_face_flux(double gamma, double Gg, State<double> &WL, State<double> &WR,
-                                                  float nx, float ny, float nz, float S, State<double> &F) {
-  double n0 = nx, n1 = ny, n2 = nz;
-  const double r = Fast::rsqrt((n0 * n0 + n1 * n1) + n2 * n2);
-  n0 *= r, n1 *= r, n2 *= r;
-  State<double> UL, UR;
-  p2c<double, Fast>(Gg, WL, UL);
-  p2c<double, Fast>(Gg, WR, UR);
-  return hllc_flux<double, Fast>(gamma, Gg, UL, UR, WL, WR, n0, n1, n2, (double)S, F);
+// which a frame built from n alone (as the reference's rotation is) turns
+// into a non-orthonormal basis for normals near a coordinate axis.
+// F receives S * flux; returns max(lL, lR) as hllc_flux does.
+__device__ __forceinline__ double mixed_face_flux(double gamma, double Gg, const State<double> &WL,
+                                                  const State<double> &WR, float nx, float ny, float nz, float S,
+                                                  State<double> &F) {
+  using A = Fast;
+  double n[3] = {nx, ny, nz};
+  const double rn = A::rsqrt(dot3(n, n));
+  n[0] *= rn, n[1] *= rn, n[2] *= rn;
+  MixedSide L, R;
+  mixed_side(gamma, Gg, WL, n, L);
+  mixed_side(gamma, Gg, WR, n, R);
+  const double lL = smin(L.cmin, R.cmin);
+  const double lR = smax(L.cmax, R.cmax);
+  bool left;
+  double ct, Fn, FE;
+  if (lL >= 0.0 || lR <= 0.0) {
+    left = lL >= 0.0;
+    const MixedSide &X = left ? L : R;
+    ct = X.vn;
+    Fn = X.mn * X.vn + X.p;
+    FE = X.mn;
+  } else {
+    const double AL = lL * L.E - L.mn;
+    const double AR = lR * R.E - R.mn;
+    const double BL = lL * L.mn - L.mn * L.vn - L.p;
+    const double BR = lR * R.mn - R.mn * R.vn - R.p;
+    const double a = AR * lL - AL * lR;
+    const double b = AL + BL * lR - AR - BR * lL;
+    const double c = BR - BL;
+    const double sgn = b < 0.0 ? -1.0 : 1.0;
+    const double scrh = -0.5 * (b + sgn * A::sqrt_(b * b - 4.0 * a * c));
+    const double us = A::div(c, scrh);
+    const double ps = A::div(AL * us - BL, 1.0 - us * lL);
+    left = us >= 0.0;
+    const MixedSide &X = left ? L : R;
+    const double lam = left ? lL : lR;
+    const double dif = A::rcp(lam - us);
+    const double ms = (lam * (X.E + ps) - X.mn) * us * dif;  // star normal momentum
+    ct = X.vn + lam * ((lam - X.vn) * dif - 1.0);
+    Fn = X.mn * X.vn + lam * (ms - X.mn) + X.p;
+    FE = ms;  // F_E = m*_n in the star region
+  }
+  const MixedSide &X = left ? L : R;
+  const double cm = ct * X.s;                 // c_t m = c_t s v
+  const double cn = Fn - ct * X.mn;           // normal remainder (the pressure-like part)
+  const double dS = (double)S;
+  F.d = X.D * ct * dS;
+  F.v[0] = (cm * X.v[0] + cn * n[0]) * dS;
+  F.v[1] = (cm * X.v[1] + cn * n[1]) * dS;
+  F.v[2] = (cm * X.v[2] + cn * n[2]) * dS;
+  F.p = FE * dS;
+  return smax(lL, lR);
 }
 
 // Radiation 4-momentum source, solvers/RadRHD/src/radRHD_utils.cpp:8-101 with
