@@ -58,7 +58,7 @@ def per_kernel_bytes(N, F, mixed, rad):
     """Algorithmic bytes per launch: each array element counted once per kernel
     that touches it, gathers once, at the storage precision of the variant."""
     if mixed:
-        face = 20 * N + 48 * F  # W32 read; ids 8 + (n,S) 16 + packed flux write 24 per face
+        face = 20 * N + 44 * F  # W32 read; ids 8 + S*n 12 + packed flux write 24 per face
         cell = 128 * N + 24 * F  # ids+signs 16 + V 8 + U r/w 80 + p32 read 4 + W32 write 20; packed flux read
     else:
         face = 80 * N + 80 * F  # U+W read; ids 8 + (n,S) 32 + flux write 40

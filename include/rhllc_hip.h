@@ -152,12 +152,15 @@ int rhllc_eval_rad_source(int64_t n, const double *W, const double *Ur, const do
 int rhllc_copy_probe(int32_t device, int64_t nbytes, int32_t reps, double *gbps);
 
 /* ---- c2p surrogate (no counterpart in the reference) --------------------
- * A 3-64-64-1 SiLU MLP maps (log D, log1p(|m|/D), log(tau/D)), tau = E - D,
- * to log p.  The model is RHLLC_MLP_MODEL floats: the parameters W1[64][3]
- * b1[64] W2[64][64] b2[64] W3[64] b3 followed by the normalisation
- * mu_x[3] sd_x[3] mu_y sd_y. */
+ * A 3-64-64-1 SiLU MLP on log-scaled functions of (D, |m|, tau = E - D):
+ * inputs (log D, log1p(|m|/D), log(p0/D)), output log(p/p0), where p0 is the
+ * c2p equation's root with the Lorentz factor frozen at its p = 0 value,
+ * W0 = E/sqrt(E^2 - m^2): p0 = (E - D W0)/(Gg W0^2 - 1), Gg = gamma/(gamma-1).
+ * The model is RHLLC_MLP_MODEL floats: the parameters W1[64][3] b1[64]
+ * W2[64][64] b2[64] W3[64] b3, then the normalisation mu_x[3] sd_x[3] mu_y
+ * sd_y, then the gamma the model was trained for. */
 #define RHLLC_MLP_PARAMS 4481
-#define RHLLC_MLP_MODEL 4489
+#define RHLLC_MLP_MODEL 4490
 
 typedef struct rhllc_mlp_config {
   int32_t device;
@@ -190,7 +193,8 @@ int rhllc_mlp_grad(int32_t device, const float *model, int32_t n, const float *X
  * accumulator registers per lane, to check the lane layout the kernels use. */
 int rhllc_mlp_selftest(int32_t device, float *out);
 
-/* Installs a model for RHLLC_GUESS_MLP (mixed path). */
+/* Installs a model for RHLLC_GUESS_MLP (mixed path); its gamma must be the
+ * handle's (RHLLC_ERR_ARG otherwise). */
 int rhllc_set_mlp(rhllc_solver *h, const float *model);
 
 /* Device name and compute units of the current device (for reports). */

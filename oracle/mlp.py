@@ -4,7 +4,10 @@ The network has no counterpart in the reference; this numpy version is the
 checker for the device kernels' forward pass and gradients.  Parameter layout
 and features follow include/rhllc_hip.h (RHLLC_MLP_*):
 
-    W1[64][3] b1[64] W2[64][64] b2[64] W3[64] b3 | mu_x[3] sd_x[3] mu_y sd_y
+    W1[64][3] b1[64] W2[64][64] b2[64] W3[64] b3 | mu_x[3] sd_x[3] mu_y sd_y | gamma
+
+Inputs (log D, log1p(|m|/D), log(p0/D)), output log(p/p0), p0 the c2p root
+with the Lorentz factor frozen at W0 = E/sqrt(E^2 - m^2) (cold_pressure).
 
 With bf16=True the inputs of the 64x64 products (H1, W2, dZ2, W2^T) are
 rounded to bfloat16 (round to nearest even) as the MFMA kernels do; all other
@@ -16,7 +19,7 @@ import numpy as np
 
 H, IN = 64, 3
 OFF_W1, OFF_B1, OFF_W2, OFF_B2, OFF_W3, OFF_B3 = 0, 192, 256, 4352, 4416, 4480
-N_PARAMS, N_MODEL = 4481, 4489
+N_PARAMS, N_MODEL = 4481, 4490
 
 
 def bf16(x):
@@ -41,14 +44,31 @@ def dsilu(z):
     return s * (1.0 + z * (1.0 - s))
 
 
-def features(U, model):
-    """Normalised features (log D, log1p(|m|/D), log(tau/D)) of conserved states U[n,5]."""
+def cold_pressure(Gg, D, m, E):
+    """Root of E + p = D W0 + Gg p W0^2 with W0 = E/sqrt(E^2 - m^2) fixed, floored at 1e-12 D."""
+    with np.errstate(invalid="ignore", divide="ignore"):
+        q = (E - m) * (E + m)
+        W0 = E / np.sqrt(q)
+        p0 = (E - D * W0) / (Gg * (E * E / q) - 1.0)
+        return np.where(q > 0, np.fmax(p0, 1e-12 * D), np.nan)
+
+
+def raw_features(U, gamma):
+    """Un-normalised features (log D, log1p(|m|/D), log(p0/D)) as float32, and p0."""
     U = np.asarray(U, dtype=np.float64)
-    norm = np.asarray(model, dtype=np.float64)[N_PARAMS:]
     D, m, E = U[:, 0], np.linalg.norm(U[:, 1:4], axis=1), U[:, 4]
-    tau = E - D
-    raw = np.column_stack([np.log(D), np.log1p(m / D), np.log(np.where(tau > 1e-300 * D, tau / D, 1e-300))])
-    return (raw - norm[0:3]) / norm[3:6]
+    p0 = cold_pressure(gamma / (gamma - 1.0), D, m, E)
+    f = np.float32
+    with np.errstate(invalid="ignore", divide="ignore"):
+        raw = np.column_stack([np.log(D.astype(f)), np.log1p((m / D).astype(f)), np.log((p0 / D).astype(f))])
+    return raw.astype(np.float64), p0
+
+
+def features(U, model):
+    """Normalised features of conserved states U[n,5], and p0."""
+    norm = np.asarray(model, dtype=np.float64)[N_PARAMS:]
+    raw, p0 = raw_features(U, norm[8])
+    return (raw - norm[0:3]) / norm[3:6], p0
 
 
 def forward(P, X, bf=True):
@@ -64,8 +84,9 @@ def forward(P, X, bf=True):
 
 def predict_p(model, U, bf=True):
     norm = np.asarray(model, dtype=np.float64)[N_PARAMS:]
-    out, _ = forward(model, features(U, model), bf)
-    return np.exp(out * norm[7] + norm[6])
+    X, p0 = features(U, model)
+    out, _ = forward(model, X, bf)
+    return p0 * np.exp(out * norm[7] + norm[6])
 
 
 def loss_and_grad(P, X, y, ok, bf=True):

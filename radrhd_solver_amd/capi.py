@@ -14,7 +14,7 @@ GUESS_REFERENCE, GUESS_PREVIOUS, GUESS_MLP = 0, 1, 2
 HIST_BINS = 41
 
 
-MLP_PARAMS, MLP_MODEL = 4481, 4489
+MLP_PARAMS, MLP_MODEL = 4481, 4490
 
 
 class MlpConfig(C.Structure):
@@ -286,7 +286,7 @@ def eval_rad_source(W, Ur, Fr, ka, ks):
 
 
 def mlp_train(steps=20000, batch=65536, lr=2e-3, gamma=5.0 / 3.0, seed=20261202, norm_samples=1 << 20, device=0):
-    """Trains the c2p surrogate on the device; returns (model[4489] float32, loss per step)."""
+    """Trains the c2p surrogate on the device; returns (model[4490] float32, loss per step)."""
     cfg = MlpConfig(device, gamma, steps, batch, lr, seed, norm_samples)
     model = np.zeros(MLP_MODEL, np.float32)
     hist = np.zeros(steps, np.float32)

@@ -1,11 +1,12 @@
-// Storage format of the mixed path's face flux: each component as an FP32
-// value hi plus, for the momentum and energy, one signed byte e counting
-// 2^(E-158) units, E the biased exponent of hi (1/256 of hi's ulp for normal
-// hi).  x = hi + e 2^(E-158) holds x to 2^-33 relative -- FP32 alone (2^-24)
-// leaves a visible error in the momentum update wherever the pressure term
-// p n S of the four faces cancels around a cell while their roundings do not.
-// Record per face: float4 (D, mx, my, mz) + uint2 (bits of E, bytes e of
-// mx, my, mz, E).  Host and device use the same functions.
+// Packed FP64 5-vectors of the mixed path: the face flux and the conserved
+// state U.  Each component is stored as an FP32 value hi plus a signed 6-bit
+// count e of 2^(E-156) units, E the biased exponent of hi (1/64 of hi's ulp
+// for normal hi), so x = hi + e 2^(E-156) holds x to 2^-30 relative.
+// FP32 alone (2^-24) is not enough for either: the momentum flux p n S of a
+// cell's four faces cancels while their roundings do not, and U accumulates
+// one rounding per step.  Record: float4 (c0..c3) + uint2 (bits of c4,
+// e0 | e1 << 6 | e2 << 12 | e3 << 18 | e4 << 24) = 24 bytes.
+// Host and device use the same functions.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,43 +28,44 @@ __host__ __device__ __forceinline__ float bits_f32(uint32_t u) {
   return x;
 }
 
-// hi = (float)x; *e = the byte of the remainder.
-__host__ __device__ __forceinline__ float split(double x, uint32_t &byte) {
+// hi = (float)x; *e = the 6-bit remainder field.
+__host__ __device__ __forceinline__ float split(double x, uint32_t &e) {
   const float hi = (float)x;
   const int E = (int)((f32_bits(hi) >> 23) & 0xFFu);
-  double q = rint(ldexp(x - (double)hi, 158 - E));
-  q = fmin(fmax(q, -127.0), 127.0);  // NaN remainders give -127: hi is NaN/inf then
-  byte = (uint32_t)(uint8_t)(int8_t)(int)q;
+  double q = rint(ldexp(x - (double)hi, 156 - E));
+  q = fmin(fmax(q, -31.0), 31.0);  // NaN remainders give -31: hi is NaN/inf then
+  e = (uint32_t)((int)q & 63);
   return hi;
 }
 
-__host__ __device__ __forceinline__ double join(float hi, uint32_t byte) {
+__host__ __device__ __forceinline__ double join(float hi, uint32_t e6) {
   const int E = (int)((f32_bits(hi) >> 23) & 0xFFu);
-  return (double)hi + ldexp((double)(int8_t)(uint8_t)byte, E - 158);
+  const int e = (int)((e6 & 63u) ^ 32u) - 32;
+  return (double)hi + ldexp((double)e, E - 156);
 }
 
 struct Packed {
-  float4 a;  // D, mx, my, mz
-  uint2 b;   // bits of E, extension bytes (mx | my << 8 | mz << 16 | E << 24)
+  float4 a;
+  uint2 b;
 };
 
-__host__ __device__ __forceinline__ Packed pack(double d, double mx, double my, double mz, double E) {
+__host__ __device__ __forceinline__ Packed pack(double c0, double c1, double c2, double c3, double c4) {
   Packed o;
-  uint32_t e0, e1, e2, e3;
-  o.a.x = (float)d;
-  o.a.y = split(mx, e0);
-  o.a.z = split(my, e1);
-  o.a.w = split(mz, e2);
-  o.b.x = f32_bits(split(E, e3));
-  o.b.y = e0 | (e1 << 8) | (e2 << 16) | (e3 << 24);
+  uint32_t e0, e1, e2, e3, e4;
+  o.a.x = split(c0, e0);
+  o.a.y = split(c1, e1);
+  o.a.z = split(c2, e2);
+  o.a.w = split(c3, e3);
+  o.b.x = f32_bits(split(c4, e4));
+  o.b.y = e0 | (e1 << 6) | (e2 << 12) | (e3 << 18) | (e4 << 24);
   return o;
 }
 
 __host__ __device__ __forceinline__ void unpack(const float4 &a, const uint2 &b, double out[5]) {
-  out[0] = a.x;
-  out[1] = join(a.y, b.y & 0xFFu);
-  out[2] = join(a.z, (b.y >> 8) & 0xFFu);
-  out[3] = join(a.w, (b.y >> 16) & 0xFFu);
+  out[0] = join(a.x, b.y);
+  out[1] = join(a.y, b.y >> 6);
+  out[2] = join(a.z, b.y >> 12);
+  out[3] = join(a.w, b.y >> 18);
   out[4] = join(bits_f32(b.x), b.y >> 24);
 }
 

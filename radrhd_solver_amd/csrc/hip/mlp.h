@@ -1,9 +1,19 @@
-// MLP surrogate of the c2p pressure: (log D, log1p(|m|/D), log(tau/D)) -> log p,
-// 3 -> 64 -> 64 -> 1 with SiLU, no counterpart in the reference.
+// MLP surrogate of the c2p pressure, 3 -> 64 -> 64 -> 1 with SiLU, no
+// counterpart in the reference.
+//
+// Inputs are log-scaled functions of (D, |m|, tau = E - D):
+//   x = (log D, log1p(|m|/D), log(p0/D)),   output log(p/p0),
+// with p0 the pressure of the c2p equation E + p = D W + Gg p W^2 solved with
+// the Lorentz factor frozen at its p = 0 value W0 = E / sqrt(E^2 - m^2).
+// p >= p0 always (W falls as p grows) and p/p0 stays in [1, 5] over the
+// training range.  log(tau/D) itself is not usable as an input: at W ~ 50 it
+// carries the thermal energy as a 1e-4 relative difference against |m|/D,
+// below BF16 resolution, while p0 is formed in FP64 before rounding.
 //
 // Parameter vector (FP32, kParams floats):
 //   W1[64][3] b1[64] W2[64][64] b2[64] W3[64] b3[1]     (weights stored [out][in])
-// followed by the feature/target normalisation mu_x[3] sd_x[3] mu_y sd_y.
+// followed by mu_x[3] sd_x[3] mu_y sd_y (normalisation) and the adiabatic
+// index gamma the model was trained for.
 //
 // Wave-level layout.  A wave evaluates 64 samples with the 64x64 layer on
 // v_mfma_f32_16x16x16_bf16 (FP32 accumulate).  For the 16x16x16 instruction
@@ -31,8 +41,8 @@ constexpr int kOffB2 = kOffW2 + kH * kH;   // 4352
 constexpr int kOffW3 = kOffB2 + kH;        // 4416
 constexpr int kOffB3 = kOffW3 + kH;        // 4480
 constexpr int kParams = kOffB3 + 1;        // 4481
-constexpr int kOffNorm = kParams;          // mu_x[3], sd_x[3], mu_y, sd_y
-constexpr int kModel = kParams + 8;        // 4489
+constexpr int kOffNorm = kParams;          // mu_x[3], sd_x[3], mu_y, sd_y, gamma
+constexpr int kModel = kParams + 9;        // 4490
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -51,15 +61,36 @@ __device__ __forceinline__ float dsilu(float z) {
   return s * (1.0f + z * (1.0f - s));
 }
 
-// Features of a conserved state (FP64 in, FP32 out), normalised.
-__device__ __forceinline__ void features(const float *norm, double D, double m, double E, float x[3]) {
-  const double tau = E - D;
-  const double r0 = log(D);
-  const double r1 = log1p(m / D);
-  const double r2 = log(tau > 1e-300 * D ? tau / D : 1e-300);
-  x[0] = (float)((r0 - norm[0]) / norm[3]);
-  x[1] = (float)((r1 - norm[1]) / norm[4]);
-  x[2] = (float)((r2 - norm[2]) / norm[5]);
+// Frozen-Lorentz-factor pressure p0 (see the top of this file), floored at
+// 1e-12 D; NaN for E <= |m| (no physical state).
+__device__ __forceinline__ double cold_pressure(double Gg, double D, double m, double E) {
+  const double q = (E - m) * (E + m);
+  const double W0 = E / sqrt(q);
+  const double p0 = (E - D * W0) / (Gg * (E * E / q) - 1.0);
+  return q > 0.0 ? fmax(p0, 1e-12 * D) : __builtin_nan("");
+}
+
+// Raw (un-normalised) features of a conserved state; *p0 receives the cold pressure.
+__device__ __forceinline__ void raw_features(double Gg, double D, double m, double E, float r[3], double &p0) {
+  p0 = cold_pressure(Gg, D, m, E);
+  r[0] = logf((float)D);
+  r[1] = log1pf((float)(m / D));
+  r[2] = logf((float)(p0 / D));
+}
+
+// Normalised features; *p0 as raw_features.
+__device__ __forceinline__ void features(const float *norm, double Gg, double D, double m, double E, float x[3],
+                                         double &p0) {
+  float r[3];
+  raw_features(Gg, D, m, E, r, p0);
+  x[0] = (r[0] - norm[0]) / norm[3];
+  x[1] = (r[1] - norm[1]) / norm[4];
+  x[2] = (r[2] - norm[2]) / norm[5];
+}
+
+// Pressure from the network output o (normalised log(p/p0)).
+__device__ __forceinline__ double predicted_pressure(const float *norm, float o, double p0) {
+  return p0 * (double)expf(o * norm[7] + norm[6]);
 }
 
 // Forward pass of one wave over 64 samples.  x[t][f] holds the normalised

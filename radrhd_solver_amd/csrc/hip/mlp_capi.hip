@@ -150,6 +150,7 @@ extern "C" int rhllc_mlp_train(const rhllc_mlp_config *cfg, float *model_out, fl
     const int im = k < 3 ? kOffNorm + k : kOffNorm + 6, is = k < 3 ? kOffNorm + 3 + k : kOffNorm + 7;
     model[im] = (float)mu, model[is] = (float)sd;
   }
+  model[kOffNorm + 8] = (float)cfg->gamma;
   // Glorot-uniform weights, zero biases (seeded).
   uint64_t rs = cfg->seed * 0x2545F4914F6CDD1Dull + 1;
   auto uni = [&](double a) {

@@ -51,13 +51,12 @@ __device__ void make_sample(double gamma, uint64_t seed, uint64_t k, float x[3],
   rhllc_dev::p2c<double, Ieee>(Gg, W, U);
   int it = 0;
   const int code = rhllc_dev::c2p(Gg, U, Wc, -1.0, it);
-  ok = code == 0 && Wc.p > 0.0;
   const double m = sqrt(rhllc_dev::dot3(U.v, U.v));
-  const double tau = U.p - U.d;
-  x[0] = (float)log(U.d);
-  x[1] = (float)log1p(m / U.d);
-  x[2] = (float)log(tau > 1e-300 * U.d ? tau / U.d : 1e-300);
-  y = ok ? (float)log(Wc.p) : 0.0f;
+  double p0;
+  raw_features(Gg, U.d, m, U.p, x, p0);
+  ok = code == 0 && Wc.p > 0.0 && p0 > 0.0 && isfinite(x[0] + x[1] + x[2]);
+  y = ok ? (float)log(Wc.p / p0) : 0.0f;
+  if (!ok) x[0] = x[1] = x[2] = 0.0f;
 }
 
 __global__ void k_mlp_gen(double gamma, uint64_t seed, const uint32_t *step, int n, const float *norm, float *X,
@@ -299,17 +298,19 @@ __global__ void k_mlp_pack(const float *P, short *W2b, short *W2Tb) {
   W2Tb[j * kH + i] = b;
 }
 
-// Inference: p = exp(out * sd_y + mu_y) for n conserved states (U: n x 5).
+// Inference: p = p0 exp(out * sd_y + mu_y) for n conserved states (U: n x 5).
 __global__ __launch_bounds__(256) void k_mlp_eval(const float *__restrict__ P, const short *__restrict__ W2b, int64_t n,
                                                   const double *__restrict__ U, double *__restrict__ p_out) {
   const int l = threadIdx.x & 63;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float xl[3] = {0.f, 0.f, 0.f};
   const float *norm = P + kOffNorm;
+  const double gamma = norm[8];
+  double p0 = 0.0;
   if (i < n) {
     const double *u = U + 5 * i;
     const double m = sqrt((u[1] * u[1] + u[2] * u[2]) + u[3] * u[3]);
-    features(norm, u[0], m, u[4], xl);
+    features(norm, gamma / (gamma - 1.0), u[0], m, u[4], xl, p0);
   }
   float x[4][3];
 #pragma unroll
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(256) void k_mlp_eval(const float *__restrict__ P, c
   float out[4];
   forward_wave(P, W2b, x, out);
   const float o = out[l >> 4];  // this lane's own sample 16 (l/16) + l%16 = l
-  if (i < n) p_out[i] = exp((double)o * norm[7] + norm[6]);
+  if (i < n) p_out[i] = predicted_pressure(norm, o, p0);
 }
 
 // MFMA lane-layout self test: A[i][k] = i + 0.25 k... written so every

@@ -191,7 +191,7 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
 
   std::vector<int2> lr(F);
   std::vector<double4> nS(h->mixed ? 0 : F);
-  std::vector<float4> nS32(h->mixed ? F : 0);
+  std::vector<float3> Sn32(h->mixed ? F : 0);
   for (int32_t i = 0; i < F; ++i) {
     const int32_t f = h->face_new_to_old[i];
     const uint8_t *r = gf + (size_t)f * RHLLC_GEO_FACE_BYTES;
@@ -203,7 +203,7 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
     const int32_t rr = fr[f] >= 0 ? cnew[fr[f]] : (fr[f] == RHLLC_BOUND_OUT_SOURCE ? -1 : l);
     lr[i] = make_int2(l, rr);
     if (h->mixed)
-      nS32[i] = make_float4((float)n[0], (float)n[1], (float)n[2], (float)S);
+      Sn32[i] = make_float3((float)(n[0] * S), (float)(n[1] * S), (float)(n[2] * S));
     else
       nS[i] = make_double4(n[0], n[1], n[2], S);
   }
@@ -240,10 +240,10 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
   d.cell_faces = cf_d;
   d.cell_V = V_d;
   if (h->mixed) {
-    float4 *q;
+    float3 *q;
     HIP_TRY(h->alloc(&q, F));
-    HIP_TRY(hipMemcpy(q, nS32.data(), sizeof(float4) * F, hipMemcpyHostToDevice));
-    d.face_nS32 = q;
+    HIP_TRY(hipMemcpy(q, Sn32.data(), sizeof(float3) * F, hipMemcpyHostToDevice));
+    d.face_Sn32 = q;
     HIP_TRY(h->alloc(&d.F4f, F));
     HIP_TRY(h->alloc(&d.FXf, F));
   } else {
@@ -514,6 +514,10 @@ extern "C" int rhllc_set_radiation(rhllc_solver *h, const float *Ur, const float
 
 extern "C" int rhllc_set_mlp(rhllc_solver *h, const float *model) {
   if (!h || !model) return fail(RHLLC_ERR_ARG, "null argument");
+  const double mg = model[RHLLC_MLP_MODEL - 1];
+  if (!(std::fabs(mg - h->prm.gamma) <= 1e-6 * h->prm.gamma))
+    return fail(RHLLC_ERR_ARG, "model was trained for gamma " + std::to_string(mg) + ", the handle uses " +
+                                   std::to_string(h->prm.gamma));
   HIP_TRY(hipSetDevice(h->prm.device));
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->drop_graph();
@@ -570,7 +574,7 @@ extern "C" int rhllc_info(rhllc_solver *h, int64_t *n_cells, int64_t *n_faces, i
     const int64_t N = h->n_cells, F = h->n_faces;
     // Algorithmic bytes of one step (both kernels), each array element counted
     // once per kernel that touches it.
-    *device_bytes = h->mixed ? 148 * N + 72 * F : 233 * N + 120 * F;
+    *device_bytes = h->mixed ? 148 * N + 68 * F : 233 * N + 120 * F;
     if (h->rad) *device_bytes += 24 * N;
   }
   return RHLLC_OK;

@@ -34,15 +34,15 @@ struct Dev {
   // faces
   const int2 *face_lr;      // (left cell, right cell); right < 0: fixed ghost state
   const double4 *face_nS;   // (nx, ny, nz, S), FP64 path
-  const float4 *face_nS32;  // (nx, ny, nz, S), FP32 path
+  const float3 *face_Sn32;  // S * (nx, ny, nz) in FP32, mixed path
   double4 *F4;              // flux (d, mx, my, mz) * S, FP64 path
   double *FE;               // flux E * S
-  float4 *F4f;              // mixed path: flux (d, mx, my, mz) * S, FP32 parts (flux_ext.h)
-  uint2 *FXf;               // mixed path: bits of flux E * S, extension bytes
+  float4 *F4f;              // mixed path: packed flux * S (flux_ext.h)
+  uint2 *FXf;
   // cells
   const int4 *cell_faces;  // face index | 0x80000000 when the cell is id_r (sign bit 0)
   const double *cell_V;
-  double4 *U4;   // (D, mx, my, mz)
+  double4 *U4;   // (D, mx, my, mz), FP64 on both paths (c2p amplifies errors in U)
   double *UE;    // E
   double4 *W4;   // (rho, vx, vy, vz), FP64 path
   double *Wp;    // p (both paths: the FP64 pressure)

@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kBlock) void k_face(Dev d) {
     } else {
       // FP32 storage (state, geometry, flux), FP64 arithmetic with the fast
       // divide/sqrt policy; U is rebuilt from the FP32 primitives.
-      const float4 nS = d.face_nS32[i];
+      const float3 Sn = d.face_Sn32[i];
       State<double> WL, WR, F;
       {
         const float4 a = d.W4f[lr.x];
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_face(Dev d) {
       } else {
         WR.d = d.ghostW[0], WR.v[0] = d.ghostW[1], WR.v[1] = d.ghostW[2], WR.v[2] = d.ghostW[3], WR.p = d.ghostW[4];
       }
-      const double s = mixed_face_flux(d.gamma, d.Gg, WL, WR, nS.x, nS.y, nS.z, nS.w, F);
+      const double s = mixed_face_flux(d.gamma, d.Gg, WL, WR, Sn.x, Sn.y, Sn.z, F);
       speed = smax(speed, s);
       const rhllc_ext::Packed pk = rhllc_ext::pack(F.d, F.v[0], F.v[1], F.v[2], F.p);
       d.F4f[i] = pk.a;
@@ -142,7 +142,8 @@ __device__ __forceinline__ double mlp_guess(const Dev &d, bool live, const State
   using namespace rhllc_mlp;
   const float *norm = d.mlpP + kOffNorm;
   float xl[3] = {0.f, 0.f, 0.f};
-  if (live) features(norm, U.d, sqrt(dot3(U.v, U.v)), U.p, xl);
+  double p0 = 0.0;
+  if (live) features(norm, d.Gg, U.d, sqrt(dot3(U.v, U.v)), U.p, xl, p0);
   const int l = threadIdx.x & 63;
   float x[4][3];
 #pragma unroll
@@ -151,7 +152,7 @@ __device__ __forceinline__ double mlp_guess(const Dev &d, bool live, const State
     for (int f = 0; f < 3; ++f) x[t][f] = __shfl(xl[f], 16 * t + (l & 15), 64);
   float out[4];
   forward_wave(d.mlpP, d.mlpW2b, x, out);
-  return exp((double)out[l >> 4] * norm[7] + norm[6]);
+  return predicted_pressure(norm, out[l >> 4], p0);
 }
 
 template <int VARIANT, bool RAD, bool STATS>
@@ -443,8 +444,8 @@ __global__ void k_eval_flux(double gamma, int64_t n, const double *UL, const dou
     State<double> a, b, ff;
     a.d = (float)wl.d, a.v[0] = (float)wl.v[0], a.v[1] = (float)wl.v[1], a.v[2] = (float)wl.v[2], a.p = (float)wl.p;
     b.d = (float)wr.d, b.v[0] = (float)wr.v[0], b.v[1] = (float)wr.v[1], b.v[2] = (float)wr.v[2], b.p = (float)wr.p;
-    speed[i] = mixed_face_flux(gamma, Gg, a, b, (float)nS4[4 * i], (float)nS4[4 * i + 1], (float)nS4[4 * i + 2],
-                               (float)nS4[4 * i + 3], ff);
+    const double *g = nS4 + 4 * i;
+    speed[i] = mixed_face_flux(gamma, Gg, a, b, (float)(g[0] * g[3]), (float)(g[1] * g[3]), (float)(g[2] * g[3]), ff);
     const rhllc_ext::Packed pk = rhllc_ext::pack(ff.d, ff.v[0], ff.v[1], ff.v[2], ff.p);
     rhllc_ext::unpack(pk.a, pk.b, F + 5 * i);
   }

@@ -401,16 +401,18 @@ __device__ __forceinline__ void mixed_side(double G1, double Gg, const State<dou
   o.cmin = (a - sroot) * r;
 }
 
-// The FP32 normal is renormalised first: rounding leaves |n|^2 = 1 +- 1e-7,
-// which a frame built from n alone (as the reference's rotation is) turns
-// into a non-orthonormal basis for normals near a coordinate axis.
+// The face geometry is the FP32 vector S n; the unit normal and the area are
+// recovered in FP64 (n exactly unit to rounding, which the frame-free form
+// relies on: m = m_n n + m_t holds only for |n| = 1).
 // F receives S * flux; returns max(lL, lR) as hllc_flux does.
 __device__ __forceinline__ double mixed_face_flux(double gamma, double Gg, const State<double> &WL,
-                                                  const State<double> &WR, float nx, float ny, float nz, float S,
+                                                  const State<double> &WR, float sx, float sy, float sz,
                                                   State<double> &F) {
   using A = Fast;
-  double n[3] = {nx, ny, nz};
-  const double rn = A::rsqrt(dot3(n, n));
+  double n[3] = {sx, sy, sz};
+  const double q = dot3(n, n);
+  const double rn = q > 0.0 ? A::rsqrt(q) : 0.0;  // a zero-area face gets n = 0 and a zero flux
+  const double S = q * rn;
   n[0] *= rn, n[1] *= rn, n[2] *= rn;
   MixedSide L, R;
   mixed_side(gamma, Gg, WL, n, L);
@@ -449,7 +451,7 @@ __device__ __forceinline__ double mixed_face_flux(double gamma, double Gg, const
   const MixedSide &X = left ? L : R;
   const double cm = ct * X.s;                 // c_t m = c_t s v
   const double cn = Fn - ct * X.mn;           // normal remainder (the pressure-like part)
-  const double dS = (double)S;
+  const double dS = S;
   F.d = X.D * ct * dS;
   F.v[0] = (cm * X.v[0] + cn * n[0]) * dS;
   F.v[1] = (cm * X.v[1] + cn * n[1]) * dS;

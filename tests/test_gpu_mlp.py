@@ -25,7 +25,7 @@ def test_mfma_16x16x16_bf16_lane_layout():
 def _model(rng):
     m = np.zeros(R.N_MODEL, np.float32)
     m[: R.N_PARAMS] = rng.normal(scale=0.25, size=R.N_PARAMS)
-    m[R.N_PARAMS:] = [0.5, 1.0, -2.0, 2.0, 1.5, 3.0, -1.0, 4.0]
+    m[R.N_PARAMS:] = [0.5, 1.0, -2.0, 2.0, 1.5, 3.0, 0.5, 0.4, 5.0 / 3.0]
     return m
 
 
