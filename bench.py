@@ -9,11 +9,13 @@ r < 0.1 else 1, gamma = 5/3, CFL 0.4, with a seeded FP32 radiation field
 as an explicit 4-momentum source after every hydro update.
 
 A "step" is one hipGraph replay of face kernel -> cell kernel -> dt kernel
-over the whole mesh with the state resident in HBM.  `value` is the mixed
-FP32-flux / FP64-c2p step's cell-updates per second; the plain FP64 step is
-timed beside it on the same mesh and state (`fp64`), and the per-field L1
-difference of the conserved variables between the two after the same number
-of steps is reported.
+over the whole mesh with the state resident in HBM.  `value` is the
+cell-updates per second of the mixed step (FP32 storage of W, geometry and
+face flux; FP64 U and arithmetic) with the c2p surrogate start (the stored
+model for gamma 5/3, trained on the device first when there is none).  Beside
+it, on the same mesh and state: the mixed step started from the previous
+pressure (`mixed_prev`) and the plain FP64 step (`fp64`), with the per-field
+L1 difference of the conserved variables against FP64 after the same steps.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--hexes 159]
 """
@@ -68,7 +70,7 @@ def per_kernel_bytes(N, F, mixed, rad):
     return face, cell
 
 
-def run_gpu(mesh, W0, rad, precision, guess, steps, warmup, kernel_steps):
+def run_gpu(mesh, W0, rad, precision, guess, steps, warmup, kernel_steps, model=None):
     from radrhd_solver_amd import capi
 
     h = mesh.h_min()
@@ -77,7 +79,9 @@ def run_gpu(mesh, W0, rad, precision, guess, steps, warmup, kernel_steps):
     s.upload_W(W0)
     if rad is not None:
         s.set_radiation(*rad)
-    log(f"solver ready ({'mixed' if precision else 'fp64'}) in {time.time() - t0:.1f} s")
+    if model is not None:
+        s.set_mlp(model)
+    log(f"solver ready ({'mixed' if precision else 'fp64'}, guess {guess}) in {time.time() - t0:.1f} s")
     s.run(warmup, graph=True)
     s.sync()
     t0 = time.perf_counter()
@@ -172,11 +176,23 @@ def main():
     N, F = mesh.n_cells, mesh.n_faces
     log(f"mesh {n}^3 x 6 = {N} tets, {F} faces in {time.time() - t0:.1f} s")
 
-    mixed = run_gpu(mesh, W0, rad, capi.PREC_MIXED, capi.GUESS_PREVIOUS, args.steps, args.warmup, args.kernel_steps)
+    from radrhd_solver_amd import surrogate
+
+    t0 = time.time()
+    model, minfo = surrogate.load_or_train(5.0 / 3.0)
+    log(f"c2p surrogate: {minfo.get('source')} ({time.time() - t0:.1f} s)")
+    mixed = run_gpu(mesh, W0, rad, capi.PREC_MIXED, capi.GUESS_MLP, args.steps, args.warmup, args.kernel_steps,
+                    model=model)
+    prev = run_gpu(mesh, W0, rad, capi.PREC_MIXED, capi.GUESS_PREVIOUS, args.steps, args.warmup, args.kernel_steps)
     fp64 = None
     if not args.no_fp64:
         fp64 = run_gpu(mesh, W0, rad, capi.PREC_FP64, capi.GUESS_REFERENCE, args.steps, args.warmup,
                        args.kernel_steps)
+    V = mesh.cells["V"]
+
+    def rel_l1(a, b):
+        return [float((np.abs(a[:, k] - b[:, k]) * V).sum() / max((np.abs(b[:, k]) * V).sum(), 1e-300))
+                for k in range(5)]
 
     ms = mixed["seconds"] / args.steps * 1e3
     value = N * args.steps / mixed["seconds"] / 1e6
@@ -200,10 +216,11 @@ def main():
         "data": "synthetic: seeded jittered/permuted Kuhn-tet mesh, blast-wave state, seeded radiation field",
         "config": {
             "workload": f"full-form blast wave, {n}^3 hexes x 6 = {N} tets, {F} faces, radiation source, "
-            "mixed FP32 flux / FP64 c2p, hipGraph-replayed step",
+            "mixed precision (FP32 W/geometry/flux storage, FP64 U and arithmetic), surrogate c2p start, "
+            "hipGraph-replayed step",
             "n_cells": N,
             "n_faces": F,
-            "c2p_start": "previous-step pressure",
+            "c2p_start": "MLP surrogate where tau changed > 2% in the step, previous pressure elsewhere",
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -220,14 +237,20 @@ def main():
             "step_GBps": round((fb + cb) / (ms * 1e-3) / 1e9, 1),
         },
         "kernels_ms": {k: round(v, 5) for k, v in kt.items()},
-        "c2p": {k: mixed["stats"][k] for k in ("c2p_fail", "fix_calls", "fix_fail", "fallback_cells")},
+        "c2p": {k: mixed["stats"][k] for k in ("c2p_fail", "fix_calls", "fix_fail", "fallback_cells", "mlp_starts")},
+        "surrogate": {k: minfo.get(k) for k in ("source", "steps", "batch", "lr", "loss_first", "loss_last", "seconds")},
+        "mixed_prev": {
+            "value": round(N * args.steps / prev["seconds"] / 1e6, 2),
+            "ms_per_step": round(prev["seconds"] / args.steps * 1e3, 4),
+            "kernels_ms": {k: round(v, 5) for k, v in prev["kernels"].items()},
+            "c2p": {k: prev["stats"][k] for k in ("c2p_fail", "fix_calls", "fix_fail", "fallback_cells")},
+        },
     }
     if fp64 is not None:
         fv = N * args.steps / fp64["seconds"] / 1e6
-        V = mesh.cells["V"]
-        U64, U32 = fp64["U"], mixed["U"]
-        l1 = [float((np.abs(U32[:, k] - U64[:, k]) * V).sum() / max((np.abs(U64[:, k]) * V).sum(), 1e-300))
-              for k in range(5)]
+        l1 = rel_l1(mixed["U"], fp64["U"])
+        out["mixed_prev"]["rel_l1_vs_fp64"] = rel_l1(prev["U"], fp64["U"])
+        out["mixed_prev"]["speedup_vs_fp64"] = round(out["mixed_prev"]["value"] / fv, 3)
         fb64, cb64 = per_kernel_bytes(N, F, False, True)
         out["fp64"] = {
             "value": round(fv, 2),

@@ -48,8 +48,10 @@ extern "C" {
 /* Initial pressure of the c2p Newton iteration. */
 #define RHLLC_GUESS_REFERENCE 0 /* max(|m|-E, 0, pmin) as rhllc_flux_stab.cpp:139-151 */
 #define RHLLC_GUESS_PREVIOUS 1  /* the previous step's pressure */
-#define RHLLC_GUESS_MLP 2       /* the MLP surrogate (rhllc_set_mlp); the previous pressure,
-                                   then the reference start, then PhysPressureFix as fallbacks */
+#define RHLLC_GUESS_MLP 2       /* the MLP surrogate (rhllc_set_mlp) in cells whose tau = E - D
+                                   changed by more than 2% in the step (the previous pressure
+                                   elsewhere); then the previous pressure, the reference start
+                                   and PhysPressureFix as fallbacks */
 
 typedef struct rhllc_params {
   double gamma;       /* adiabatic index of the ideal-gas EOS (reference: 5/3 constexpr) */
@@ -73,6 +75,7 @@ typedef struct rhllc_stats {
   int64_t fix_calls;       /* PhysPressureFix invocations */
   int64_t fix_fail;        /* PhysPressureFix failures (the reference aborts) */
   int64_t fallback_cells;  /* GUESS_MLP/PREVIOUS: cells that needed a fallback start */
+  int64_t mlp_starts;      /* GUESS_MLP: cells started from the surrogate */
   int64_t newton_iters;    /* sum of Newton iterations (collect_stats) */
   int64_t newton_hist[RHLLC_HIST_BINS]; /* histogram of per-cell iterations (collect_stats) */
   double max_speed;        /* max signal speed of the last step (rhllc::max_signal_speed) */

@@ -52,6 +52,7 @@ class Stats(C.Structure):
         ("fix_calls", C.c_int64),
         ("fix_fail", C.c_int64),
         ("fallback_cells", C.c_int64),
+        ("mlp_starts", C.c_int64),
         ("newton_iters", C.c_int64),
         ("newton_hist", C.c_int64 * HIST_BINS),
         ("max_speed", C.c_double),
@@ -67,6 +68,7 @@ class Stats(C.Structure):
             "fix_calls": self.fix_calls,
             "fix_fail": self.fix_fail,
             "fallback_cells": self.fallback_cells,
+            "mlp_starts": self.mlp_starts,
             "newton_iters": self.newton_iters,
             "newton_hist": list(self.newton_hist),
             "max_speed": self.max_speed,

@@ -440,6 +440,7 @@ extern "C" int rhllc_get_stats(rhllc_solver *h, rhllc_stats *out) {
   out->fix_calls = (int64_t)s.fix_calls;
   out->fix_fail = (int64_t)s.fix_fail;
   out->fallback_cells = (int64_t)s.fallback;
+  out->mlp_starts = (int64_t)s.mlp_starts;
   out->newton_iters = (int64_t)s.newton_iters;
   for (int k = 0; k < RHLLC_HIST_BINS; ++k) out->newton_hist[k] = (int64_t)s.hist[k];
   out->max_speed = s.last_speed;

@@ -19,7 +19,7 @@ struct Scalars {
   double last_speed;  // max signal speed of the last finished step
   unsigned long long steps;
   unsigned long long c2p_fail[6];
-  unsigned long long fix_calls, fix_fail, fallback, newton_iters;
+  unsigned long long fix_calls, fix_fail, fallback, newton_iters, mlp_starts;
   unsigned long long hist[kHistBins];
   unsigned long long max_residual_bits;
 };

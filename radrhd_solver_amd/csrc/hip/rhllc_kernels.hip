@@ -155,13 +155,20 @@ __device__ __forceinline__ double mlp_guess(const Dev &d, bool live, const State
   return predicted_pressure(norm, out[l >> 4], p0);
 }
 
+// Surrogate start only where the previous pressure is stale: the cell's
+// tau = E - D changed by more than this fraction in the step.  From a start
+// off by e, Newton needs about 2.4 iterations at e = 1e-3, 2.8 at 1e-2 and
+// 3.1 at 6e-2; the surrogate's start is off by ~1-2%, so below a few percent
+// change the previous pressure is the better start.
+constexpr double kMlpGate = 0.02;
+
 template <int VARIANT, bool RAD, bool STATS>
 __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
   constexpr bool MIXED = VARIANT >= kMixedPrev;
   const int c = (int)(xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x);
   const bool live = c < d.n_cells;
   int iters = 0, code = 0;
-  bool fell_back = false, fixed = false, fix_failed = false;
+  bool fell_back = false, fixed = false, fix_failed = false, stale = false;
   double resid = 0.0;
   const double tau = d.sc->tau;
   State<double> U, W;
@@ -180,22 +187,29 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
         s.d -= F.d, s.v[0] -= F.v[0], s.v[1] -= F.v[1], s.v[2] -= F.v[2], s.p -= F.p;
       }
     }
-    const double k = tau / d.cell_V[c];
+    const double k = MIXED ? Fast::div(tau, d.cell_V[c]) : tau / d.cell_V[c];
     s.d *= k, s.v[0] *= k, s.v[1] *= k, s.v[2] *= k, s.p *= k;
     load_U(d, c, U);
     U.d -= s.d, U.v[0] -= s.v[0], U.v[1] -= s.v[1], U.v[2] -= s.v[2], U.p -= s.p;
+    if (VARIANT == kMixedMlp) stale = !(fabs(s.p - s.d) <= kMlpGate * fabs(U.p - U.d));
   }
   // Start of the Newton iteration.  The reference start ignores the previous
   // W; the FP64 previous-pressure variant reads the FP64 pressure, the mixed
-  // path the FP32 one; the surrogate evaluates the MLP on the updated U.
+  // path the FP32 one.  The surrogate variant takes the previous pressure too,
+  // except in stale cells, where the MLP evaluated on the updated U gives the
+  // start; a wave runs the network (all 64 lanes, MFMA) only if one of its
+  // cells is stale.
   double guess = -1.0;
   if (VARIANT == kFp64Prev && live) guess = d.Wp[c];
-  if (VARIANT == kMixedPrev && live) guess = (double)d.Wpf[c];
-  if (VARIANT == kMixedMlp) guess = mlp_guess(d, live, U);
+  if ((VARIANT == kMixedPrev || VARIANT == kMixedMlp) && live) guess = (double)d.Wpf[c];
+  if (VARIANT == kMixedMlp && __ballot(stale) != 0) {
+    const double g = mlp_guess(d, live, U);
+    if (stale) guess = g;
+  }
   if (live) {
     // c2p writes W only when it succeeds.
     code = MIXED ? c2p<Fast>(d.Gg, U, W, guess, iters) : c2p(d.Gg, U, W, guess, iters);
-    if (code && VARIANT == kMixedMlp) {
+    if (code && VARIANT == kMixedMlp && stale) {
       int it2 = 0;
       fell_back = true;
       code = c2p<Fast>(d.Gg, U, W, (double)d.Wpf[c], it2);
@@ -224,7 +238,10 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
       const float2 kk = d.rad_k[c];
       const double Fr[3] = {(double)fu.x, (double)fu.y, (double)fu.z};
       double G[4];
-      rad_source(W, (double)fu.w, Fr, (double)kk.x, (double)kk.y, G);
+      if (MIXED)
+        rad_source<Fast>(W, (double)fu.w, Fr, (double)kk.x, (double)kk.y, G);
+      else
+        rad_source(W, (double)fu.w, Fr, (double)kk.x, (double)kk.y, G);
       const double ds = 1;
       U.p += ds * tau * G[0];
       U.v[0] += ds * tau * G[1];
@@ -256,6 +273,7 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
   count_if(&d.sc->fix_calls, fixed);
   count_if(&d.sc->fix_fail, fix_failed);
   count_if(&d.sc->fallback, fell_back);
+  if (VARIANT == kMixedMlp) count_if(&d.sc->mlp_starts, stale);
   if (STATS) {
     __shared__ unsigned int hist[kHistBins];
     __shared__ unsigned long long isum[kBlock / 64];
@@ -342,7 +360,7 @@ __global__ void k_clear(Dev d) {
     Scalars *sc = d.sc;
     sc->steps = 0;
     for (int k = 0; k < 6; ++k) sc->c2p_fail[k] = 0;
-    sc->fix_calls = sc->fix_fail = sc->fallback = sc->newton_iters = 0;
+    sc->fix_calls = sc->fix_fail = sc->fallback = sc->newton_iters = sc->mlp_starts = 0;
     for (int k = 0; k < kHistBins; ++k) sc->hist[k] = 0;
     sc->max_residual_bits = 0;
   }

@@ -464,7 +464,9 @@ __device__ __forceinline__ double mixed_face_flux(double gamma, double Gg, const
 // the M1 Eddington tensor (ISOTROPIC_INTENSITY == 1 branch); the thermal
 // emission term is omitted (q = 0).  EddTensor(a, b) is evaluated once per
 // component from shared subexpressions; every component keeps the reference's
-// operation order, so the values are bitwise those of nine separate calls.
+// operation order, so the values are bitwise those of nine separate calls
+// (A = Ieee; the mixed path runs the same sequence with the Fast policy).
+template <typename A = Ieee>
 __device__ __forceinline__ void edd_matrix(double U, const double *F, double D[3][3]) {
   double fm = dot3(F, F);
   if (fm < 1e-150 || U < 1e-150) {
@@ -474,8 +476,8 @@ __device__ __forceinline__ void edd_matrix(double U, const double *F, double D[3
       for (int b = 0; b < 3; ++b) D[a][b] = a == b ? 0.33333333333333 : 0.0;
     return;
   }
-  fm = sqrt(fm);
-  const double n[3] = {F[0] / fm, F[1] / fm, F[2] / fm};
+  fm = A::sqrt_(fm);
+  const double n[3] = {A::div(F[0], fm), A::div(F[1], fm), A::div(F[2], fm)};
   if (fm > U) {
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -483,9 +485,9 @@ __device__ __forceinline__ void edd_matrix(double U, const double *F, double D[3
       for (int b = 0; b < 3; ++b) D[a][b] = n[a] * n[b];
     return;
   }
-  double f2 = fm / U;
+  double f2 = A::div(fm, U);
   f2 = f2 * f2;
-  const double chi_2 = (1.5 + 2.0 * f2) / (5.0 + 2.0 * sqrt(4.0 - 3.0 * f2));
+  const double chi_2 = A::div(1.5 + 2.0 * f2, 5.0 + 2.0 * A::sqrt_(4.0 - 3.0 * f2));
   const double c = 3.0 * chi_2 - 0.5;
   const double dg = 0.5 - chi_2;
 #pragma unroll
@@ -498,18 +500,19 @@ __device__ __forceinline__ void edd_matrix(double U, const double *F, double D[3
     }
 }
 
+template <typename A = Ieee>
 __device__ __forceinline__ void rad_source(const State<double> &W, double Ur, const double *Fr, double abs_op,
                                            double scat_op, double *G) {
   const double tot_op = abs_op + scat_op;
   const double rho = W.d;
   double gamma = dot3(W.v, W.v);
   double u2 = gamma;
-  const double gamma2 = 1.0 / (1.0 - gamma);
+  const double gamma2 = A::rcp(1.0 - gamma);
   u2 *= gamma2;
-  gamma = sqrt(gamma2);
+  gamma = A::sqrt_(gamma2);
   const double u[3] = {gamma * W.v[0], gamma * W.v[1], gamma * W.v[2]};
   double Dm[3][3];
-  edd_matrix(Ur, Fr, Dm);
+  edd_matrix<A>(Ur, Fr, Dm);
   double uuD = 0.;
   const double uF = dot3(u, Fr);
   double uD[3];

@@ -108,6 +108,10 @@ def test_mixed_step_with_surrogate_start(trained):
         s.close()
     Up, sp = res[capi.GUESS_PREVIOUS]
     Um, sm = res[capi.GUESS_MLP]
-    print("prev-p stats", sp["newton_iters"], sp["fallback_cells"], "mlp stats", sm["newton_iters"], sm["fallback_cells"])
+    print("prev-p stats", sp["newton_iters"], sp["fallback_cells"], "mlp stats", sm["newton_iters"],
+          sm["fallback_cells"], "surrogate starts", sm["mlp_starts"])
+    assert sm["mlp_starts"] > 0  # the blast front changes tau by more than the 2% gate
     assert sm["fix_fail"] == 0 and sum(sm["c2p_fail"]) == 0
-    assert np.allclose(Um, Up, rtol=1e-9, atol=1e-12)
+    # both starts converge to the same Newton tolerance, so the states agree far below the mixed error
+    scale = np.abs(Up).max(axis=0)
+    assert (np.abs(Um - Up) <= 1e-9 * scale).all()
