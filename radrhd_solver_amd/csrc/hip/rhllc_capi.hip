@@ -92,10 +92,16 @@ struct rhllc_solver {
 
   bool ready() const { return has_state && (variant != kMixedMlp || mlp_P != nullptr); }
 
+  // Cell kernel, plus the list kernel of the surrogate-start variant.
+  hipError_t enqueue_cells() {
+    hipError_t e = launch_cell(d, variant, rad, prm.collect_stats != 0, stream);
+    if (e == hipSuccess && variant == kMixedMlp) e = launch_cell_mlp(d, rad, prm.collect_stats != 0, stream);
+    return e;
+  }
   hipError_t enqueue_step() {
     hipError_t e = launch_face(d, mixed, stream);
     if (e != hipSuccess) return e;
-    e = launch_cell(d, variant, rad, prm.collect_stats != 0, stream);
+    e = enqueue_cells();
     if (e != hipSuccess) return e;
     return launch_dt(d, stream);
   }
@@ -260,6 +266,7 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
   HIP_TRY(h->alloc(&d.Wp, N));
   HIP_TRY(h->alloc(&d.W4f, N));
   HIP_TRY(h->alloc(&d.Wpf, N));
+  if (variant == kMixedMlp) HIP_TRY(h->alloc(&d.mlp_list, N));
   HIP_TRY(h->alloc(&d.sc, 1));
   HIP_TRY(h->alloc(&d.speed_slots, (size_t)kSpeedSlots * kSlotStride));
   std::memset(h->host_sc, 0, sizeof(Scalars));
@@ -545,7 +552,7 @@ extern "C" int rhllc_time_kernels(rhllc_solver *h, int32_t nsteps, double out[4]
     hipError_t e = hipEventRecord(ev[0], h->stream);
     if (e == hipSuccess) e = launch_face(h->d, h->mixed, h->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[1], h->stream);
-    if (e == hipSuccess) e = launch_cell(h->d, h->variant, h->rad, h->prm.collect_stats != 0, h->stream);
+    if (e == hipSuccess) e = h->enqueue_cells();
     if (e == hipSuccess) e = hipEventRecord(ev[2], h->stream);
     if (e == hipSuccess) e = launch_dt(h->d, h->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[3], h->stream);

@@ -19,9 +19,11 @@ struct Scalars {
   double last_speed;  // max signal speed of the last finished step
   unsigned long long steps;
   unsigned long long c2p_fail[6];
-  unsigned long long fix_calls, fix_fail, fallback, newton_iters, mlp_starts;
+  unsigned long long fix_calls, fix_fail, fallback, newton_iters;
   unsigned long long hist[kHistBins];
   unsigned long long max_residual_bits;
+  unsigned long long mlp_starts;
+  unsigned int mlp_count;  // cells appended to Dev::mlp_list in the current step
 };
 
 // Everything a kernel needs, passed by value.  Cells and faces are in the
@@ -56,6 +58,7 @@ struct Dev {
   // surrogate c2p guess (RHLLC_GUESS_MLP): FP32 model and bf16 W2
   const float *mlpP;
   const short *mlpW2b;
+  int32_t *mlp_list;  // kMixedMlp: cells deferred to k_cell_mlp (n_cells entries)
   Scalars *sc;
   double *speed_slots;  // kSpeedSlots * kSlotStride
 };
@@ -70,6 +73,7 @@ enum Variant : int {
 hipError_t launch_p2c_init(const Dev &d, hipStream_t s);
 hipError_t launch_face(const Dev &d, bool mixed, hipStream_t s);
 hipError_t launch_cell(const Dev &d, int variant, bool rad, bool stats, hipStream_t s);
+hipError_t launch_cell_mlp(const Dev &d, bool rad, bool stats, hipStream_t s);
 hipError_t launch_dt(const Dev &d, hipStream_t s);
 hipError_t launch_clear_scalars(const Dev &d, hipStream_t s);
 // Mixed path: W (FP64) recovered from U by c2p (start from the FP32 pressure).

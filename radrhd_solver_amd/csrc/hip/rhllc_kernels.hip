@@ -158,58 +158,54 @@ __device__ __forceinline__ double mlp_guess(const Dev &d, bool live, const State
 // Surrogate start only where the previous pressure is stale: the cell's
 // tau = E - D changed by more than this fraction in the step.  From a start
 // off by e, Newton needs about 2.4 iterations at e = 1e-3, 2.8 at 1e-2 and
-// 3.1 at 6e-2; the surrogate's start is off by ~1-2%, so below a few percent
-// change the previous pressure is the better start.
+// 3.1 at 6e-2; the surrogate's start is off by ~0.5%, so below a few percent
+// change the previous pressure is as good a start and costs nothing.
 constexpr double kMlpGate = 0.02;
 
-template <int VARIANT, bool RAD, bool STATS>
-__global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
-  constexpr bool MIXED = VARIANT >= kMixedPrev;
-  const int c = (int)(xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x);
-  const bool live = c < d.n_cells;
-  int iters = 0, code = 0;
-  bool fell_back = false, fixed = false, fix_failed = false, stale = false;
-  double resid = 0.0;
-  const double tau = d.sc->tau;
-  State<double> U, W;
-  if (live) {
-    const int4 cf = d.cell_faces[c];
-    const int fidx[4] = {cf.x, cf.y, cf.z, cf.w};
-    State<double> s;
-    s.d = 0.0, s.v[0] = 0.0, s.v[1] = 0.0, s.v[2] = 0.0, s.p = 0.0;
+// Gather of the four face fluxes in slot order with their signs
+// (rhllc_3d_stab.cpp:30-45) and U -= tau/V * sum.  Also returns the change of
+// tau = E - D (for the surrogate gate).
+template <bool MIXED>
+__device__ __forceinline__ void update_U(const Dev &d, int c, double tau, State<double> &U, double &dtau) {
+  const int4 cf = d.cell_faces[c];
+  const int fidx[4] = {cf.x, cf.y, cf.z, cf.w};
+  State<double> s;
+  s.d = 0.0, s.v[0] = 0.0, s.v[1] = 0.0, s.v[2] = 0.0, s.p = 0.0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      State<double> F;
-      load_flux<MIXED>(d, fidx[j] & 0x7FFFFFFF, F);
-      if (fidx[j] >= 0) {
-        s.d += F.d, s.v[0] += F.v[0], s.v[1] += F.v[1], s.v[2] += F.v[2], s.p += F.p;
-      } else {
-        s.d -= F.d, s.v[0] -= F.v[0], s.v[1] -= F.v[1], s.v[2] -= F.v[2], s.p -= F.p;
-      }
+  for (int j = 0; j < 4; ++j) {
+    State<double> F;
+    load_flux<MIXED>(d, fidx[j] & 0x7FFFFFFF, F);
+    if (fidx[j] >= 0) {
+      s.d += F.d, s.v[0] += F.v[0], s.v[1] += F.v[1], s.v[2] += F.v[2], s.p += F.p;
+    } else {
+      s.d -= F.d, s.v[0] -= F.v[0], s.v[1] -= F.v[1], s.v[2] -= F.v[2], s.p -= F.p;
     }
-    const double k = MIXED ? Fast::div(tau, d.cell_V[c]) : tau / d.cell_V[c];
-    s.d *= k, s.v[0] *= k, s.v[1] *= k, s.v[2] *= k, s.p *= k;
-    load_U(d, c, U);
-    U.d -= s.d, U.v[0] -= s.v[0], U.v[1] -= s.v[1], U.v[2] -= s.v[2], U.p -= s.p;
-    if (VARIANT == kMixedMlp) stale = !(fabs(s.p - s.d) <= kMlpGate * fabs(U.p - U.d));
   }
-  // Start of the Newton iteration.  The reference start ignores the previous
-  // W; the FP64 previous-pressure variant reads the FP64 pressure, the mixed
-  // path the FP32 one.  The surrogate variant takes the previous pressure too,
-  // except in stale cells, where the MLP evaluated on the updated U gives the
-  // start; a wave runs the network (all 64 lanes, MFMA) only if one of its
-  // cells is stale.
-  double guess = -1.0;
-  if (VARIANT == kFp64Prev && live) guess = d.Wp[c];
-  if ((VARIANT == kMixedPrev || VARIANT == kMixedMlp) && live) guess = (double)d.Wpf[c];
-  if (VARIANT == kMixedMlp && __ballot(stale) != 0) {
-    const double g = mlp_guess(d, live, U);
-    if (stale) guess = g;
-  }
+  const double k = MIXED ? Fast::div(tau, d.cell_V[c]) : tau / d.cell_V[c];
+  s.d *= k, s.v[0] *= k, s.v[1] *= k, s.v[2] *= k, s.p *= k;
+  load_U(d, c, U);
+  U.d -= s.d, U.v[0] -= s.v[0], U.v[1] -= s.v[1], U.v[2] -= s.v[2], U.p -= s.p;
+  dtau = s.p - s.d;
+}
+
+// Everything after the update for one cell: c2p from `guess` with the
+// fallback chain, PhysPressureFix, the optional radiation source with its
+// second c2p, the stores and the counters.  Called by all lanes of a wave
+// (ballots) and, with STATS, by all threads of the block (histogram).
+//   first fallback: the previous pressure (when the start was the surrogate's)
+//   then the reference start (all variants but kFp64Ref), then PhysPressureFix
+template <int VARIANT, bool RAD, bool STATS>
+__device__ __forceinline__ void finish_cell(const Dev &d, int c, bool live, State<double> U, double guess,
+                                            bool from_mlp, double tau) {
+  constexpr bool MIXED = VARIANT >= kMixedPrev;
+  int iters = 0, code = 0;
+  bool fell_back = false, fixed = false, fix_failed = false;
+  double resid = 0.0;
+  State<double> W;
   if (live) {
     // c2p writes W only when it succeeds.
     code = MIXED ? c2p<Fast>(d.Gg, U, W, guess, iters) : c2p(d.Gg, U, W, guess, iters);
-    if (code && VARIANT == kMixedMlp && stale) {
+    if (code && from_mlp) {
       int it2 = 0;
       fell_back = true;
       code = c2p<Fast>(d.Gg, U, W, (double)d.Wpf[c], it2);
@@ -273,7 +269,6 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
   count_if(&d.sc->fix_calls, fixed);
   count_if(&d.sc->fix_fail, fix_failed);
   count_if(&d.sc->fallback, fell_back);
-  if (VARIANT == kMixedMlp) count_if(&d.sc->mlp_starts, stale);
   if (STATS) {
     __shared__ unsigned int hist[kHistBins];
     __shared__ unsigned long long isum[kBlock / 64];
@@ -294,6 +289,64 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
       atomicAdd(&d.sc->newton_iters, t);
       atomicMax(&d.sc->max_residual_bits, (unsigned long long)__double_as_longlong(r));
     }
+    __syncthreads();  // the shared arrays are reused if the caller loops
+  }
+}
+
+// One thread per cell.  Start of the Newton iteration: the reference start
+// (kFp64Ref), the FP64 previous pressure (kFp64Prev) or the FP32 one (mixed).
+// kMixedMlp: cells whose tau changed by more than kMlpGate are not finished
+// here; they are appended to d.mlp_list (one atomic per wave) and finished by
+// k_cell_mlp, which starts their c2p from the surrogate on full waves.
+template <int VARIANT, bool RAD, bool STATS>
+__global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
+  constexpr bool MIXED = VARIANT >= kMixedPrev;
+  const int c = (int)(xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x);
+  bool live = c < d.n_cells;
+  const double tau = d.sc->tau;
+  State<double> U;
+  double dtau = 0.0;
+  if (live) update_U<MIXED>(d, c, tau, U, dtau);
+  if (VARIANT == kMixedMlp) {
+    const bool stale = live && !(fabs(dtau) <= kMlpGate * fabs(U.p - U.d));
+    const unsigned long long m = __ballot(stale);
+    if (m) {
+      const int lane = threadIdx.x & 63;
+      const int first = __ffsll((long long)m) - 1;
+      unsigned int base = 0;
+      if (lane == first) base = atomicAdd(&d.sc->mlp_count, (unsigned int)__popcll(m));
+      base = __shfl(base, first, 64);
+      if (stale) {
+        d.mlp_list[base + __popcll(m & ((1ull << lane) - 1ull))] = c;
+        d.U4[c] = make_double4(U.d, U.v[0], U.v[1], U.v[2]);
+        d.UE[c] = U.p;
+        live = false;
+      }
+    }
+  }
+  double guess = -1.0;
+  if (VARIANT == kFp64Prev && live) guess = d.Wp[c];
+  if (MIXED && live) guess = (double)d.Wpf[c];
+  finish_cell<VARIANT, RAD, STATS>(d, c, live, U, guess, false, tau);
+}
+
+// Cells deferred by k_cell<kMixedMlp>: list entries [0, mlp_count), 64 per
+// wave so the surrogate's MFMA pass runs on full waves.  A fixed grid loops
+// over the list (the count is only known on the device); every block runs
+// the same number of iterations, and all of them end when the list does.
+template <bool RAD, bool STATS>
+__global__ __launch_bounds__(kBlock) void k_cell_mlp(Dev d) {
+  const unsigned int n = d.sc->mlp_count;
+  const double tau = d.sc->tau;
+  for (unsigned int b0 = blockIdx.x * kBlock; b0 < n; b0 += gridDim.x * kBlock) {
+    const unsigned int i = b0 + threadIdx.x;
+    const bool live = i < n;
+    const int c = live ? d.mlp_list[i] : 0;
+    State<double> U;
+    U.d = 1.0, U.v[0] = 0.0, U.v[1] = 0.0, U.v[2] = 0.0, U.p = 2.0;  // dummy lanes
+    if (live) load_U(d, c, U);
+    const double g = mlp_guess(d, live, U);
+    finish_cell<kMixedMlp, RAD, STATS>(d, c, live, U, g, true, tau);
   }
 }
 
@@ -319,6 +372,8 @@ __global__ __launch_bounds__(1024) void k_dt(Dev d) {
     sc->last_speed = s;
     sc->tau = d.cfl_hmin / s;
     sc->steps += 1;
+    sc->mlp_starts += sc->mlp_count;
+    sc->mlp_count = 0;
   }
 }
 
@@ -361,6 +416,7 @@ __global__ void k_clear(Dev d) {
     sc->steps = 0;
     for (int k = 0; k < 6; ++k) sc->c2p_fail[k] = 0;
     sc->fix_calls = sc->fix_fail = sc->fallback = sc->newton_iters = sc->mlp_starts = 0;
+    sc->mlp_count = 0;
     for (int k = 0; k < kHistBins; ++k) sc->hist[k] = 0;
     sc->max_residual_bits = 0;
   }
@@ -416,6 +472,26 @@ hipError_t launch_cell(const Dev &d, int variant, bool rad, bool stats, hipStrea
       break;
     default:
       return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Grid of the list kernel: enough blocks to fill the chip; each loops over
+// the list, whose length is read on the device.
+hipError_t launch_cell_mlp(const Dev &d, bool rad, bool stats, hipStream_t s) {
+  if (d.n_cells == 0) return hipSuccess;
+  const unsigned int nb = blocks_for(d.n_cells) < 2048u ? blocks_for(d.n_cells) : 2048u;
+  const dim3 g(nb), b(kBlock);
+  if (rad) {
+    if (stats)
+      hipLaunchKernelGGL((k_cell_mlp<true, true>), g, b, 0, s, d);
+    else
+      hipLaunchKernelGGL((k_cell_mlp<true, false>), g, b, 0, s, d);
+  } else {
+    if (stats)
+      hipLaunchKernelGGL((k_cell_mlp<false, true>), g, b, 0, s, d);
+    else
+      hipLaunchKernelGGL((k_cell_mlp<false, false>), g, b, 0, s, d);
   }
   return hipGetLastError();
 }
