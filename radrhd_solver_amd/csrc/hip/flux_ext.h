@@ -1,11 +1,17 @@
-// Packed FP64 5-vectors of the mixed path: the face flux and the conserved
-// state U.  Each component is stored as an FP32 value hi plus a signed 6-bit
-// count e of 2^(E-156) units, E the biased exponent of hi (1/64 of hi's ulp
-// for normal hi), so x = hi + e 2^(E-156) holds x to 2^-30 relative.
-// FP32 alone (2^-24) is not enough for either: the momentum flux p n S of a
-// cell's four faces cancels while their roundings do not, and U accumulates
-// one rounding per step.  Record: float4 (c0..c3) + uint2 (bits of c4,
-// e0 | e1 << 6 | e2 << 12 | e3 << 18 | e4 << 24) = 24 bytes.
+// FP32 values with a small integer extension, for the mixed path's stored
+// FP64 quantities: hi = (float)x plus a signed BITS-bit count e of
+// 2^(E - 150 - (BITS - 1)) units, E the biased exponent of hi, i.e.
+// 2^-(BITS-1) of hi's ulp for normal hi.  x ~ hi + e 2^(E-150-(BITS-1)) to
+// 2^-(23+BITS) relative (the remainder is rounded; |x - hi| <= ulp/2).
+//
+// Used for
+//  * the face flux S*(D, m, E): float4 (D, mx, my, mz) + uint2 (bits of E,
+//    five 6-bit counts) = 24 bytes, 2^-29 relative.  FP32 alone is not
+//    enough: the momentum flux p n S of a cell's faces cancels around the
+//    cell while the roundings do not, and at Lorentz factors ~10 c2p
+//    amplifies an error in E by W^2 E/p (~1e3-1e4) into the pressure;
+//  * the cell record's s = rho h W^2 and D (8-bit counts): they must match
+//    the record's FP32 v and p far better than FP32 rounding does.
 // Host and device use the same functions.
 #pragma once
 
@@ -28,20 +34,25 @@ __host__ __device__ __forceinline__ float bits_f32(uint32_t u) {
   return x;
 }
 
-// hi = (float)x; *e = the 6-bit remainder field.
+// hi = (float)x; e = the remainder in units, rounded (the value 2^(BITS-1)
+// that |x - hi| = ulp/2 can give is stored as 2^(BITS-1) - 1).
+template <int BITS>
 __host__ __device__ __forceinline__ float split(double x, uint32_t &e) {
+  constexpr int kTop = (1 << (BITS - 1)) - 1;
   const float hi = (float)x;
   const int E = (int)((f32_bits(hi) >> 23) & 0xFFu);
-  double q = rint(ldexp(x - (double)hi, 156 - E));
-  q = fmin(fmax(q, -31.0), 31.0);  // NaN remainders give -31: hi is NaN/inf then
-  e = (uint32_t)((int)q & 63);
+  int q = (int)rint(ldexp(x - (double)hi, 150 + (BITS - 1) - E));
+  q = q < kTop ? q : kTop;
+  e = (uint32_t)q & ((1u << BITS) - 1u);
   return hi;
 }
 
-__host__ __device__ __forceinline__ double join(float hi, uint32_t e6) {
+template <int BITS>
+__host__ __device__ __forceinline__ double join(float hi, uint32_t e) {
+  constexpr uint32_t kSign = 1u << (BITS - 1);
   const int E = (int)((f32_bits(hi) >> 23) & 0xFFu);
-  const int e = (int)((e6 & 63u) ^ 32u) - 32;
-  return (double)hi + ldexp((double)e, E - 156);
+  const int q = (int)((e & ((1u << BITS) - 1u)) ^ kSign) - (int)kSign;
+  return (double)hi + ldexp((double)q, E - 150 - (BITS - 1));
 }
 
 struct Packed {
@@ -49,24 +60,25 @@ struct Packed {
   uint2 b;
 };
 
-__host__ __device__ __forceinline__ Packed pack(double c0, double c1, double c2, double c3, double c4) {
+// Face flux record (6-bit counts for all five components).
+__host__ __device__ __forceinline__ Packed pack(double d, double mx, double my, double mz, double E) {
   Packed o;
   uint32_t e0, e1, e2, e3, e4;
-  o.a.x = split(c0, e0);
-  o.a.y = split(c1, e1);
-  o.a.z = split(c2, e2);
-  o.a.w = split(c3, e3);
-  o.b.x = f32_bits(split(c4, e4));
+  o.a.x = split<6>(d, e0);
+  o.a.y = split<6>(mx, e1);
+  o.a.z = split<6>(my, e2);
+  o.a.w = split<6>(mz, e3);
+  o.b.x = f32_bits(split<6>(E, e4));
   o.b.y = e0 | (e1 << 6) | (e2 << 12) | (e3 << 18) | (e4 << 24);
   return o;
 }
 
 __host__ __device__ __forceinline__ void unpack(const float4 &a, const uint2 &b, double out[5]) {
-  out[0] = join(a.x, b.y);
-  out[1] = join(a.y, b.y >> 6);
-  out[2] = join(a.z, b.y >> 12);
-  out[3] = join(a.w, b.y >> 18);
-  out[4] = join(bits_f32(b.x), b.y >> 24);
+  out[0] = join<6>(a.x, b.y);
+  out[1] = join<6>(a.y, b.y >> 6);
+  out[2] = join<6>(a.z, b.y >> 12);
+  out[3] = join<6>(a.w, b.y >> 18);
+  out[4] = join<6>(bits_f32(b.x), b.y >> 24);
 }
 
 }  // namespace rhllc_ext

@@ -264,8 +264,10 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
   HIP_TRY(h->alloc(&d.UE, N));
   HIP_TRY(h->alloc(&d.W4, N));
   HIP_TRY(h->alloc(&d.Wp, N));
-  HIP_TRY(h->alloc(&d.W4f, N));
-  HIP_TRY(h->alloc(&d.Wpf, N));
+  if (h->mixed) {
+    HIP_TRY(h->alloc(&d.Wva, N));
+    HIP_TRY(h->alloc(&d.Wsd, N));
+  }
   if (variant == kMixedMlp) HIP_TRY(h->alloc(&d.mlp_list, N));
   HIP_TRY(h->alloc(&d.sc, 1));
   HIP_TRY(h->alloc(&d.speed_slots, (size_t)kSpeedSlots * kSlotStride));
@@ -294,16 +296,12 @@ extern "C" int rhllc_upload_state(rhllc_solver *h, const void *recs, int64_t str
   const uint8_t *base = static_cast<const uint8_t *>(recs);
   std::vector<double4> w4(N), u4(off_conv >= 0 ? N : 0);
   std::vector<double> wp(N), ue(off_conv >= 0 ? N : 0);
-  std::vector<float4> w4f(N);
-  std::vector<float> wpf(N);
   for (int32_t i = 0; i < N; ++i) {
     const uint8_t *r = base + (size_t)h->cell_new_to_old[i] * (size_t)stride;
     double w[5];
     std::memcpy(w, r + off_phys, 40);
     w4[i] = make_double4(w[0], w[1], w[2], w[3]);
     wp[i] = w[4];
-    w4f[i] = make_float4((float)w[0], (float)w[1], (float)w[2], (float)w[3]);
-    wpf[i] = (float)w[4];
     if (off_conv >= 0) {
       double u[5];
       std::memcpy(u, r + off_conv, 40);
@@ -317,12 +315,13 @@ extern "C" int rhllc_upload_state(rhllc_solver *h, const void *recs, int64_t str
   if (off_conv >= 0) {
     HIP_TRY(hipMemcpy(d.U4, u4.data(), sizeof(double4) * N, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.UE, ue.data(), sizeof(double) * N, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.W4f, w4f.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.Wpf, wpf.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    if (h->mixed) HIP_TRY(launch_seed_mixed(d, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
   } else {
     // rhllc::Init (rhllc_init.cpp:190-192): HllcPhysToConv, max_signal_speed = 1,
     // tau = GetTimeStep.
     HIP_TRY(launch_p2c_init(d, h->stream));
+    if (h->mixed) HIP_TRY(launch_seed_mixed(d, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipMemcpy(h->host_sc, d.sc, sizeof(Scalars), hipMemcpyDeviceToHost));
     h->host_sc->tau = d.cfl_hmin / 1.0;
@@ -582,7 +581,7 @@ extern "C" int rhllc_info(rhllc_solver *h, int64_t *n_cells, int64_t *n_faces, i
     const int64_t N = h->n_cells, F = h->n_faces;
     // Algorithmic bytes of one step (both kernels), each array element counted
     // once per kernel that touches it.
-    *device_bytes = h->mixed ? 148 * N + 68 * F : 233 * N + 120 * F;
+    *device_bytes = h->mixed ? 172 * N + 68 * F : 233 * N + 120 * F;
     if (h->rad) *device_bytes += 24 * N;
   }
   return RHLLC_OK;

@@ -39,8 +39,8 @@ struct Dev {
   const float3 *face_Sn32;  // S * (nx, ny, nz) in FP32, mixed path
   double4 *F4;              // flux (d, mx, my, mz) * S, FP64 path
   double *FE;               // flux E * S
-  float4 *F4f;              // mixed path: packed flux * S (flux_ext.h)
-  uint2 *FXf;
+  float4 *F4f;              // mixed path: flux * S, FP32 parts (flux_ext.h)
+  uint2 *FXf;               // mixed path: bits of the E part, extension counts
   // cells
   const int4 *cell_faces;  // face index | 0x80000000 when the cell is id_r (sign bit 0)
   const double *cell_V;
@@ -48,8 +48,8 @@ struct Dev {
   double *UE;    // E
   double4 *W4;   // (rho, vx, vy, vz), FP64 path
   double *Wp;    // p (both paths: the FP64 pressure)
-  float4 *W4f;   // (rho, vx, vy, vz), FP32 copy read by the mixed face kernel
-  float *Wpf;    // p, FP32 copy
+  float4 *Wva;   // mixed path: (vx, vy, vz, p) in FP32 (mixed_record)
+  float4 *Wsd;   // mixed path: (s, D, cs2, extension counts of s and D)
   // radiation source (optional): (Fx, Fy, Fz, Ur) and (kappa_a, kappa_s)
   const float4 *rad_FU;
   const float2 *rad_k;
@@ -77,6 +77,7 @@ hipError_t launch_cell_mlp(const Dev &d, bool rad, bool stats, hipStream_t s);
 hipError_t launch_dt(const Dev &d, hipStream_t s);
 hipError_t launch_clear_scalars(const Dev &d, hipStream_t s);
 // Mixed path: W (FP64) recovered from U by c2p (start from the FP32 pressure).
+hipError_t launch_seed_mixed(const Dev &d, hipStream_t s);
 hipError_t launch_recover_W(const Dev &d, hipStream_t s);
 
 hipError_t launch_copy(const void *src, void *dst, int64_t n16, hipStream_t s);

@@ -48,9 +48,17 @@ __device__ __forceinline__ void load_W(const Dev &d, int c, State<double> &W) {
   const double4 a = d.W4[c];
   W.d = a.x, W.v[0] = a.y, W.v[1] = a.z, W.v[2] = a.w, W.p = d.Wp[c];
 }
-__device__ __forceinline__ void load_W32(const Dev &d, int c, State<float> &W) {
-  const float4 a = d.W4f[c];
-  W.d = a.x, W.v[0] = a.y, W.v[1] = a.z, W.v[2] = a.w, W.p = d.Wpf[c];
+// The FP32 primitives of the mixed path, widened (rho = D sqrt(1 - v^2)).
+__device__ __forceinline__ void load_W32(const Dev &d, int c, State<double> &W) {
+  const MixedIn r = mixed_in(d.Wva[c], d.Wsd[c]);
+  W.v[0] = r.v[0], W.v[1] = r.v[1], W.v[2] = r.v[2], W.p = r.p;
+  W.d = r.D * sqrt(1.0 - dot3(W.v, W.v));
+}
+__device__ __forceinline__ void store_W32(const Dev &d, int c, const State<double> &W) {
+  float4 va, sd;
+  mixed_record(d.gamma, d.Gg, W, va, sd);
+  d.Wva[c] = va;
+  d.Wsd[c] = sd;
 }
 
 // Per-block max of non-negative speeds into one of kSpeedSlots slots.
@@ -94,21 +102,22 @@ __global__ __launch_bounds__(kBlock) void k_face(Dev d) {
       d.F4[i] = make_double4(F.d, F.v[0], F.v[1], F.v[2]);
       d.FE[i] = F.p;
     } else {
-      // FP32 storage (state, geometry, flux), FP64 arithmetic with the fast
-      // divide/sqrt policy; U is rebuilt from the FP32 primitives.
+      // FP32 storage (cell records, geometry, flux), FP64 arithmetic with the
+      // fast divide/sqrt policy.
       const float3 Sn = d.face_Sn32[i];
-      State<double> WL, WR, F;
-      {
-        const float4 a = d.W4f[lr.x];
-        WL.d = a.x, WL.v[0] = a.y, WL.v[1] = a.z, WL.v[2] = a.w, WL.p = d.Wpf[lr.x];
-      }
+      State<double> F;
+      const MixedIn WL = mixed_in(d.Wva[lr.x], d.Wsd[lr.x]);
+      MixedIn WR;
       if (lr.y >= 0) {
-        const float4 a = d.W4f[lr.y];
-        WR.d = a.x, WR.v[0] = a.y, WR.v[1] = a.z, WR.v[2] = a.w, WR.p = d.Wpf[lr.y];
+        WR = mixed_in(d.Wva[lr.y], d.Wsd[lr.y]);
       } else {
-        WR.d = d.ghostW[0], WR.v[0] = d.ghostW[1], WR.v[1] = d.ghostW[2], WR.v[2] = d.ghostW[3], WR.p = d.ghostW[4];
+        State<double> g;
+        g.d = d.ghostW[0], g.v[0] = d.ghostW[1], g.v[1] = d.ghostW[2], g.v[2] = d.ghostW[3], g.p = d.ghostW[4];
+        float4 va, sd;
+        mixed_record(d.gamma, d.Gg, g, va, sd);
+        WR = mixed_in(va, sd);
       }
-      const double s = mixed_face_flux(d.gamma, d.Gg, WL, WR, Sn.x, Sn.y, Sn.z, F);
+      const double s = mixed_face_flux(WL, WR, Sn.x, Sn.y, Sn.z, F);
       speed = smax(speed, s);
       const rhllc_ext::Packed pk = rhllc_ext::pack(F.d, F.v[0], F.v[1], F.v[2], F.p);
       d.F4f[i] = pk.a;
@@ -208,7 +217,7 @@ __device__ __forceinline__ void finish_cell(const Dev &d, int c, bool live, Stat
     if (code && from_mlp) {
       int it2 = 0;
       fell_back = true;
-      code = c2p<Fast>(d.Gg, U, W, (double)d.Wpf[c], it2);
+      code = c2p<Fast>(d.Gg, U, W, (double)d.Wva[c].w, it2);
       iters += it2;
     }
     if (code && VARIANT != kFp64Ref) {
@@ -219,12 +228,10 @@ __device__ __forceinline__ void finish_cell(const Dev &d, int c, bool live, Stat
     }
     if (code) {
       // PhysPressureFix receives the cell's previous phys_val (rhllc_3d_stab.cpp:49-52).
-      if (!MIXED) {
+      if (!MIXED)
         load_W(d, c, W);
-      } else {
-        const float4 a = d.W4f[c];
-        W.d = a.x, W.v[0] = a.y, W.v[1] = a.z, W.v[2] = a.w, W.p = d.Wpf[c];
-      }
+      else
+        load_W32(d, c, W);
       fixed = true;
       fix_failed = pressure_fix(d.Gg, U, W) != 0;
     }
@@ -257,8 +264,7 @@ __device__ __forceinline__ void finish_cell(const Dev &d, int c, bool live, Stat
       d.W4[c] = make_double4(W.d, W.v[0], W.v[1], W.v[2]);
       d.Wp[c] = W.p;
     } else {
-      d.W4f[c] = make_float4((float)W.d, (float)W.v[0], (float)W.v[1], (float)W.v[2]);
-      d.Wpf[c] = (float)W.p;
+      store_W32(d, c, W);
     }
   }
   count_if(&d.sc->c2p_fail[1], live && code == 1);
@@ -326,7 +332,7 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
   }
   double guess = -1.0;
   if (VARIANT == kFp64Prev && live) guess = d.Wp[c];
-  if (MIXED && live) guess = (double)d.Wpf[c];
+  if (MIXED && live) guess = (double)d.Wva[c].w;
   finish_cell<VARIANT, RAD, STATS>(d, c, live, U, guess, false, tau);
 }
 
@@ -377,8 +383,7 @@ __global__ __launch_bounds__(1024) void k_dt(Dev d) {
   }
 }
 
-// HllcPhysToConv (rhllc_utils.cpp:25-33): U = p2c(W), W clamped in place;
-// also seeds the FP32 primitive copy.
+// HllcPhysToConv (rhllc_utils.cpp:25-33): U = p2c(W), W clamped in place.
 __global__ __launch_bounds__(kBlock) void k_p2c_init(Dev d) {
   const int c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= d.n_cells) return;
@@ -389,8 +394,16 @@ __global__ __launch_bounds__(kBlock) void k_p2c_init(Dev d) {
   d.Wp[c] = W.p;
   d.U4[c] = make_double4(U.d, U.v[0], U.v[1], U.v[2]);
   d.UE[c] = U.p;
-  d.W4f[c] = make_float4((float)W.d, (float)W.v[0], (float)W.v[1], (float)W.v[2]);
-  d.Wpf[c] = (float)W.p;
+}
+
+// Mixed path: the FP32 cell records from the FP64 primitives (after an
+// upload or HllcPhysToConv).
+__global__ __launch_bounds__(kBlock) void k_seed_mixed(Dev d) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= d.n_cells) return;
+  State<double> W;
+  load_W(d, c, W);
+  store_W32(d, c, W);
 }
 
 // Mixed path: FP64 primitives from the FP64 conserved state (for download).
@@ -399,8 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_recover_W(Dev d) {
   if (c >= d.n_cells) return;
   State<double> U, W;
   load_U(d, c, U);
-  const float4 a = d.W4f[c];
-  W.d = a.x, W.v[0] = a.y, W.v[1] = a.z, W.v[2] = a.w, W.p = d.Wpf[c];
+  load_W32(d, c, W);
   int it = 0;
   c2p(d.Gg, U, W, W.p, it);  // W keeps the FP32 values if c2p fails
   d.W4[c] = make_double4(W.d, W.v[0], W.v[1], W.v[2]);
@@ -501,6 +513,11 @@ hipError_t launch_dt(const Dev &d, hipStream_t s) {
   return hipGetLastError();
 }
 
+hipError_t launch_seed_mixed(const Dev &d, hipStream_t s) {
+  if (d.n_cells == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_seed_mixed, dim3(blocks_for(d.n_cells)), dim3(kBlock), 0, s, d);
+  return hipGetLastError();
+}
 hipError_t launch_recover_W(const Dev &d, hipStream_t s) {
   if (d.n_cells == 0) return hipSuccess;
   hipLaunchKernelGGL(k_recover_W, dim3(blocks_for(d.n_cells)), dim3(kBlock), 0, s, d);
@@ -535,11 +552,14 @@ __global__ void k_eval_flux(double gamma, int64_t n, const double *UL, const dou
   } else {
     // The mixed face path: FP32 primitives and geometry, FP64 fast arithmetic,
     // flux rounded to FP32.
-    State<double> a, b, ff;
-    a.d = (float)wl.d, a.v[0] = (float)wl.v[0], a.v[1] = (float)wl.v[1], a.v[2] = (float)wl.v[2], a.p = (float)wl.p;
-    b.d = (float)wr.d, b.v[0] = (float)wr.v[0], b.v[1] = (float)wr.v[1], b.v[2] = (float)wr.v[2], b.p = (float)wr.p;
+    // The cell records the mixed cell kernel would store for these states.
+    State<double> ff;
+    float4 la, ls, ra, rs;
+    mixed_record(gamma, Gg, wl, la, ls);
+    mixed_record(gamma, Gg, wr, ra, rs);
     const double *g = nS4 + 4 * i;
-    speed[i] = mixed_face_flux(gamma, Gg, a, b, (float)(g[0] * g[3]), (float)(g[1] * g[3]), (float)(g[2] * g[3]), ff);
+    speed[i] = mixed_face_flux(mixed_in(la, ls), mixed_in(ra, rs), (float)(g[0] * g[3]), (float)(g[1] * g[3]),
+                               (float)(g[2] * g[3]), ff);
     const rhllc_ext::Packed pk = rhllc_ext::pack(ff.d, ff.v[0], ff.v[1], ff.v[2], ff.p);
     rhllc_ext::unpack(pk.a, pk.b, F + 5 * i);
   }

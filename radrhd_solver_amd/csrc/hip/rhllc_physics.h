@@ -22,6 +22,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "flux_ext.h"
+
 namespace rhllc_dev {
 
 constexpr int kMaxIter = 40;  // rhllc_flux_stab.cpp:8
@@ -55,11 +57,13 @@ __device__ __forceinline__ R dot3(const R *a, const R *b) {
 // the root take one step and one correction of the result, which brings both
 // to within about an ulp.
 struct Ieee {
+  static constexpr bool kFast = false;
   __device__ static __forceinline__ double div(double a, double b) { return a / b; }
   __device__ static __forceinline__ double rcp(double x) { return 1.0 / x; }
   __device__ static __forceinline__ double sqrt_(double x) { return sqrt(x); }
 };
 struct Fast {
+  static constexpr bool kFast = true;
   __device__ static __forceinline__ double rcp(double x) {
     double r = __builtin_amdgcn_rcp(x);
     r = fma(r, fma(-x, r, 1.0), r);
@@ -73,11 +77,13 @@ struct Fast {
   }
   __device__ static __forceinline__ double sqrt_(double x) {
     // y ~ 1/sqrt(x), one Newton step, then s = x*y with one correction.
-    if (!(x > 0.0)) return sqrt(x);  // zero, negative, NaN: the IEEE result
+    // Negative or NaN x gives NaN through v_rsq_f64; x = 0 (rsq = inf) is
+    // returned as is.
     double y = __builtin_amdgcn_rsq(x);
     y = y * fma(-0.5 * x * y, y, 1.5);
     const double s = x * y;
-    return fma(fma(-s, s, x), 0.5 * y, s);
+    const double r = fma(fma(-s, s, x), 0.5 * y, s);
+    return x == 0.0 ? x : r;
   }
   __device__ static __forceinline__ double rsqrt(double x) {  // x > 0
     const double y = __builtin_amdgcn_rsq(x);
@@ -201,11 +207,19 @@ __device__ __forceinline__ int pressure_fix(double Gg, State<double> &U, State<d
 // max(|m| - E, 0, pmin); otherwise the start is max(guess, pmin).
 // Returns 0 or the reference's codes 1..4; 5 = the reference's DIE_IF(lor2 < 1).
 // On failure W is left unchanged.  *iters = Newton iterations done.
-// A = Ieee is the reference's arithmetic; the mixed path runs the same
-// operation sequence with the Fast divide/sqrt (a few ulp per operation).
-// The per-iteration form 1 / (1 - m^2/alpha^2) is kept on purpose: it is
-// exactly 1 for m = 0, where rearranged forms round to 1 - ulp half the time
-// and would trip the lor2 < 1 test on gas at rest.
+// A = Ieee is the reference's arithmetic, operation for operation.  The mixed
+// path (A = Fast) runs the same iteration with the Fast divide/sqrt and two
+// changes:
+//  * per iteration one reciprocal r = 1/(alpha (alpha^2 - m^2)) gives both
+//    Gamma^2 = alpha^3 r (clamped to >= 1: rounding takes it to 1 - ulp for
+//    gas at rest) and m^2 Gamma^3/alpha^3 = m^2 Gamma r; the reference's
+//    lor2 < 1 failure becomes alpha^2 - m^2 <= 0, the same states;
+//  * it also stops once |dp| < 1e-6 p.  Newton's next correction is then
+//    K dp^2 with K = |f''/2f'|, and K p <= 23 over the surrogate's training
+//    box (rho 1e-4..1e2, p/rho 1e-6..1e2, W <= 50, gamma 4/3 and 5/3), so the
+//    pressure is within 2.3e-11 p of the root and |f|/(E + p) <= ~1e-11 --
+//    the reference's own stop (|dp| < 1e-11 E) usually costs one more
+//    iteration to confirm what the previous one already reached.
 template <typename A = Ieee>
 __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<double> &W, double guess, int &iters) {
   const double tol = 1.e-11;
@@ -227,24 +241,39 @@ __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<doub
   for (iter = 0; iter < kMaxIter; iter++) {
     alpha = E + p;
     alpha2 = alpha * alpha;
-    lor2 = 1.0 - A::div(m2, alpha2);
-    lor2 = A::rcp(lor2);
-    if (lor2 < 1.0) {
-      iters = iter + 1;
-      return 5;
+    double mlor3;  // m^2 Gamma^3 / alpha^3
+    if (!A::kFast) {
+      lor2 = 1.0 - A::div(m2, alpha2);
+      lor2 = A::rcp(lor2);
+      if (lor2 < 1.0) {
+        iters = iter + 1;
+        return 5;
+      }
+      lor = A::sqrt_(lor2);
+      mlor3 = A::div(m2 * lor2 * lor, alpha2 * alpha);
+    } else {
+      const double q = alpha2 - m2;
+      if (!(q > 0.0)) {
+        iters = iter + 1;
+        return 5;
+      }
+      const double r = A::rcp(alpha * q);
+      lor2 = smax(alpha2 * alpha * r, 1.0);
+      lor = A::sqrt_(lor2);
+      mlor3 = m2 * lor * r;
     }
-    lor = A::sqrt_(lor2);
     tau = lor * D_1;
     theta = p * tau;
     h = 1.0 + Gg * theta;
     dh_dp = Gg * tau;
     dh_dtau = Gg * p;
     yp = D * h * lor - E - p;
-    dyp = D * lor * dh_dp - A::div(m2 * lor2 * lor, alpha2 * alpha) * (lor * dh_dtau + D * h) - 1.0;
+    dyp = D * lor * dh_dp - mlor3 * (lor * dh_dtau + D * h) - 1.0;
     dp = A::div(yp, dyp);
     p -= dp;
     if (p < pmin) p = pmin;
     if (fabs(dp) < tol * E) break;
+    if (A::kFast && fabs(dp) < 1e-6 * p) break;
   }
   iters = iter < kMaxIter ? iter + 1 : kMaxIter;
   if (p < 0.0) return 2;
@@ -363,39 +392,79 @@ __device__ __forceinline__ R hllc_flux(R G1, R Gg, const State<R> &cUL, const St
 
 // ---------------------------------------------------------------- mixed path
 // Face flux of the mixed path: FP32 primitives and geometry in, FP64
-// arithmetic with the Fast policy.  Same HLLC as hllc_flux (same signal
-// speeds, same quadratic for the contact speed, same side selection), written
-// without the rotation matrix: in the face frame every tangential flux
-// component is its tangential momentum times one factor c_t, so the global
-// momentum flux is c_t m + (F_n - c_t m_n) n, with F_n the normal component.
-// Only v.n and m.n of each side are needed.  U is rebuilt from the FP32
-// primitives (GetConvValueStab, with its |v| >= 1 clamp).
+// arithmetic with the Fast policy and FMA contraction.  Same HLLC as
+// hllc_flux (same signal speeds, same quadratic for the contact speed, same
+// side selection), written without the rotation matrix: in the face frame
+// every tangential flux component is its tangential momentum times one factor
+// c_t, so the global momentum flux is c_t m + (F_n - c_t m_n) n, with F_n the
+// normal component.  Only v.n and m.n of each side are needed.
+// What the face kernel reads of a cell on the mixed path, stored by the cell
+// kernel as two float4: (vx, vy, vz, p) in FP32, and (s, D, cs2, e) with
+// s = rho h W^2 (m = s v, E = s - p), D = rho W, cs2 = gamma p/(rho h).
+// s and D are formed in FP64 from the FP32-rounded primitives and stored as
+// FP32 plus 8-bit extensions packed in e (flux_ext.h), so they match the
+// stored v and p to 2^-31: at Lorentz factors ~10 an s rounded on its own
+// (inconsistent with v, p by 6e-8) raised the transverse-momentum L1 of a
+// shear layer from 1e-6 to 2e-5.  Forming these once per cell, instead of once
+// per face side, removes the p2c and sound-speed divides and roots from the
+// face kernel.  rho is not stored: rho = D sqrt(1 - v^2).
+struct MixedIn {
+  double v[3], p, s, D, cs2;
+};
+
+// The record of an FP64 primitive state (|v| < 1).
+__device__ __forceinline__ void mixed_record(double gamma, double Gg, const State<double> &W, float4 &va,
+                                             float4 &sd) {
+#pragma clang fp contract(fast)
+  using A = Fast;
+  va = make_float4((float)W.v[0], (float)W.v[1], (float)W.v[2], (float)W.p);
+  const double rho = (float)W.d, p = va.w;
+  const double v[3] = {va.x, va.y, va.z};
+  const double lor2 = A::rcp(1.0 - dot3(v, v));
+  const double rh = rho + Gg * p;  // rho h
+  uint32_t es, eD;
+  sd.x = rhllc_ext::split<8>(rh * lor2, es);
+  sd.y = rhllc_ext::split<8>(rho * A::sqrt_(lor2), eD);
+  sd.z = (float)A::div(gamma * p, rh);
+  sd.w = rhllc_ext::bits_f32(es | (eD << 8));
+}
+
+__device__ __forceinline__ MixedIn mixed_in(const float4 &va, const float4 &sd) {
+  MixedIn o;
+  const uint32_t e = rhllc_ext::f32_bits(sd.w);
+  o.v[0] = va.x, o.v[1] = va.y, o.v[2] = va.z, o.p = va.w;
+  o.s = rhllc_ext::join<8>(sd.x, e);
+  o.D = rhllc_ext::join<8>(sd.y, e >> 8);
+  o.cs2 = sd.z;
+  return o;
+}
+
 struct MixedSide {
-  double d, p, v[3];        // primitives (v after the clamp)
-  double D, E, s;           // conserved D, E and s = rho h W^2 (m = s v)
+  double p, v[3];           // primitives (v after the clamp)
+  double D, E, s;           // conserved D, E and s (m = s v)
   double vn, mn;            // v.n, m.n
   double cmin, cmax;        // signal speeds
 };
 
-__device__ __forceinline__ void mixed_side(double G1, double Gg, const State<double> &W, const double *n,
-                                           MixedSide &o) {
+__device__ __forceinline__ void mixed_side(const MixedIn &w, const double *n, MixedSide &o) {
+#pragma clang fp contract(fast)
   using A = Fast;
-  o.d = W.d, o.p = W.p;
-  o.v[0] = W.v[0], o.v[1] = W.v[1], o.v[2] = W.v[2];
+  o.p = w.p;
+  o.v[0] = w.v[0], o.v[1] = w.v[1], o.v[2] = w.v[2];
   double V2 = dot3(o.v, o.v);
-  if (V2 >= 1.0) {
-    const double g = A::div(0.9999, A::sqrt_(V2));
+  if (__builtin_expect(V2 >= 1.0, 0)) {
+    // GetConvValueStab's clamp, for a velocity that rounded to |v| >= 1 in
+    // FP32 (Lorentz factors above ~3000); s and D keep their FP64 values.
+    const double g = 0.9999 * A::rsqrt(V2);
     o.v[0] *= g, o.v[1] *= g, o.v[2] *= g;
     V2 = 0.9999 * 0.9999;
   }
-  const double h = 1.0 + A::div(Gg * W.p, W.d);
-  const double lor2 = A::rcp(1.0 - V2);
-  o.s = W.d * h * lor2;
-  o.D = W.d * A::sqrt_(lor2);
-  o.E = o.s - W.p;
+  o.s = w.s;
+  o.D = w.D;
+  o.E = w.s - w.p;
   o.vn = dot3(o.v, n);
   o.mn = o.s * o.vn;
-  const double cs2 = A::div(G1 * W.p, W.d * h);
+  const double cs2 = w.cs2;
   const double vn2 = o.vn * o.vn;
   const double sroot = A::sqrt_(cs2 * (1.0 - vn2 - (V2 - vn2) * cs2) * (1.0 - V2));
   const double r = A::rcp(1.0 - V2 * cs2);
@@ -406,11 +475,13 @@ __device__ __forceinline__ void mixed_side(double G1, double Gg, const State<dou
 
 // The face geometry is the FP32 vector S n; the unit normal and the area are
 // recovered in FP64 (n exactly unit to rounding, which the frame-free form
-// relies on: m = m_n n + m_t holds only for |n| = 1).
+// relies on: m = m_n n + m_t holds only for |n| = 1).  The contact solve is
+// evaluated for every face and its result taken only in the subsonic case
+// (lL < 0 < lR), so the face needs no branch.
 // F receives S * flux; returns max(lL, lR) as hllc_flux does.
-__device__ __forceinline__ double mixed_face_flux(double gamma, double Gg, const State<double> &WL,
-                                                  const State<double> &WR, float sx, float sy, float sz,
-                                                  State<double> &F) {
+__device__ __forceinline__ double mixed_face_flux(const MixedIn &WL, const MixedIn &WR, float sx, float sy,
+                                                  float sz, State<double> &F) {
+#pragma clang fp contract(fast)
   using A = Fast;
   double n[3] = {sx, sy, sz};
   const double q = dot3(n, n);
@@ -418,48 +489,40 @@ __device__ __forceinline__ double mixed_face_flux(double gamma, double Gg, const
   const double S = q * rn;
   n[0] *= rn, n[1] *= rn, n[2] *= rn;
   MixedSide L, R;
-  mixed_side(gamma, Gg, WL, n, L);
-  mixed_side(gamma, Gg, WR, n, R);
+  mixed_side(WL, n, L);
+  mixed_side(WR, n, R);
   const double lL = smin(L.cmin, R.cmin);
   const double lR = smax(L.cmax, R.cmax);
-  bool left;
-  double ct, Fn, FE;
-  if (lL >= 0.0 || lR <= 0.0) {
-    left = lL >= 0.0;
-    const MixedSide &X = left ? L : R;
-    ct = X.vn;
-    Fn = X.mn * X.vn + X.p;
-    FE = X.mn;
-  } else {
-    const double AL = lL * L.E - L.mn;
-    const double AR = lR * R.E - R.mn;
-    const double BL = lL * L.mn - L.mn * L.vn - L.p;
-    const double BR = lR * R.mn - R.mn * R.vn - R.p;
-    const double a = AR * lL - AL * lR;
-    const double b = AL + BL * lR - AR - BR * lL;
-    const double c = BR - BL;
-    const double sgn = b < 0.0 ? -1.0 : 1.0;
-    const double scrh = -0.5 * (b + sgn * A::sqrt_(b * b - 4.0 * a * c));
-    const double us = A::div(c, scrh);
-    const double ps = A::div(AL * us - BL, 1.0 - us * lL);
-    left = us >= 0.0;
-    const MixedSide &X = left ? L : R;
-    const double lam = left ? lL : lR;
-    const double dif = A::rcp(lam - us);
-    const double ms = (lam * (X.E + ps) - X.mn) * us * dif;  // star normal momentum
-    ct = X.vn + lam * ((lam - X.vn) * dif - 1.0);
-    Fn = X.mn * X.vn + lam * (ms - X.mn) + X.p;
-    FE = ms;  // F_E = m*_n in the star region
-  }
-  const MixedSide &X = left ? L : R;
-  const double cm = ct * X.s;                 // c_t m = c_t s v
-  const double cn = Fn - ct * X.mn;           // normal remainder (the pressure-like part)
-  const double dS = S;
-  F.d = X.D * ct * dS;
-  F.v[0] = (cm * X.v[0] + cn * n[0]) * dS;
-  F.v[1] = (cm * X.v[1] + cn * n[1]) * dS;
-  F.v[2] = (cm * X.v[2] + cn * n[2]) * dS;
-  F.p = FE * dS;
+  // contact solve (rhllc_flux_stab.cpp:362-421)
+  const double AL = lL * L.E - L.mn;
+  const double AR = lR * R.E - R.mn;
+  const double BL = lL * L.mn - L.mn * L.vn - L.p;
+  const double BR = lR * R.mn - R.mn * R.vn - R.p;
+  const double a = AR * lL - AL * lR;
+  const double b = AL + BL * lR - AR - BR * lL;
+  const double c = BR - BL;
+  const double sgn = b < 0.0 ? -1.0 : 1.0;
+  const double scrh = -0.5 * (b + sgn * A::sqrt_(b * b - 4.0 * a * c));
+  const double us = A::div(c, scrh);
+  const double ps = A::div(AL * us - BL, 1.0 - us * lL);
+  const bool sup = lL >= 0.0 || lR <= 0.0;
+  const bool left = sup ? lL >= 0.0 : us >= 0.0;
+  const double vn = left ? L.vn : R.vn, mn = left ? L.mn : R.mn, p = left ? L.p : R.p;
+  const double E = left ? L.E : R.E, D = left ? L.D : R.D, s = left ? L.s : R.s;
+  const double v0 = left ? L.v[0] : R.v[0], v1 = left ? L.v[1] : R.v[1], v2 = left ? L.v[2] : R.v[2];
+  const double lam = left ? lL : lR;
+  const double dif = A::rcp(lam - us);
+  const double ms = (lam * (E + ps) - mn) * us * dif;  // star normal momentum
+  const double ct = sup ? vn : vn + lam * ((lam - vn) * dif - 1.0);
+  const double Fn = sup ? mn * vn + p : mn * vn + lam * (ms - mn) + p;
+  const double FE = sup ? mn : ms;  // F_E = m*_n in the star region
+  const double cm = ct * s;         // c_t m = c_t s v
+  const double cn = Fn - ct * mn;   // normal remainder (the pressure-like part)
+  F.d = D * ct * S;
+  F.v[0] = (cm * v0 + cn * n[0]) * S;
+  F.v[1] = (cm * v1 + cn * n[1]) * S;
+  F.v[2] = (cm * v2 + cn * n[2]) * S;
+  F.p = FE * S;
   return smax(lL, lR);
 }
 

@@ -24,7 +24,10 @@ def _prims(rng, n, vmax=0.9):
 
 
 def test_eval_flux_fp32_close_to_fp64():
-    """Mixed face flux (FP32 inputs/geometry/output, FP64 fast arithmetic) vs the FP64 flux: 1e-5 of scale."""
+    """Mixed face flux (FP32 cell records and geometry, FP64 fast arithmetic, packed FP32 output) vs the
+    FP64 flux of the unrounded states: 2e-5 of scale at worst, 1e-7 typical.  The worst cases are set by
+    rounding the inputs to FP32 (states up to |v| = 0.9, where the flux's sensitivity to v grows like W^2),
+    not by the arithmetic."""
     rng = np.random.default_rng(SEED)
     n = 20000
     WL, WR = _prims(rng, n), _prims(rng, n)
@@ -39,7 +42,7 @@ def test_eval_flux_fp32_close_to_fp64():
     scale = np.abs(Fo).max(axis=1) + np.abs(WL[:, 4]) * nS[:, 3]
     err = np.abs(Fg - Fo).max(axis=1) / scale
     print("fp32 flux rel err: max", err.max(), "median", np.median(err))
-    assert err.max() < 1e-5
+    assert err.max() < 2e-5 and np.median(err) < 1e-7
     assert np.abs(sg - so).max() < 1e-6
 
 
@@ -68,3 +71,28 @@ def test_mixed_first_step_face_flux_and_state():
     print("first-step U rel L1", l1)
     assert ferr.max() < 1e-5
     assert max(l1) < 1e-6
+
+
+@pytest.mark.parametrize("case", ["blast", "shear"])
+def test_mixed_c2p_residual_and_iterations(case):
+    """Every cell of the mixed step ends at |f(p)|/(E + p) <= 1e-10 (the residual the FP64 step
+    reaches), with no c2p failure, on the blast wave and on a W ~ 10 periodic shear layer."""
+    if case == "blast":
+        m = M.kuhn_box(24, 24, 24, seed=SEED)
+        W0, gamma, steps = M.init_state(m, M.INIT_BLAST), 5.0 / 3.0, 40
+    else:
+        m = M.kuhn_box(24, 24, 24, origin=(-0.5, -0.5, -0.5), periodic=7, seed=SEED)
+        W0, gamma, steps = M.init_state(m, M.INIT_SHEAR), 4.0 / 3.0, 40
+    out = {}
+    for prec, guess in ((capi.PREC_FP64, capi.GUESS_REFERENCE), (capi.PREC_MIXED, capi.GUESS_PREVIOUS)):
+        s = capi.Solver(m, gamma=gamma, cfl=0.4, precision=prec, guess=guess, collect_stats=True)
+        s.upload_W(W0)
+        s.run(steps)
+        s.sync()
+        out[prec] = s.stats()
+        s.close()
+    for prec, st in out.items():
+        it = st["newton_iters"] / (m.n_cells * steps)
+        print(case, "precision", prec, "mean Newton iterations", round(it, 3), "max residual", st["max_residual"])
+        assert sum(st["c2p_fail"]) == 0 and st["fix_calls"] == 0
+        assert st["max_residual"] <= 1e-10
