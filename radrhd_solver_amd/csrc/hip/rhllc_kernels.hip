@@ -79,50 +79,62 @@ __device__ __forceinline__ void fold_speed(const Dev &d, double speed) {
   }
 }
 
+// One face: gather, HLLC flux, flux store; returns max(lL, lR).
+template <bool MIXED>
+__device__ __forceinline__ double face_flux(const Dev &d, int i) {
+  double speed = 0.0;  // rhllc_3d_stab.cpp:18 -- per-thread max starts at 0
+  const int2 lr = d.face_lr[i];
+  if (!MIXED) {
+    const double4 nS = d.face_nS[i];
+    State<double> UL, WL, UR, WR, F;
+    load_U(d, lr.x, UL);
+    load_W(d, lr.x, WL);
+    if (lr.y >= 0) {
+      load_U(d, lr.y, UR);
+      load_W(d, lr.y, WR);
+    } else {
+      UR.d = d.ghostU[0], UR.v[0] = d.ghostU[1], UR.v[1] = d.ghostU[2], UR.v[2] = d.ghostU[3], UR.p = d.ghostU[4];
+      WR.d = d.ghostW[0], WR.v[0] = d.ghostW[1], WR.v[1] = d.ghostW[2], WR.v[2] = d.ghostW[3], WR.p = d.ghostW[4];
+    }
+    const double s = hllc_flux<double>(d.gamma, d.Gg, UL, UR, WL, WR, nS.x, nS.y, nS.z, nS.w, F);
+    speed = smax(speed, s);
+    d.F4[i] = make_double4(F.d, F.v[0], F.v[1], F.v[2]);
+    d.FE[i] = F.p;
+  } else {
+    // FP32 storage (cell records, geometry, flux), FP64 arithmetic with the
+    // fast divide/sqrt policy.
+    const float3 Sn = d.face_Sn32[i];
+    State<double> F;
+    const MixedIn WL = mixed_in(d.Wva[lr.x], d.Wsd[lr.x]);
+    MixedIn WR;
+    if (lr.y >= 0) {
+      WR = mixed_in(d.Wva[lr.y], d.Wsd[lr.y]);
+    } else {
+      State<double> g;
+      g.d = d.ghostW[0], g.v[0] = d.ghostW[1], g.v[1] = d.ghostW[2], g.v[2] = d.ghostW[3], g.p = d.ghostW[4];
+      float4 va, sd;
+      mixed_record(d.gamma, d.Gg, g, va, sd);
+      WR = mixed_in(va, sd);
+    }
+    const double s = mixed_face_flux(WL, WR, Sn.x, Sn.y, Sn.z, F);
+    speed = smax(speed, s);
+    const rhllc_ext::Packed pk = rhllc_ext::pack(F.d, F.v[0], F.v[1], F.v[2], F.p);
+    d.F4f[i] = pk.a;
+    d.FXf[i] = pk.b;
+  }
+  return speed;
+}
+
+// kFacesPerThread faces per thread, block-strided so loads and stores stay
+// coalesced.
 template <bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_face(Dev d) {
-  const int i = (int)(xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x);
-  double speed = 0.0;  // rhllc_3d_stab.cpp:18 -- per-thread max starts at 0
-  if (i < d.n_faces) {
-    const int2 lr = d.face_lr[i];
-    if (!MIXED) {
-      const double4 nS = d.face_nS[i];
-      State<double> UL, WL, UR, WR, F;
-      load_U(d, lr.x, UL);
-      load_W(d, lr.x, WL);
-      if (lr.y >= 0) {
-        load_U(d, lr.y, UR);
-        load_W(d, lr.y, WR);
-      } else {
-        UR.d = d.ghostU[0], UR.v[0] = d.ghostU[1], UR.v[1] = d.ghostU[2], UR.v[2] = d.ghostU[3], UR.p = d.ghostU[4];
-        WR.d = d.ghostW[0], WR.v[0] = d.ghostW[1], WR.v[1] = d.ghostW[2], WR.v[2] = d.ghostW[3], WR.p = d.ghostW[4];
-      }
-      const double s = hllc_flux<double>(d.gamma, d.Gg, UL, UR, WL, WR, nS.x, nS.y, nS.z, nS.w, F);
-      speed = smax(speed, s);
-      d.F4[i] = make_double4(F.d, F.v[0], F.v[1], F.v[2]);
-      d.FE[i] = F.p;
-    } else {
-      // FP32 storage (cell records, geometry, flux), FP64 arithmetic with the
-      // fast divide/sqrt policy.
-      const float3 Sn = d.face_Sn32[i];
-      State<double> F;
-      const MixedIn WL = mixed_in(d.Wva[lr.x], d.Wsd[lr.x]);
-      MixedIn WR;
-      if (lr.y >= 0) {
-        WR = mixed_in(d.Wva[lr.y], d.Wsd[lr.y]);
-      } else {
-        State<double> g;
-        g.d = d.ghostW[0], g.v[0] = d.ghostW[1], g.v[1] = d.ghostW[2], g.v[2] = d.ghostW[3], g.p = d.ghostW[4];
-        float4 va, sd;
-        mixed_record(d.gamma, d.Gg, g, va, sd);
-        WR = mixed_in(va, sd);
-      }
-      const double s = mixed_face_flux(WL, WR, Sn.x, Sn.y, Sn.z, F);
-      speed = smax(speed, s);
-      const rhllc_ext::Packed pk = rhllc_ext::pack(F.d, F.v[0], F.v[1], F.v[2], F.p);
-      d.F4f[i] = pk.a;
-      d.FXf[i] = pk.b;
-    }
+  const int base = (int)(xcd_block(blockIdx.x, gridDim.x) * (kBlock * kFacesPerThread) + threadIdx.x);
+  double speed = 0.0;
+#pragma unroll
+  for (int k = 0; k < kFacesPerThread; ++k) {
+    const int i = base + k * kBlock;
+    if (i < d.n_faces) speed = smax(speed, face_flux<MIXED>(d, i));
   }
   fold_speed(d, speed);
 }
@@ -267,14 +279,16 @@ __device__ __forceinline__ void finish_cell(const Dev &d, int c, bool live, Stat
       store_W32(d, c, W);
     }
   }
-  count_if(&d.sc->c2p_fail[1], live && code == 1);
-  count_if(&d.sc->c2p_fail[2], live && code == 2);
-  count_if(&d.sc->c2p_fail[3], live && code == 3);
-  count_if(&d.sc->c2p_fail[4], live && code == 4);
-  count_if(&d.sc->c2p_fail[5], live && code == 5);
-  count_if(&d.sc->fix_calls, fixed);
-  count_if(&d.sc->fix_fail, fix_failed);
-  count_if(&d.sc->fallback, fell_back);
+  if (__ballot(live && (code != 0 || fixed || fell_back)) != 0) {  // rare: skip eight ballots otherwise
+    count_if(&d.sc->c2p_fail[1], live && code == 1);
+    count_if(&d.sc->c2p_fail[2], live && code == 2);
+    count_if(&d.sc->c2p_fail[3], live && code == 3);
+    count_if(&d.sc->c2p_fail[4], live && code == 4);
+    count_if(&d.sc->c2p_fail[5], live && code == 5);
+    count_if(&d.sc->fix_calls, fixed);
+    count_if(&d.sc->fix_fail, fix_failed);
+    count_if(&d.sc->fallback, fell_back);
+  }
   if (STATS) {
     __shared__ unsigned int hist[kHistBins];
     __shared__ unsigned long long isum[kBlock / 64];
@@ -435,6 +449,9 @@ __global__ void k_clear(Dev d) {
 }
 
 static inline unsigned int blocks_for(int64_t n) { return (unsigned int)((n + kBlock - 1) / kBlock); }
+static inline unsigned int face_blocks(int64_t n) {
+  return (unsigned int)((n + kBlock * kFacesPerThread - 1) / (kBlock * kFacesPerThread));
+}
 
 hipError_t launch_p2c_init(const Dev &d, hipStream_t s) {
   if (d.n_cells == 0) return hipSuccess;
@@ -445,9 +462,9 @@ hipError_t launch_p2c_init(const Dev &d, hipStream_t s) {
 hipError_t launch_face(const Dev &d, bool mixed, hipStream_t s) {
   if (d.n_faces == 0) return hipSuccess;
   if (mixed)
-    hipLaunchKernelGGL(k_face<true>, dim3(blocks_for(d.n_faces)), dim3(kBlock), 0, s, d);
+    hipLaunchKernelGGL(k_face<true>, dim3(face_blocks(d.n_faces)), dim3(kBlock), 0, s, d);
   else
-    hipLaunchKernelGGL(k_face<false>, dim3(blocks_for(d.n_faces)), dim3(kBlock), 0, s, d);
+    hipLaunchKernelGGL(k_face<false>, dim3(face_blocks(d.n_faces)), dim3(kBlock), 0, s, d);
   return hipGetLastError();
 }
 

@@ -227,13 +227,21 @@ __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<doub
   const double D = U.d;
   const double E = U.p;
   const double m2 = dot3(U.v, U.v);
-  const double Q = E - A::sqrt_(m2 + D * D);
   iters = 0;
-  if (Q < 0.0) return 1;
-  const double m = A::sqrt_(m2);
+  double m, pmin;
+  if (!A::kFast) {
+    const double Q = E - A::sqrt_(m2 + D * D);
+    if (Q < 0.0) return 1;
+    m = A::sqrt_(m2);
+    pmin = A::sqrt_(A::div(m2, 1.0 - eps2)) - E;
+  } else {
+    // E < sqrt(m^2 + D^2) without the root; sqrt(m^2/(1 - eps2)) = m / sqrt(1 - eps2).
+    if (!(E >= 0.0 && E * E >= m2 + D * D)) return 1;
+    m = A::sqrt_(m2);
+    pmin = m * 1.0000000000005 - E;
+  }
   double p = smax(m - E, 0.0);
   const double D_1 = A::rcp(D);
-  const double pmin = A::sqrt_(A::div(m2, 1.0 - eps2)) - E;
   if (guess >= 0.0) p = guess;
   p = smax(p, pmin);
   double alpha, alpha2, lor2, lor, tau, theta, h, dh_dp, dh_dtau, yp = 0.0, dp, dyp;
@@ -279,15 +287,24 @@ __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<doub
   if (p < 0.0) return 2;
   if (isnan(p)) return 4;
   if (iter >= kMaxIter || fabs(A::div(yp, E + p)) > 1.e-4 || p < (m - E)) return 3;
-  alpha = E + p;
-  alpha2 = alpha * alpha;
-  lor2 = A::div(alpha2, alpha2 - m2);
-  lor = A::sqrt_(lor2);
-  tau = lor * D_1;
-  W.d = A::rcp(tau);
-  W.p = p;
-  const double den = U.p + p;
-  W.v[0] = A::div(U.v[0], den), W.v[1] = A::div(U.v[1], den), W.v[2] = A::div(U.v[2], den);
+  if (!A::kFast) {
+    alpha = E + p;
+    alpha2 = alpha * alpha;
+    lor2 = A::div(alpha2, alpha2 - m2);
+    lor = A::sqrt_(lor2);
+    tau = lor * D_1;
+    W.d = A::rcp(tau);
+    W.p = p;
+    const double den = U.p + p;
+    W.v[0] = A::div(U.v[0], den), W.v[1] = A::div(U.v[1], den), W.v[2] = A::div(U.v[2], den);
+  } else {
+    // rho = D / Gamma = D sqrt(alpha^2 - m^2) / alpha, v = m / alpha.
+    alpha = E + p;
+    const double ia = A::rcp(alpha);
+    W.d = D * A::sqrt_(smax(alpha * alpha - m2, 0.0)) * ia;
+    W.p = p;
+    W.v[0] = U.v[0] * ia, W.v[1] = U.v[1] * ia, W.v[2] = U.v[2] * ia;
+  }
   return 0;
 }
 
@@ -543,7 +560,13 @@ __device__ __forceinline__ void edd_matrix(double U, const double *F, double D[3
     return;
   }
   fm = A::sqrt_(fm);
-  const double n[3] = {A::div(F[0], fm), A::div(F[1], fm), A::div(F[2], fm)};
+  double n[3];
+  if (!A::kFast) {
+    n[0] = A::div(F[0], fm), n[1] = A::div(F[1], fm), n[2] = A::div(F[2], fm);
+  } else {
+    const double r = A::rcp(fm);
+    n[0] = F[0] * r, n[1] = F[1] * r, n[2] = F[2] * r;
+  }
   if (fm > U) {
 #pragma unroll
     for (int a = 0; a < 3; ++a)
