@@ -7,8 +7,8 @@
 
 namespace rhllc_dev {
 
-constexpr int kBlock = 256;
-constexpr int kFacesPerThread = 1;  // k_face (2 measured slower: fewer waves to hide latency)        // threads per workgroup of the face and cell kernels
+constexpr int kBlock = 256;         // threads per workgroup of the face and cell kernels
+constexpr int kFacesPerThread = 1;  // faces per k_face thread (2 measured slower: fewer waves to hide latency)
 constexpr int kSpeedSlots = 1024;  // per-step max-speed partials (one atomic slot per block group)
 constexpr int kSlotStride = 16;    // doubles between slots: one 128-byte line each
 constexpr int kHistBins = 41;
