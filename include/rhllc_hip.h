@@ -42,8 +42,9 @@ extern "C" {
 
 /* Arithmetic of the step. */
 #define RHLLC_PREC_FP64 0  /* all FP64, bitwise equal to the reference compiled strict */
-#define RHLLC_PREC_MIXED 1 /* FP32 storage of W, face geometry and face flux (the flux with a
-                              6-bit extension per component), FP64 U and FP64 arithmetic */
+#define RHLLC_PREC_MIXED 1 /* FP32 storage of primitives (per-cell records), face geometry and
+                              face flux (with small extensions where FP32 is too coarse),
+                              FP64 U and FP64 arithmetic */
 
 /* Initial pressure of the c2p Newton iteration. */
 #define RHLLC_GUESS_REFERENCE 0 /* max(|m|-E, 0, pmin) as rhllc_flux_stab.cpp:139-151 */
