@@ -6,10 +6,11 @@
 //
 // Used for
 //  * the face flux S*(D, m, E): float4 (D, mx, my, mz) + uint2 (bits of E,
-//    five 6-bit counts) = 24 bytes, 2^-29 relative.  FP32 alone is not
-//    enough: the momentum flux p n S of a cell's faces cancels around the
-//    cell while the roundings do not, and at Lorentz factors ~10 c2p
-//    amplifies an error in E by W^2 E/p (~1e3-1e4) into the pressure;
+//    four 8-bit counts for mx, my, mz, E) = 24 bytes, 2^-31 relative for m
+//    and E.  FP32 alone is not enough for those: the momentum flux p n S of a
+//    cell's faces cancels around the cell while the roundings do not, and at
+//    Lorentz factors ~10 c2p amplifies an error in E by W^2 E/p (~1e3-1e4)
+//    into the pressure.  D in FP32 leaves the errors unchanged;
 //  * the cell record's s = rho h W^2 and D (8-bit counts): they must match
 //    the record's FP32 v and p far better than FP32 rounding does.
 // Host and device use the same functions.
@@ -60,25 +61,25 @@ struct Packed {
   uint2 b;
 };
 
-// Face flux record (6-bit counts for all five components).
+// Face flux record.
 __host__ __device__ __forceinline__ Packed pack(double d, double mx, double my, double mz, double E) {
   Packed o;
-  uint32_t e0, e1, e2, e3, e4;
-  o.a.x = split<6>(d, e0);
-  o.a.y = split<6>(mx, e1);
-  o.a.z = split<6>(my, e2);
-  o.a.w = split<6>(mz, e3);
-  o.b.x = f32_bits(split<6>(E, e4));
-  o.b.y = e0 | (e1 << 6) | (e2 << 12) | (e3 << 18) | (e4 << 24);
+  uint32_t e1, e2, e3, e4;
+  o.a.x = (float)d;
+  o.a.y = split<8>(mx, e1);
+  o.a.z = split<8>(my, e2);
+  o.a.w = split<8>(mz, e3);
+  o.b.x = f32_bits(split<8>(E, e4));
+  o.b.y = e1 | (e2 << 8) | (e3 << 16) | (e4 << 24);
   return o;
 }
 
 __host__ __device__ __forceinline__ void unpack(const float4 &a, const uint2 &b, double out[5]) {
-  out[0] = join<6>(a.x, b.y);
-  out[1] = join<6>(a.y, b.y >> 6);
-  out[2] = join<6>(a.z, b.y >> 12);
-  out[3] = join<6>(a.w, b.y >> 18);
-  out[4] = join<6>(bits_f32(b.x), b.y >> 24);
+  out[0] = a.x;
+  out[1] = join<8>(a.y, b.y);
+  out[2] = join<8>(a.z, b.y >> 8);
+  out[3] = join<8>(a.w, b.y >> 16);
+  out[4] = join<8>(bits_f32(b.x), b.y >> 24);
 }
 
 }  // namespace rhllc_ext
