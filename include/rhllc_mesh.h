@@ -108,6 +108,58 @@ int rhllc_morton_order(int64_t n_cells, const void *geo_cells, int64_t n_faces,
                        const void *geo_faces, int32_t *cell_new_to_old,
                        int32_t *face_new_to_old);
 
+/*
+ * Legacy-format VTK files of tetrahedral unstructured grids (no VTK library).
+ * Replaces files_sys::vtk::Read (lib/files_sys/include/reader_vtk.h:43-61),
+ * which reads the file with VTK 9.3's vtkGenericDataObjectReader.
+ * Versions 2.0-5.1, ASCII or BINARY; every cell must be a VTK_TETRA (10).
+ * Points declared "float" are rounded to float, as VTK stores them.
+ *
+ *   rhllc_vtk_open   parses the file; *n_points, *n_cells receive the sizes
+ *   rhllc_vtk_copy   points: 3*n_points doubles; tets: 4*n_cells point ids
+ *                    (either may be NULL)
+ *   rhllc_vtk_close  frees the parsed mesh
+ */
+typedef struct rhllc_vtk_mesh rhllc_vtk_mesh;
+int rhllc_vtk_open(const char *path, rhllc_vtk_mesh **out, int64_t *n_points, int64_t *n_cells);
+int rhllc_vtk_copy(const rhllc_vtk_mesh *m, double *points, int32_t *tets);
+void rhllc_vtk_close(rhllc_vtk_mesh *m);
+
+/* Boundary-code rules of SetTypeOfBound (lib/geometry/src/make_internal_format.cpp:12-108). */
+#define RHLLC_GEO_BOUND_CONE 0 /* GEOMETRY_TYPE Cone, the reference's build (include/solvers_config.h:29):
+                                  normal within 1e-7 of -x: -3; of +x: -5; otherwise -2 */
+#define RHLLC_GEO_BOUND_FREE 1 /* GEOMETRY_TYPE Cube: every boundary face -1 */
+
+/*
+ * The per-cell arrays that BuildDataFromVTK writes
+ * (lib/geometry/src/make_internal_format.cpp:272-336), from a tet mesh:
+ *   neighbors[4N]  neighbour cell*4 + its local face, or a boundary code
+ *                  (GetNeighborFace3D + SetTypeOfBound; file neighbors.bin)
+ *   normals[12N]   outward unit normal of local face j at [12i + 3j]
+ *                  (NormalAndSquareFace3D; file normals.bin)
+ *   areas[4N]      face areas (areas.bin)
+ *   volumes[N]     -(a . (b x c)) / 6, signed as in GetVolume3D (volume.bin)
+ *   centers[3N]    insphere centres, CenterOfTetra3D (centers.bin)
+ * Local face j of cell i is VTK's vtkTetra face j: points (0,1,3), (1,2,3),
+ * (2,0,3), (0,2,1) of the cell.  Results are bitwise the reference's.
+ * Fails on repeated or out-of-range point ids and on a face shared by more
+ * than two cells.
+ */
+int rhllc_tet_geometry(int64_t n_points, const double *points, int64_t n_cells, const int32_t *tets,
+                       int32_t bound_rule, int32_t *neighbors, double *normals, double *areas,
+                       double *volumes, double *centers);
+
+/*
+ * ConvertBinToGeo (lib/geometry/src/convert_bin_to_geo.cpp:3-48): geometry
+ * records from the arrays above, numbered by the rule at the top of this
+ * header.  *n_faces receives the face count; with geo_cells and geo_faces both
+ * NULL only the count is computed.  geo_cells: n_cells * 56 bytes, geo_faces:
+ * *n_faces * 40 bytes.
+ */
+int rhllc_bin_to_geo(int64_t n_cells, const int32_t *neighbors, const double *normals, const double *areas,
+                     const double *volumes, const double *centers, int64_t *n_faces, void *geo_cells,
+                     void *geo_faces);
+
 /* Read/write "int32 n, then n records" files (reader_bin.h:113-135,232-252). */
 int64_t rhllc_file_count(const char *path);
 int rhllc_file_read_records(const char *path, size_t rec_bytes, int64_t n, void *out);

@@ -114,6 +114,9 @@ def ref_lib():
         L.ref_rotation_n.argtypes = [i64, vp, vp]
         L.ref_flux_n.argtypes = [i64, vp, vp, vp, vp, vp, vp, vp]
         L.ref_run.argtypes = [i64, vp, vp, i64, vp, dbl, dbl, i32, vp, vp, vp, vp]
+        L.ref_convert_bin_to_geo.argtypes = [i64, vp, vp, vp, vp, vp, i64, vp, vp]
+        L.ref_convert_bin_to_geo.restype = i64
+        L.ref_cone_states.argtypes = [vp, vp, vp]
         L.ref_set_log_dir(("/tmp/").encode())
         _ref = L
     return _ref
@@ -252,6 +255,25 @@ def ref_run(mesh, W0, cfl, h_min, steps):
         steps, Wo.ctypes.data, Uo.ctypes.data, taus.ctypes.data, speeds.ctypes.data,
     )
     return Wo, Uo, taus, speeds
+
+
+def ref_convert_bin_to_geo(nb, normals, areas, vols, cents):
+    """The reference's ConvertBinToGeo: (cells bytes [N*56], faces bytes [F*40])."""
+    L = ref_lib()
+    arrs = [np.ascontiguousarray(nb, np.int32)] + [np.ascontiguousarray(a, np.float64) for a in (normals, areas, vols, cents)]
+    n = arrs[3].shape[0]
+    cells = np.zeros(n * 56, np.uint8)
+    nf = L.ref_convert_bin_to_geo(n, *[a.ctypes.data for a in arrs], 0, cells.ctypes.data, None)
+    faces = np.zeros(nf * 40, np.uint8)
+    L.ref_convert_bin_to_geo(n, *[a.ctypes.data for a in arrs], nf, cells.ctypes.data, faces.ctypes.data)
+    return cells, faces
+
+
+def ref_cone_states():
+    """(W_init, W_source, U_source) of the reference build's GEOMETRY_TYPE Cone."""
+    W0, Ws, Us = np.zeros(5), np.zeros(5), np.zeros(5)
+    ref_lib().ref_cone_states(W0.ctypes.data, Ws.ctypes.data, Us.ctypes.data)
+    return W0, Ws, Us
 
 
 def ref_p2c(W):

@@ -4,7 +4,8 @@
 // the reference's hot-path sources where they lie under /root/reference
 // (solvers/rhllc/src/rhllc_flux_stab.cpp, rhllc_3d_stab.cpp, rhllc_utils.cpp,
 // lib/geometry/src/linear_alg.cpp, lib/grids/src/solvers_struct.cpp,
-// lib/json/src/json_struct.cpp, solvers/rhllc/src/rhllc_flux.cpp) into
+// lib/json/src/json_struct.cpp, solvers/rhllc/src/rhllc_flux.cpp,
+// lib/geometry/src/convert_bin_to_geo.cpp) into
 // oracle/_ref/libref_rhllc.so.  Tests use it to pin the C restatement in
 // oracle/rhllc_oracle.c; it is never part of the product.
 //
@@ -17,8 +18,11 @@
 #include <string>
 #include <vector>
 
+#include "convert_bin_to_geo.h"
+#include "gas_state.h"
 #include "global_value.h"
 #include "rhllc_calc.h"
+#include "rhllc_flux.h"
 #include "rhllc_flux_stab.h"
 #include "rhllc_utils.h"
 #include "linear_alg.h"
@@ -126,6 +130,54 @@ int ref_run(int64_t n_cells, const void *geo_cells, const double *W0, int64_t n_
     uo[0] = u.d, uo[1] = u.v[0], uo[2] = u.v[1], uo[3] = u.v[2], uo[4] = u.p;
   }
   return 0;
+}
+
+// ConvertBinToGeo (lib/geometry/src/convert_bin_to_geo.cpp:3-48) on the five
+// preprocessing arrays; writes up to max_faces face records and returns the
+// face count.
+int64_t ref_convert_bin_to_geo(int64_t n_cells, const int32_t *nb, const double *normals, const double *areas,
+                               const double *vol, const double *cen, int64_t max_faces, void *geo_cells,
+                               void *geo_faces) {
+  std::vector<IntId> neigh(nb, nb + 4 * n_cells);
+  std::vector<Normals> nrm(n_cells);
+  std::vector<Type> ar(areas, areas + 4 * n_cells), v(vol, vol + n_cells);
+  std::vector<Vector3> c(n_cells);
+  for (int64_t i = 0; i < n_cells; ++i) {
+    for (int j = 0; j < 4; ++j)
+      nrm[i].n[j] = Vector3(normals[12 * i + 3 * j], normals[12 * i + 3 * j + 1], normals[12 * i + 3 * j + 2]);
+    c[i] = Vector3(cen[3 * i], cen[3 * i + 1], cen[3 * i + 2]);
+  }
+  std::vector<face_t> faces;
+  std::vector<elem_t> cells;
+  ConvertBinToGeo(neigh, nrm, ar, v, c, faces, cells);
+  uint8_t *gc = static_cast<uint8_t *>(geo_cells);
+  uint8_t *gf = static_cast<uint8_t *>(geo_faces);
+  for (int64_t i = 0; i < n_cells; ++i) std::memcpy(gc + 56 * i, (const void *)&cells[i].geo, 56);
+  for (int64_t i = 0; i < (int64_t)faces.size() && i < max_faces; ++i)
+    std::memcpy(gf + 40 * i, (const void *)&faces[i].geo, 40);
+  return (int64_t)faces.size();
+}
+
+// The GEOMETRY_TYPE Cone states of this build: the default initial W of
+// rhllc_init.cpp:130-132 and the e_bound_out_source ghost of
+// rhllc_utils.cpp:80-84 (W, and U from rhllc::GetConvValue), from the
+// reference's own constants and GetPressure (lib/physics/include/gas_state.h:38-43).
+void ref_cone_states(double *W_init, double *W_src, double *U_src) {
+  flux_t w0;
+  w0.d = kM_hydrogen * 1e14 / kDensity;
+  w0.p = GetPressure(w0.d, kEv);
+  w0.v = Vector3(0, 0, 0);
+  flux_t ws, us;
+  ws.d = kM_hydrogen * 1e14 / kDensity;
+  ws.p = GetPressure(ws.d, 10 * kEv);
+  ws.v = Vector3(1e3 / kVelocity, 0, 0);
+  rhllc::GetConvValue(ws, us);
+  const flux_t *src[3] = {&w0, &ws, &us};
+  double *dst[3] = {W_init, W_src, U_src};
+  for (int k = 0; k < 3; ++k) {
+    dst[k][0] = src[k]->d, dst[k][1] = src[k]->v[0], dst[k][2] = src[k]->v[1], dst[k][3] = src[k]->v[2];
+    dst[k][4] = src[k]->p;
+  }
 }
 
 }  // extern "C"

@@ -42,6 +42,12 @@ def load_mesh_lib() -> C.CDLL:
         lib.rhllc_file_count.restype = i64
         lib.rhllc_file_read_records.argtypes = [C.c_char_p, C.c_size_t, i64, vp]
         lib.rhllc_file_write_records.argtypes = [C.c_char_p, C.c_size_t, i64, vp]
+        lib.rhllc_vtk_open.argtypes = [C.c_char_p, vp, vp, vp]
+        lib.rhllc_vtk_copy.argtypes = [vp, vp, vp]
+        lib.rhllc_vtk_close.argtypes = [vp]
+        lib.rhllc_vtk_close.restype = None
+        lib.rhllc_tet_geometry.argtypes = [i64, vp, i64, vp, i32, vp, vp, vp, vp, vp]
+        lib.rhllc_bin_to_geo.argtypes = [i64, vp, vp, vp, vp, vp, vp, vp, vp]
         lib.rhllc_mesh_last_error.restype = C.c_char_p
         _mesh = lib
     return _mesh

@@ -15,7 +15,10 @@ import sys
 from ._lib import HIP_LIB, MESH_LIB, PKG_DIR
 
 CSRC = os.path.join(PKG_DIR, "csrc")
-MESH_SRC = [os.path.join(CSRC, "mesh", f) for f in ("kuhn_box.cpp", "mesh_tools.cpp", "morton.cpp")]
+MESH_SRC = [
+    os.path.join(CSRC, "mesh", f)
+    for f in ("kuhn_box.cpp", "mesh_tools.cpp", "morton.cpp", "vtk_legacy.cpp", "tet_geometry.cpp")
+]
 HIP_SRC = [os.path.join(CSRC, "hip", f) for f in ("rhllc_kernels.hip", "rhllc_capi.hip", "mlp_kernels.hip", "mlp_capi.hip")]
 HIP_FLAGS = [
     "--offload-arch=gfx950",
@@ -34,7 +37,9 @@ def _run(cmd):
 
 
 def build_mesh() -> str:
-    _run(["g++", "-std=c++17", "-O2", "-fopenmp", "-fPIC", "-shared", "-Wall", "-o", MESH_LIB] + MESH_SRC)
+    # -ffp-contract=off: the mesh geometry is compared bitwise with the reference's.
+    _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", "-Wall", "-o", MESH_LIB]
+         + MESH_SRC)
     return MESH_LIB
 
 
@@ -57,18 +62,23 @@ def build_hip() -> str:
 
 SHIM_DIR = os.path.join(PKG_DIR, "shim")
 SOLVER_EXE = os.path.join(PKG_DIR, "bin", "rhllc_solver")
+MAKE_GEO_EXE = os.path.join(PKG_DIR, "bin", "rhllc_make_geo")
 DROPIN_LIB = os.path.join(PKG_DIR, "librhllc_dropin.so")
 
 
 def build_shim() -> str:
-    """The drop-in C++ interface (librhllc_dropin.so) and the rhllc_solver executable."""
+    """The drop-in C++ interface (librhllc_dropin.so) and the rhllc_solver and
+    rhllc_make_geo executables."""
     os.makedirs(os.path.dirname(SOLVER_EXE), exist_ok=True)
     inc = ["-I", os.path.join(os.path.dirname(PKG_DIR), "include")]
-    link = ["-L", PKG_DIR, "-lrhllc_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."]
-    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wno-invalid-offsetof", "-o", DROPIN_LIB,
-          os.path.join(SHIM_DIR, "rhllc_dropin.cpp")] + inc + link)
+    link = ["-L", PKG_DIR, "-lrhllc_hip", "-lradrhd_mesh", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."]
+    _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-Wall", "-Wno-invalid-offsetof",
+          "-o", DROPIN_LIB, os.path.join(SHIM_DIR, "rhllc_dropin.cpp")] + inc + link)
+    exe_link = ["-L", PKG_DIR, "-lrhllc_dropin", "-lrhllc_hip", "-lradrhd_mesh", "-Wl,-rpath,$ORIGIN/.."]
     _run(["g++", "-std=c++17", "-O2", "-Wall", "-o", SOLVER_EXE, os.path.join(SHIM_DIR, "rhllc_solver_main.cpp")]
-         + inc + ["-L", PKG_DIR, "-lrhllc_dropin", "-lrhllc_hip", "-Wl,-rpath,$ORIGIN/.."])
+         + inc + exe_link)
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-o", MAKE_GEO_EXE, os.path.join(SHIM_DIR, "rhllc_make_geo_main.cpp")]
+         + inc + exe_link)
     return SOLVER_EXE
 
 

@@ -159,6 +159,106 @@ def morton_order(mesh: Mesh):
     return cp, fp
 
 
+BOUND_CONE, BOUND_FREE = 0, 1
+
+
+def read_vtk(path: str):
+    """(points[P,3] float64, tets[N,4] int32) of a legacy-format VTK tet grid."""
+    lib = load_mesh_lib()
+    h = C.c_void_p()
+    n_p, n_c = C.c_int64(), C.c_int64()
+    if lib.rhllc_vtk_open(os.fsencode(path), C.byref(h), C.byref(n_p), C.byref(n_c)) != 0:
+        raise ValueError(lib.rhllc_mesh_last_error().decode())
+    try:
+        points = np.zeros((n_p.value, 3), np.float64)
+        tets = np.zeros((n_c.value, 4), np.int32)
+        if lib.rhllc_vtk_copy(h, points.ctypes.data, tets.ctypes.data) != 0:
+            raise ValueError(lib.rhllc_mesh_last_error().decode())
+    finally:
+        lib.rhllc_vtk_close(h)
+    return points, tets
+
+
+@dataclass
+class TetGeometry:
+    """The per-cell arrays of the reference's preprocessing (BuildDataFromVTK,
+    lib/geometry/src/make_internal_format.cpp:272-336)."""
+
+    neighbors: np.ndarray  # int32[4N]: neighbour cell*4 + local face, or a boundary code
+    normals: np.ndarray  # float64[N,4,3] outward unit normals
+    areas: np.ndarray  # float64[4N]
+    volumes: np.ndarray  # float64[N]
+    centers: np.ndarray  # float64[N,3]
+
+    def write(self, base: str) -> None:
+        """neighbors.bin, normals.bin, areas.bin, volume.bin, centers.bin
+        (lib/global_value.h:74-78), each "int32 n, then n records"."""
+        write_records(base + "neighbors.bin", self.neighbors)
+        write_records(base + "normals.bin", self.normals)
+        write_records(base + "areas.bin", self.areas)
+        write_records(base + "volume.bin", self.volumes)
+        write_records(base + "centers.bin", self.centers)
+
+    @staticmethod
+    def read(base: str) -> "TetGeometry":
+        return TetGeometry(
+            read_records(base + "neighbors.bin", np.int32),
+            read_records(base + "normals.bin", np.dtype((np.float64, (4, 3)))),
+            read_records(base + "areas.bin", np.float64),
+            read_records(base + "volume.bin", np.float64),
+            read_records(base + "centers.bin", np.dtype((np.float64, 3))),
+        )
+
+
+def tet_geometry(points: np.ndarray, tets: np.ndarray, bound: int = BOUND_CONE) -> TetGeometry:
+    lib = load_mesh_lib()
+    points = np.ascontiguousarray(points, np.float64)
+    tets = np.ascontiguousarray(tets, np.int32)
+    n = tets.shape[0]
+    g = TetGeometry(
+        np.zeros(4 * n, np.int32),
+        np.zeros((n, 4, 3), np.float64),
+        np.zeros(4 * n, np.float64),
+        np.zeros(n, np.float64),
+        np.zeros((n, 3), np.float64),
+    )
+    rc = lib.rhllc_tet_geometry(
+        points.shape[0], points.ctypes.data, n, tets.ctypes.data, bound, g.neighbors.ctypes.data,
+        g.normals.ctypes.data, g.areas.ctypes.data, g.volumes.ctypes.data, g.centers.ctypes.data,
+    )
+    if rc != 0:
+        raise ValueError(lib.rhllc_mesh_last_error().decode())
+    return g
+
+
+def bin_to_geo(g: TetGeometry) -> Mesh:
+    """ConvertBinToGeo (lib/geometry/src/convert_bin_to_geo.cpp:3-48)."""
+    lib = load_mesh_lib()
+    n = g.volumes.shape[0]
+    arrs = [np.ascontiguousarray(a) for a in (g.neighbors, g.normals, g.areas, g.volumes, g.centers)]
+    ptrs = [a.ctypes.data for a in arrs]
+    n_f = C.c_int64()
+    if lib.rhllc_bin_to_geo(n, *ptrs, C.byref(n_f), None, None) != 0:
+        raise ValueError(lib.rhllc_mesh_last_error().decode())
+    cells = np.zeros(n, GEO_CELL_DTYPE)
+    faces = np.zeros(n_f.value, GEO_FACE_DTYPE)
+    if lib.rhllc_bin_to_geo(n, *ptrs, C.byref(n_f), cells.ctypes.data, faces.ctypes.data) != 0:
+        raise ValueError(lib.rhllc_mesh_last_error().decode())
+    return Mesh(cells, faces)
+
+
+def build_geo_from_vtk(vtk_path: str, base: str, bound: int = BOUND_CONE):
+    """make_all_geo_data's RHLLC part (utils/src/make_all_geo_data.cpp:12-41):
+    BuildDataFromVTK's five arrays, then BinToGeo's geo_faces.bin and
+    geo_cells.bin, all written under the directory prefix `base`."""
+    points, tets = read_vtk(vtk_path)
+    g = tet_geometry(points, tets, bound)
+    g.write(base)
+    m = bin_to_geo(g)
+    m.write(base + "geo_faces.bin", base + "geo_cells.bin")
+    return g, m
+
+
 def write_records(path: str, recs: np.ndarray) -> None:
     with open(path, "wb") as f:
         f.write(np.int32(recs.shape[0]).tobytes())

@@ -10,6 +10,7 @@
 #include <sstream>
 
 #include "../../include/rhllc_hip.h"
+#include "../../include/rhllc_mesh.h"
 #include "json_min.h"
 
 hllc_value_t _hllc_cfg;
@@ -23,6 +24,37 @@ namespace {
   std::fprintf(stderr, "DIE: %s %s\n", what, detail.c_str());
   std::fflush(stderr);
   std::abort();
+}
+
+// lib/global_consts.h:34-82 and lib/physics/include/gas_state.h:38-43
+const double kM_hydrogen = 1.6735575 * 1e-24, k_boltzmann = 1.3807 * 1e-16, kDist = 1e13, kC_Light = 3 * 1e10,
+             kGravity = 6.6726 * 1e-8, kEv = 11604525.0061657;
+const double kMass = (kC_Light * kC_Light * kDist) / kGravity;
+const double kTime = kDist / kC_Light;
+const double kDensity = kMass / (kDist * kDist * kDist);
+const double kPressure = kMass / (kDist * kTime * kTime);
+const double kVelocity = kC_Light;
+double GetPressure(double rho, double tem) {
+  const double g_idealGasConst = (kM_hydrogen / k_boltzmann);
+  const double g_idealGasConst_inv = kDensity / g_idealGasConst;
+  return (g_idealGasConst_inv / kPressure) * tem * rho;
+}
+
+// Ghost state of e_bound_out_source faces under GEOMETRY_TYPE Cone
+// (rhllc_utils.cpp:80-84): W = (rho0, 1e3 / kVelocity, 0, 0, GetPressure(rho0, 10 kEv)),
+// U from W by rhllc::GetConvValue (rhllc_flux.cpp:7-16), which is not the
+// clamped GetConvValueStab of the cells.
+void cone_out_source(double gamma, double W[5], double U[5]) {
+  W[0] = kM_hydrogen * 1e14 / kDensity;
+  W[1] = 1e3 / kVelocity, W[2] = 0.0, W[3] = 0.0;
+  W[4] = GetPressure(W[0], 10 * kEv);
+  const double Gg = gamma / (gamma - 1.0);
+  const double Gamma = 1. / std::sqrt(1 - (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]));
+  const double h = 1 + Gg * W[4] / W[0];
+  const double dhGG = W[0] * h * Gamma * Gamma;
+  U[0] = Gamma * W[0];
+  U[1] = dhGG * W[1], U[2] = dhGG * W[2], U[3] = dhGG * W[3];
+  U[4] = dhGG - W[4];
 }
 
 struct DeviceGrid {
@@ -65,6 +97,11 @@ void ensure_device(grid_t &grid) {
   p.c2p_guess = guess_code(_hllc_cfg.c2p_start);
   p.reorder = 1;
   p.device = 0;
+  for (size_t i = 0; i < F && !p.has_out_source; ++i)
+    if (grid.faces[i].geo.id_r == RHLLC_BOUND_OUT_SOURCE) {
+      p.has_out_source = 1;
+      cone_out_source(_hllc_cfg.gamma, p.out_source_W, p.out_source_U);
+    }
   if (rhllc_create(&p, (int64_t)N, gc.data(), (int64_t)F, gf.data(), &g_dev.h) != RHLLC_OK)
     die("rhllc_create", rhllc_last_error());
   if (p.c2p_guess == RHLLC_GUESS_MLP) {
@@ -113,19 +150,6 @@ int read_records(const std::string &name, std::vector<T> &data, size_t rec_bytes
   }
   std::fclose(f);
   return e_completion_success;
-}
-
-// lib/global_consts.h:34-82 and lib/physics/include/gas_state.h:38-43
-const double kM_hydrogen = 1.6735575 * 1e-24, k_boltzmann = 1.3807 * 1e-16, kDist = 1e13, kC_Light = 3 * 1e10,
-             kGravity = 6.6726 * 1e-8, kEv = 11604525.0061657;
-const double kMass = (kC_Light * kC_Light * kDist) / kGravity;
-const double kTime = kDist / kC_Light;
-const double kDensity = kMass / (kDist * kDist * kDist);
-const double kPressure = kMass / (kDist * kTime * kTime);
-double GetPressure(double rho, double tem) {
-  const double g_idealGasConst = (kM_hydrogen / k_boltzmann);
-  const double g_idealGasConst_inv = kDensity / g_idealGasConst;
-  return (g_idealGasConst_inv / kPressure) * tem * rho;
 }
 
 }  // namespace
@@ -358,3 +382,79 @@ int WriteSolution(const std::string &main_dir, const grid_t &grid) {
 
 }  // namespace bin
 }  // namespace files_sys
+
+// lib/geometry/src/make_internal_format.cpp:272-336, the part the RHLLC module
+// reads: neighbors.bin, normals.bin, areas.bin, volume.bin and centers.bin
+// under glb_files.base_address, from the tetrahedra of glb_files.name_file_vtk,
+// with the boundary codes of GEOMETRY_TYPE Cone.  The radiation-transfer files
+// the reference also writes there are not produced.
+int BuildDataFromVTK(const global_files_t &glb_files) {
+  rhllc_vtk_mesh *vm = nullptr;
+  int64_t np = 0, nc = 0;
+  if (rhllc_vtk_open(glb_files.name_file_vtk.c_str(), &vm, &np, &nc)) {
+    std::fprintf(stderr, "Error reading file vtk: %s\n", rhllc_mesh_last_error());
+    return e_completion_fail;
+  }
+  std::vector<double> points(3 * np), normals(12 * nc), areas(4 * nc), volumes(nc), centers(3 * nc);
+  std::vector<int32_t> tets(4 * nc), neighbors(4 * nc);
+  rhllc_vtk_copy(vm, points.data(), tets.data());
+  rhllc_vtk_close(vm);
+  std::printf("Grid has Cell: %lld\n", (long long)nc);
+  if (rhllc_tet_geometry(np, points.data(), nc, tets.data(), RHLLC_GEO_BOUND_CONE, neighbors.data(), normals.data(),
+                         areas.data(), volumes.data(), centers.data())) {
+    std::fprintf(stderr, "Neighbor wasn't found: %s\n", rhllc_mesh_last_error());
+    return e_completion_fail;
+  }
+  const std::string &b = glb_files.base_address;
+  const bool ok = !rhllc_file_write_records((b + "neighbors.bin").c_str(), 4, 4 * nc, neighbors.data()) &&
+                  !rhllc_file_write_records((b + "normals.bin").c_str(), 96, nc, normals.data()) &&
+                  !rhllc_file_write_records((b + "areas.bin").c_str(), 8, 4 * nc, areas.data()) &&
+                  !rhllc_file_write_records((b + "volume.bin").c_str(), 8, nc, volumes.data()) &&
+                  !rhllc_file_write_records((b + "centers.bin").c_str(), 24, nc, centers.data());
+  if (!ok) std::fprintf(stderr, "Error writing geometry files under %s\n", b.c_str());
+  return ok ? e_completion_success : e_completion_fail;
+}
+
+// utils/template_functions/bin_files_to_geo.h:11-38: geo_faces.bin and
+// geo_cells.bin from the five files above.
+int BinToGeo(const std::string &base_address) {
+  const std::string &b = base_address;
+  const int64_t nc = rhllc_file_count((b + "volume.bin").c_str());
+  if (nc < 0) {
+    std::fprintf(stderr, "Error reading file volume\n");
+    return e_completion_fail;
+  }
+  std::vector<double> normals(12 * nc), areas(4 * nc), volumes(nc), centers(3 * nc);
+  std::vector<int32_t> neighbors(4 * nc);
+  const char *what = nullptr;
+  if (rhllc_file_count((b + "neighbors.bin").c_str()) != 4 * nc ||
+      rhllc_file_read_records((b + "neighbors.bin").c_str(), 4, 4 * nc, neighbors.data()))
+    what = "neighbours";
+  else if (rhllc_file_count((b + "normals.bin").c_str()) != nc ||
+           rhllc_file_read_records((b + "normals.bin").c_str(), 96, nc, normals.data()))
+    what = "normals";
+  else if (rhllc_file_count((b + "areas.bin").c_str()) != 4 * nc ||
+           rhllc_file_read_records((b + "areas.bin").c_str(), 8, 4 * nc, areas.data()))
+    what = "areas_faces";
+  else if (rhllc_file_read_records((b + "volume.bin").c_str(), 8, nc, volumes.data()))
+    what = "volume";
+  else if (rhllc_file_count((b + "centers.bin").c_str()) != nc ||
+           rhllc_file_read_records((b + "centers.bin").c_str(), 24, nc, centers.data()))
+    what = "centers";
+  if (what) {
+    std::fprintf(stderr, "Error reading file %s\n", what);
+    return e_completion_fail;
+  }
+  int64_t nf = 0;
+  if (rhllc_bin_to_geo(nc, neighbors.data(), normals.data(), areas.data(), volumes.data(), centers.data(), &nf,
+                       nullptr, nullptr)) {
+    std::fprintf(stderr, "ConvertBinToGeo: %s\n", rhllc_mesh_last_error());
+    return e_completion_fail;
+  }
+  std::vector<uint8_t> gc(56 * nc), gf(40 * nf);
+  rhllc_bin_to_geo(nc, neighbors.data(), normals.data(), areas.data(), volumes.data(), centers.data(), &nf, gc.data(),
+                   gf.data());
+  const bool ok = !rhllc_file_write_records((b + "geo_cells.bin").c_str(), 56, nc, gc.data()) &&
+                  !rhllc_file_write_records((b + "geo_faces.bin").c_str(), 40, nf, gf.data());
+  return ok ? e_completion_success : e_completion_fail;
+}

@@ -12,6 +12,8 @@
 //   files_sys::bin::ReadGridGeo / ReadHllcInit / WriteSolution
 //                                       lib/files_sys/include/reader_bin.h:113-135,232-252,
 //                                       lib/files_sys/src/writer_bin.cpp:56-114
+//   BuildDataFromVTK(glb_files)         lib/geometry/include/make_internal_format.h
+//   BinToGeo(base_address)              utils/template_functions/bin_files_to_geo.h:11-38
 // Where the reference aborts (DIE_IF), these abort too, after a message on
 // stderr; the functions that return e_type_completion return it.
 //
@@ -63,3 +65,11 @@ int ReadHllcInit(const std::string &file_init_value, std::vector<elem_t> &cells)
 int WriteSolution(const std::string &main_dir, const grid_t &grid);
 }  // namespace bin
 }  // namespace files_sys
+
+// Mesh preprocessing without VTK (legacy-format VTK files of tetrahedra), by
+// libradrhd_mesh.so.  BuildDataFromVTK writes neighbors.bin, normals.bin,
+// areas.bin, volume.bin and centers.bin under glb_files.base_address, with the
+// GEOMETRY_TYPE Cone boundary codes; BinToGeo writes geo_faces.bin and
+// geo_cells.bin from them.  Both return e_type_completion.
+int BuildDataFromVTK(const global_files_t &glb_files);
+int BinToGeo(const std::string &base_address);

@@ -13,6 +13,10 @@ compiled reference returns.  The fixtures are data only (inputs + outputs):
   flux.npz       GetFluxStab             (UL, UR, WL, WR, n, S) -> (F, max(lL, lR)), all 4 branches
   traj_tiny.npz  Hllc3dStab loop         8x2x2-hex jittered/permuted mesh, 200 steps
   shock_tube_200.json  config 0 (200x10x10 hexes, 120k tets) after 200 steps: per-field sums and hashes
+  brick.vtk.xz, sphere.vtk.xz  the reference's test meshes (tests/data), compressed
+  brick_cone.json  the reference's RHLLC test on brick.vtk to T = 0.5: geometry and state hashes
+
+``python make_golden.py --vtk`` rewrites only the last three.
 """
 import hashlib
 import json
@@ -47,9 +51,76 @@ def unit(rng, n):
     return x / np.linalg.norm(x, axis=1)[:, None]
 
 
+REF_DATA = "/root/reference/tests/data"
+
+
+def sha(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def vtk_fixtures():
+    """The reference's own test meshes (tests/data/brick.vtk, sphere.vtk), xz-compressed,
+    and the reference's RHLLC test on brick.vtk (tests/src/off_rhllc_test.cpp:22-45):
+    GEOMETRY_TYPE Cone boundaries and initial state, CFL 0.7, h_min 1e-5, run to T = 0.5.
+
+    The geometry arrays come from the oracle's restatement of the VTK-based
+    preprocessing (oracle/mesh_geo.py; VTK itself is not available), the
+    geometry records from the reference's own ConvertBinToGeo, the run from
+    the reference's own Hllc3dStab driven like rhllc_main.cpp:34-56."""
+    import lzma
+
+    from oracle import mesh_geo as G
+
+    for name in ("brick", "sphere"):
+        with open(os.path.join(REF_DATA, name + ".vtk"), "rb") as f:
+            raw = f.read()
+        with open(os.path.join(HERE, name + ".vtk.xz"), "wb") as f:
+            f.write(lzma.compress(raw, preset=9 | lzma.PRESET_EXTREME))
+
+    P, T = G.read_legacy_vtk(os.path.join(REF_DATA, "brick.vtk"))
+    nb, nrm, ar, vol, cen = G.build_data_from_vtk(P, T, "Cone")
+    cells, faces = O.ref_convert_bin_to_geo(nb, nrm, ar, vol, cen)
+    # Writable views of the raw bytes: a structured copy would not keep the padding bytes.
+    m = M.Mesh(np.frombuffer(bytearray(cells.tobytes()), M.GEO_CELL_DTYPE),
+               np.frombuffer(bytearray(faces.tobytes()), M.GEO_FACE_DTYPE))
+    W_init, W_src, U_src = O.ref_cone_states()
+    W0 = np.tile(W_init, (m.n_cells, 1))
+    cfl, h_min, t_end = 0.7, 1e-5, 0.5
+    _, _, taus, _ = O.ref_run(m, W0, cfl, h_min, 2000)
+    # rhllc_main.cpp:34-56: while (t < T) { step(tau); t += tau; tau = GetTimeStep }
+    t, steps = 0.0, 0
+    while t < t_end:
+        t += taus[steps]
+        steps += 1
+    Wt, Ut, taus, speeds = O.ref_run(m, W0, cfl, h_min, steps)
+    t = 0.0
+    for tau in taus:  # the driver's running sum, in step order
+        t += float(tau)
+    rec = {
+        "source": "reference tests/src/off_rhllc_test.cpp:22-45 on tests/data/brick.vtk",
+        "n_points": int(P.shape[0]), "n_cells": m.n_cells, "n_faces": m.n_faces,
+        "neighbors_sha256": sha(nb), "normals_sha256": sha(nrm), "areas_sha256": sha(ar),
+        "volumes_sha256": sha(vol), "centers_sha256": sha(cen),
+        "geo_cells_sha256": sha(m.cells), "geo_faces_sha256": sha(m.faces),
+        "bound_counts": {str(c): int((nb == c).sum()) for c in (-2, -3, -5)},
+        "W_init": W_init.tolist(), "W_source": W_src.tolist(), "U_source": U_src.tolist(),
+        "cfl": cfl, "h_min": h_min, "T": t_end, "steps": steps, "t": t,
+        "taus_sha256": sha(taus), "last_speed": float(speeds[-1]),
+        "U_abs_sum": [float(np.abs(Ut[:, j]).sum()) for j in range(5)],
+        "W_abs_sum": [float(np.abs(Wt[:, j]).sum()) for j in range(5)],
+        "U_sha256": sha(Ut), "W_sha256": sha(Wt),
+    }
+    with open(os.path.join(HERE, "brick_cone.json"), "w") as f:
+        json.dump(rec, f, indent=1)
+
+
 def main():
     if not O.have_ref():
         sys.exit("oracle/_ref/libref_rhllc.so is not built (make -C oracle ref)")
+    if sys.argv[1:] == ["--vtk"]:
+        vtk_fixtures()
+        print("wrote VTK fixtures to", HERE)
+        return
     rng = np.random.default_rng(SEED)
     out = {}
 
@@ -130,6 +201,7 @@ def main():
     }
     with open(os.path.join(HERE, "shock_tube_200.json"), "w") as f:
         json.dump(rec, f, indent=1)
+    vtk_fixtures()
     print("wrote fixtures to", HERE)
 
 
