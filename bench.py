@@ -61,7 +61,7 @@ def per_kernel_bytes(N, F, mixed, rad):
     that touches it, gathers once, at the storage precision of the variant."""
     if mixed:
         face = 32 * N + 44 * F  # cell record read (v, p | s, D, cs2, ext); ids 8 + S*n 12 + packed flux write 24
-        cell = 140 * N + 24 * F  # ids+signs 16 + V 8 + U r/w 80 + p32 read 4 + record write 32; packed flux read
+        cell = 148 * N + 24 * F  # ids+signs 16 + (V, closure defect) 16 + U r/w 80 + p32 read 4 + record write 32; packed flux read
     else:
         face = 80 * N + 80 * F  # U+W read; ids 8 + (n,S) 32 + flux write 40
         cell = 153 * N + 40 * F  # ids 16 + sign 1 + V 8 + U r/w 80 + p read 8 + W write 40; flux read

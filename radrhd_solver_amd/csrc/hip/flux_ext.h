@@ -5,12 +5,13 @@
 // 2^-(23+BITS) relative (the remainder is rounded; |x - hi| <= ulp/2).
 //
 // Used for
-//  * the face flux S*(D, m, E): float4 (D, mx, my, mz) + uint2 (bits of E,
-//    four 8-bit counts for mx, my, mz, E) = 24 bytes, 2^-31 relative for m
-//    and E.  FP32 alone is not enough for those: the momentum flux p n S of a
-//    cell's faces cancels around the cell while the roundings do not, and at
-//    Lorentz factors ~10 c2p amplifies an error in E by W^2 E/p (~1e3-1e4)
-//    into the pressure.  D in FP32 leaves the errors unchanged;
+//  * the face flux S*(D, m, E - D): float4 (D, mx, my, mz) + uint2 (bits of
+//    E - D, four 8-bit counts for mx, my, mz, E - D) = 24 bytes, 2^-31
+//    relative for m and E - D.  FP32 alone is not enough for those: the
+//    momentum flux p n S of a cell's faces cancels around the cell while the
+//    roundings do not, and at Lorentz factors ~10 c2p amplifies an error in E
+//    by W^2 E/p (~1e3-1e4) into the pressure.  D in FP32 leaves the errors
+//    unchanged, since its rounding reaches D and E alike (pack below);
 //  * the cell record's s = rho h W^2 and D (8-bit counts): they must match
 //    the record's FP32 v and p far better than FP32 rounding does.
 // Host and device use the same functions.
@@ -61,7 +62,12 @@ struct Packed {
   uint2 b;
 };
 
-// Face flux record.
+// Face flux record.  The energy slot holds the flux of tau = E - D, and
+// unpack returns E as (FP32 D) + tau: the rounding of the D flux then enters
+// D and E alike and leaves E - D alone.  In a cold gas (p / rho ~ 1e-6) tau is
+// a millionth of E, and an E rebuilt from separately rounded D and E fluxes
+// carried the FP32 D rounding into tau amplified ~1e6 (transverse-momentum L1
+// 4e-4 against FP64; 3e-6 with tau).
 __host__ __device__ __forceinline__ Packed pack(double d, double mx, double my, double mz, double E) {
   Packed o;
   uint32_t e1, e2, e3, e4;
@@ -69,7 +75,7 @@ __host__ __device__ __forceinline__ Packed pack(double d, double mx, double my, 
   o.a.y = split<8>(mx, e1);
   o.a.z = split<8>(my, e2);
   o.a.w = split<8>(mz, e3);
-  o.b.x = f32_bits(split<8>(E, e4));
+  o.b.x = f32_bits(split<8>(E - d, e4));
   o.b.y = e1 | (e2 << 8) | (e3 << 16) | (e4 << 24);
   return o;
 }
@@ -79,7 +85,7 @@ __host__ __device__ __forceinline__ void unpack(const float4 &a, const uint2 &b,
   out[1] = join<8>(a.y, b.y);
   out[2] = join<8>(a.z, b.y >> 8);
   out[3] = join<8>(a.w, b.y >> 16);
-  out[4] = join<8>(bits_f32(b.x), b.y >> 24);
+  out[4] = (double)a.x + join<8>(bits_f32(b.x), b.y >> 24);
 }
 
 }  // namespace rhllc_ext

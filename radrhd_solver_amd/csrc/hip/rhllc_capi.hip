@@ -238,13 +238,34 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
   double *V_d;
   HIP_TRY(h->alloc(&lr_d, F));
   HIP_TRY(h->alloc(&cf_d, N));
-  HIP_TRY(h->alloc(&V_d, N));
   HIP_TRY(hipMemcpy(lr_d, lr.data(), sizeof(int2) * F, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(cf_d, cf.data(), sizeof(int4) * N, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(V_d, V.data(), sizeof(double) * N, hipMemcpyHostToDevice));
   d.face_lr = lr_d;
   d.cell_faces = cf_d;
-  d.cell_V = V_d;
+  if (!h->mixed) {
+    HIP_TRY(h->alloc(&V_d, N));
+    HIP_TRY(hipMemcpy(V_d, V.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    d.cell_V = V_d;
+  } else {
+    // (V, delta): delta = sum over the cell's faces of +-S n as the face
+    // kernel sees it (FP32), summed in FP64; the cell kernel subtracts the
+    // cell's own flux through it.
+    std::vector<float4> Vd(N);
+    for (int32_t i = 0; i < N; ++i) {
+      const int ids[4] = {cf[i].x, cf[i].y, cf[i].z, cf[i].w};
+      double dl[3] = {0.0, 0.0, 0.0};
+      for (int j = 0; j < 4; ++j) {
+        const float3 q = Sn32[ids[j] & 0x7FFFFFFF];
+        const double sg = ids[j] >= 0 ? 1.0 : -1.0;
+        dl[0] += sg * (double)q.x, dl[1] += sg * (double)q.y, dl[2] += sg * (double)q.z;
+      }
+      Vd[i] = make_float4((float)V[i], (float)dl[0], (float)dl[1], (float)dl[2]);
+    }
+    float4 *vd_d;
+    HIP_TRY(h->alloc(&vd_d, N));
+    HIP_TRY(hipMemcpy(vd_d, Vd.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
+    d.cell_Vd = vd_d;
+  }
   if (h->mixed) {
     float3 *q;
     HIP_TRY(h->alloc(&q, F));
@@ -581,7 +602,7 @@ extern "C" int rhllc_info(rhllc_solver *h, int64_t *n_cells, int64_t *n_faces, i
     const int64_t N = h->n_cells, F = h->n_faces;
     // Algorithmic bytes of one step (both kernels), each array element counted
     // once per kernel that touches it.
-    *device_bytes = h->mixed ? 172 * N + 68 * F : 233 * N + 120 * F;
+    *device_bytes = h->mixed ? 180 * N + 68 * F : 233 * N + 120 * F;
     if (h->rad) *device_bytes += 24 * N;
   }
   return RHLLC_OK;

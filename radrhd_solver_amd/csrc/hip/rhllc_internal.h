@@ -44,7 +44,8 @@ struct Dev {
   uint2 *FXf;               // mixed path: bits of the E part, extension counts
   // cells
   const int4 *cell_faces;  // face index | 0x80000000 when the cell is id_r (sign bit 0)
-  const double *cell_V;
+  const double *cell_V;    // FP64 path
+  const float4 *cell_Vd;   // mixed path: (V, closure defect sum_j sign_j face_Sn32_j)
   double4 *U4;   // (D, mx, my, mz), FP64 on both paths (c2p amplifies errors in U)
   double *UE;    // E
   double4 *W4;   // (rho, vx, vy, vz), FP64 path

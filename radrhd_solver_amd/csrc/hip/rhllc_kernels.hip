@@ -185,9 +185,20 @@ constexpr double kMlpGate = 0.02;
 
 // Gather of the four face fluxes in slot order with their signs
 // (rhllc_3d_stab.cpp:30-45) and U -= tau/V * sum.  Also returns the change of
-// tau = E - D (for the surrogate gate).
+// tau = E - D (for the surrogate gate) and, on the mixed path, the cell's
+// FP32 pressure (the c2p start).
+//
+// Mixed path: the FP32 face vectors S n of a cell do not sum to zero, so a
+// uniform state would see the net flux F(U) . delta, delta = sum_j sign_j S n_j
+// (~6e-8 S), where the FP64 geometry closes to rounding.  In a gas near rest
+// that flux, p delta, outweighed the physical momentum change (relative L1 3e-5
+// against FP64).  Subtracting the cell's own flux through delta,
+// (D v.delta, m v.delta + p delta, m.delta) with v.delta = m.delta / (E + p),
+// restores that balance; with exact geometry delta = 0 and the update is
+// unchanged.
 template <bool MIXED>
-__device__ __forceinline__ void update_U(const Dev &d, int c, double tau, State<double> &U, double &dtau) {
+__device__ __forceinline__ void update_U(const Dev &d, int c, double tau, State<double> &U, double &dtau,
+                                         float &p32) {
   const int4 cf = d.cell_faces[c];
   const int fidx[4] = {cf.x, cf.y, cf.z, cf.w};
   State<double> s;
@@ -202,9 +213,24 @@ __device__ __forceinline__ void update_U(const Dev &d, int c, double tau, State<
       s.d -= F.d, s.v[0] -= F.v[0], s.v[1] -= F.v[1], s.v[2] -= F.v[2], s.p -= F.p;
     }
   }
-  const double k = MIXED ? Fast::div(tau, d.cell_V[c]) : tau / d.cell_V[c];
-  s.d *= k, s.v[0] *= k, s.v[1] *= k, s.v[2] *= k, s.p *= k;
   load_U(d, c, U);
+  double k;
+  if (MIXED) {
+    const float4 vd = d.cell_Vd[c];
+    p32 = d.Wva[c].w;
+    const double dl[3] = {vd.y, vd.z, vd.w}, p = p32;
+    const double mdl = dot3(U.v, dl);
+    const double vdl = mdl * Fast::rcp(U.p + p);
+    s.d -= U.d * vdl;
+    s.v[0] -= U.v[0] * vdl + p * dl[0];
+    s.v[1] -= U.v[1] * vdl + p * dl[1];
+    s.v[2] -= U.v[2] * vdl + p * dl[2];
+    s.p -= mdl;
+    k = Fast::div(tau, (double)vd.x);
+  } else {
+    k = tau / d.cell_V[c];
+  }
+  s.d *= k, s.v[0] *= k, s.v[1] *= k, s.v[2] *= k, s.p *= k;
   U.d -= s.d, U.v[0] -= s.v[0], U.v[1] -= s.v[1], U.v[2] -= s.v[2], U.p -= s.p;
   dtau = s.p - s.d;
 }
@@ -326,7 +352,8 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
   const double tau = d.sc->tau;
   State<double> U;
   double dtau = 0.0;
-  if (live) update_U<MIXED>(d, c, tau, U, dtau);
+  float p32 = -1.f;
+  if (live) update_U<MIXED>(d, c, tau, U, dtau, p32);
   if (VARIANT == kMixedMlp) {
     const bool stale = live && !(fabs(dtau) <= kMlpGate * fabs(U.p - U.d));
     const unsigned long long m = __ballot(stale);
@@ -346,7 +373,7 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
   }
   double guess = -1.0;
   if (VARIANT == kFp64Prev && live) guess = d.Wp[c];
-  if (MIXED && live) guess = (double)d.Wva[c].w;
+  if (MIXED && live) guess = (double)p32;
   finish_cell<VARIANT, RAD, STATS>(d, c, live, U, guess, false, tau);
 }
 
