@@ -46,6 +46,11 @@ extern "C" {
                               face flux (with small extensions where FP32 is too coarse),
                               FP64 U and FP64 arithmetic */
 
+/* Equation of state: the reference's compile-time EOS (include/hd_config.h:16-20). */
+#define RHLLC_EOS_IDEAL 0 /* h = 1 + gamma/(gamma-1) p/rho, runtime gamma (EOS IDEAL) */
+#define RHLLC_EOS_TAUB 1  /* Taub-Mathews h = 2.5 theta + sqrt(2.25 theta^2 + 1), theta = p/rho
+                             (EOS TAUB, the reference's shipped setting); gamma unused */
+
 /* Initial pressure of the c2p Newton iteration. */
 #define RHLLC_GUESS_REFERENCE 0 /* max(|m|-E, 0, pmin) as rhllc_flux_stab.cpp:139-151 */
 #define RHLLC_GUESS_PREVIOUS 1  /* the previous step's pressure */
@@ -66,6 +71,8 @@ typedef struct rhllc_params {
   int32_t collect_stats; /* 1: Newton histogram + iteration totals each step */
   int32_t has_out_source; /* ghost state for boundary code -3 (e_bound_out_source) */
   double out_source_W[5], out_source_U[5];
+  int32_t eos;            /* RHLLC_EOS_*; 0 (zero-initialised params) is the ideal gas.
+                             RHLLC_GUESS_MLP needs RHLLC_EOS_IDEAL */
 } rhllc_params;
 
 #define RHLLC_HIST_BINS 41 /* Newton iterations 0..40 */
@@ -139,14 +146,15 @@ int rhllc_info(rhllc_solver *h, int64_t *n_cells, int64_t *n_faces, int64_t *dev
 
 /* Pointwise device evaluations over host arrays (n states, flux_t layout),
  * for per-function parity checks.  precision selects the FP64 (reference
- * arithmetic) or the mixed path's arithmetic of the flux and of c2p; nS4
- * holds (nx, ny, nz, S) per face. */
-int rhllc_eval_flux(int32_t precision, double gamma, int64_t n, const double *UL, const double *UR,
+ * arithmetic) or the mixed path's arithmetic of the flux and of c2p; eos is
+ * RHLLC_EOS_* (gamma is used by RHLLC_EOS_IDEAL only); nS4 holds
+ * (nx, ny, nz, S) per face. */
+int rhllc_eval_flux(int32_t precision, int32_t eos, double gamma, int64_t n, const double *UL, const double *UR,
                     const double *WL, const double *WR, const double *nS4, double *F, double *speed);
-int rhllc_eval_c2p(int32_t precision, double gamma, int64_t n, const double *U, const double *guess, double *W,
-                   int32_t *codes, int32_t *iters);
-int rhllc_eval_p2c(double gamma, int64_t n, double *W, double *U);
-int rhllc_eval_pressure_fix(double gamma, int64_t n, double *U, double *W, int32_t *codes);
+int rhllc_eval_c2p(int32_t precision, int32_t eos, double gamma, int64_t n, const double *U, const double *guess,
+                   double *W, int32_t *codes, int32_t *iters);
+int rhllc_eval_p2c(int32_t eos, double gamma, int64_t n, double *W, double *U);
+int rhllc_eval_pressure_fix(int32_t eos, double gamma, int64_t n, double *U, double *W, int32_t *codes);
 int rhllc_eval_rad_source(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
                           const double *ks, double *G);
 

@@ -6,8 +6,9 @@ checker.  It wraps:
 
 * ``liboracle_rhllc.so`` -- the C restatement in ``rhllc_oracle.c`` of the
   reference's ``rhllc::Hllc3dStab`` and its callees (FP64, AoS records);
-* ``_ref/libref_rhllc.so`` -- the reference's own sources compiled by
-  ``make ref`` (present only where the reference tree was available at build
+* ``_ref/libref_rhllc.so`` and ``_ref/libref_rhllc_taub.so`` -- the
+  reference's own sources compiled by ``make ref`` with EOS IDEAL and with
+  EOS TAUB (present only where the reference tree was available at build
   time).
 """
 from __future__ import annotations
@@ -21,6 +22,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "liboracle_rhllc.so")
 REF_LIB = os.path.join(HERE, "_ref", "libref_rhllc.so")
+REF_LIB_TAUB = os.path.join(HERE, "_ref", "libref_rhllc_taub.so")
 
 FLUX = np.dtype((np.float64, 5))
 ELEM_BYTES, FACE_BYTES = 136, 80
@@ -32,7 +34,11 @@ class Physics(C.Structure):
         ("has_out_source", C.c_int),
         ("out_source_W", C.c_double * 5),
         ("out_source_U", C.c_double * 5),
+        ("eos", C.c_int),
     ]
+
+
+EOS_IDEAL, EOS_TAUB = 0, 1  # ORC_EOS_* of rhllc_oracle.h
 
 
 class Stats(C.Structure):
@@ -53,7 +59,7 @@ class Stats(C.Structure):
 
 
 _orc = None
-_ref = None
+_ref = {}
 
 
 def build(ref: bool = True) -> None:
@@ -94,16 +100,20 @@ def lib():
     return _orc
 
 
-def have_ref() -> bool:
-    return os.path.exists(REF_LIB)
+def _ref_path(eos):
+    return REF_LIB_TAUB if eos == EOS_TAUB else REF_LIB
 
 
-def ref_lib():
-    global _ref
-    if _ref is None:
-        if not have_ref():
-            raise RuntimeError("oracle/_ref/libref_rhllc.so is not built (make -C oracle ref)")
-        L = C.CDLL(REF_LIB)
+def have_ref(eos=EOS_IDEAL) -> bool:
+    return os.path.exists(_ref_path(eos))
+
+
+def ref_lib(eos=EOS_IDEAL):
+    """The reference compiled with EOS IDEAL (default) or EOS TAUB."""
+    if eos not in _ref:
+        if not have_ref(eos):
+            raise RuntimeError(f"{_ref_path(eos)} is not built (make -C oracle ref)")
+        L = C.CDLL(_ref_path(eos))
         vp, i64, dbl, i32 = C.c_void_p, C.c_int64, C.c_double, C.c_int
         L.ref_layout.argtypes = [vp]
         L.ref_gamma.restype = dbl
@@ -118,42 +128,43 @@ def ref_lib():
         L.ref_convert_bin_to_geo.restype = i64
         L.ref_cone_states.argtypes = [vp, vp, vp]
         L.ref_set_log_dir(("/tmp/").encode())
-        _ref = L
-    return _ref
+        _ref[eos] = L
+    return _ref[eos]
 
 
-def physics(gamma: float = 5.0 / 3.0) -> Physics:
+def physics(gamma: float = 5.0 / 3.0, eos: int = EOS_IDEAL) -> Physics:
     ph = Physics()
     ph.gamma = gamma
+    ph.eos = eos
     return ph
 
 
 # ---------------------------------------------------------------- oracle calls
-def p2c(W, gamma=5.0 / 3.0):
+def p2c(W, gamma=5.0 / 3.0, eos=EOS_IDEAL):
     """GetConvValueStab over n states; returns (W_clamped, U)."""
     W = np.array(W, dtype=np.float64, order="C").reshape(-1, 5)
     U = np.zeros_like(W)
-    ph = physics(gamma)
+    ph = physics(gamma, eos)
     lib().orc_p2c_n(C.byref(ph), W.shape[0], W.ctypes.data, U.ctypes.data)
     return W, U
 
 
-def c2p(U, gamma=5.0 / 3.0, W_init=None):
+def c2p(U, gamma=5.0 / 3.0, W_init=None, eos=EOS_IDEAL):
     """GetPhysValueStab over n states; returns (W, codes, iters).  W keeps W_init where c2p fails."""
     U = np.ascontiguousarray(U, dtype=np.float64).reshape(-1, 5)
     W = np.zeros_like(U) if W_init is None else np.array(W_init, dtype=np.float64, order="C").reshape(-1, 5)
     codes = np.zeros(U.shape[0], np.int32)
     iters = np.zeros(U.shape[0], np.int32)
-    ph = physics(gamma)
+    ph = physics(gamma, eos)
     lib().orc_c2p_n(C.byref(ph), U.shape[0], U.ctypes.data, W.ctypes.data, codes.ctypes.data, iters.ctypes.data)
     return W, codes, iters
 
 
-def pressure_fix(U, W, gamma=5.0 / 3.0):
+def pressure_fix(U, W, gamma=5.0 / 3.0, eos=EOS_IDEAL):
     U = np.array(U, dtype=np.float64, order="C").reshape(-1, 5)
     W = np.array(W, dtype=np.float64, order="C").reshape(-1, 5)
     codes = np.zeros(U.shape[0], np.int32)
-    ph = physics(gamma)
+    ph = physics(gamma, eos)
     lib().orc_pressure_fix_n(C.byref(ph), U.shape[0], U.ctypes.data, W.ctypes.data, codes.ctypes.data)
     return U, W, codes
 
@@ -165,13 +176,13 @@ def rotation(n):
     return T.reshape(3, 3)
 
 
-def flux(UL, UR, WL, WR, nS4, gamma=5.0 / 3.0):
+def flux(UL, UR, WL, WR, nS4, gamma=5.0 / 3.0, eos=EOS_IDEAL):
     arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 5) for a in (UL, UR, WL, WR)]
     nS4 = np.ascontiguousarray(nS4, dtype=np.float64).reshape(-1, 4)
     n = nS4.shape[0]
     F = np.zeros((n, 5))
     speed = np.zeros(n)
-    ph = physics(gamma)
+    ph = physics(gamma, eos)
     lib().orc_flux_n(C.byref(ph), n, *[a.ctypes.data for a in arrs], nS4.ctypes.data, F.ctypes.data, speed.ctypes.data)
     return F, speed
 
@@ -179,7 +190,7 @@ def flux(UL, UR, WL, WR, nS4, gamma=5.0 / 3.0):
 class State:
     """AoS elem_t / face_t records of the oracle for one mesh."""
 
-    def __init__(self, mesh, W0, gamma=5.0 / 3.0):
+    def __init__(self, mesh, W0, gamma=5.0 / 3.0, eos=EOS_IDEAL):
         self.n_cells, self.n_faces = mesh.n_cells, mesh.n_faces
         self.cells = np.zeros(self.n_cells * ELEM_BYTES, np.uint8)
         self.faces = np.zeros(self.n_faces * FACE_BYTES, np.uint8)
@@ -188,7 +199,7 @@ class State:
             self.n_cells, mesh.cells.ctypes.data, W0.ctypes.data, self.n_faces, mesh.faces.ctypes.data,
             self.cells.ctypes.data, self.faces.ctypes.data,
         )
-        self.ph = physics(gamma)
+        self.ph = physics(gamma, eos)
         self.stats = Stats()
 
     def _view(self, off):
@@ -243,8 +254,8 @@ def rad_source(W, Ur, Fr, ka, ks, emission=0):
 
 
 # ------------------------------------------------------------- reference calls
-def ref_run(mesh, W0, cfl, h_min, steps):
-    L = ref_lib()
+def ref_run(mesh, W0, cfl, h_min, steps, eos=EOS_IDEAL):
+    L = ref_lib(eos)
     W0 = np.ascontiguousarray(W0, dtype=np.float64)
     Wo = np.zeros((mesh.n_cells, 5))
     Uo = np.zeros((mesh.n_cells, 5))
@@ -276,26 +287,26 @@ def ref_cone_states():
     return W0, Ws, Us
 
 
-def ref_p2c(W):
+def ref_p2c(W, eos=EOS_IDEAL):
     W = np.array(W, dtype=np.float64, order="C").reshape(-1, 5)
     U = np.zeros_like(W)
-    ref_lib().ref_p2c_n(W.shape[0], W.ctypes.data, U.ctypes.data)
+    ref_lib(eos).ref_p2c_n(W.shape[0], W.ctypes.data, U.ctypes.data)
     return W, U
 
 
-def ref_c2p(U, W_init=None):
+def ref_c2p(U, W_init=None, eos=EOS_IDEAL):
     U = np.ascontiguousarray(U, dtype=np.float64).reshape(-1, 5)
     W = np.zeros_like(U) if W_init is None else np.array(W_init, dtype=np.float64, order="C").reshape(-1, 5)
     codes = np.zeros(U.shape[0], np.int32)
-    ref_lib().ref_c2p_n(U.shape[0], U.ctypes.data, W.ctypes.data, codes.ctypes.data)
+    ref_lib(eos).ref_c2p_n(U.shape[0], U.ctypes.data, W.ctypes.data, codes.ctypes.data)
     return W, codes
 
 
-def ref_pressure_fix(U, W):
+def ref_pressure_fix(U, W, eos=EOS_IDEAL):
     U = np.array(U, dtype=np.float64, order="C").reshape(-1, 5)
     W = np.array(W, dtype=np.float64, order="C").reshape(-1, 5)
     codes = np.zeros(U.shape[0], np.int32)
-    ref_lib().ref_pressure_fix_n(U.shape[0], U.ctypes.data, W.ctypes.data, codes.ctypes.data)
+    ref_lib(eos).ref_pressure_fix_n(U.shape[0], U.ctypes.data, W.ctypes.data, codes.ctypes.data)
     return U, W, codes
 
 
@@ -306,11 +317,11 @@ def ref_rotation(normals):
     return T.reshape(-1, 3, 3)
 
 
-def ref_flux(UL, UR, WL, WR, nS4):
+def ref_flux(UL, UR, WL, WR, nS4, eos=EOS_IDEAL):
     arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 5) for a in (UL, UR, WL, WR)]
     nS4 = np.ascontiguousarray(nS4, dtype=np.float64).reshape(-1, 4)
     n = nS4.shape[0]
     F = np.zeros((n, 5))
     speed = np.zeros(n)
-    ref_lib().ref_flux_n(n, *[a.ctypes.data for a in arrs], nS4.ctypes.data, F.ctypes.data, speed.ctypes.data)
+    ref_lib(eos).ref_flux_n(n, *[a.ctypes.data for a in arrs], nS4.ctypes.data, F.ctypes.data, speed.ctypes.data)
     return F, speed

@@ -12,7 +12,8 @@
 // The harness mirrors the driver of solvers/rhllc/src/rhllc_main.cpp:28-56:
 // fill grid_t from geometry records and an initial W, HllcPhysToConv,
 // max_signal_speed = 1, tau = GetTimeStep, then {Hllc3dStab; GetTimeStep}.
-// EOS is compiled as IDEAL (see the Makefile); gamma is the reference's
+// EOS is compiled as IDEAL into libref_rhllc.so and as TAUB (the reference's
+// shipped setting) into libref_rhllc_taub.so (see the Makefile); gamma is the reference's
 // constexpr kGamma1 = 5/3.
 #include <cstring>
 #include <string>

@@ -83,10 +83,19 @@ int orc_num_threads(void) {
 #endif
 }
 
-/* rhllc_flux_stab.cpp:10-38 (EOS IDEAL) */
+/* Taub-Mathews enthalpy, the EOS TAUB form of rhllc_flux_stab.cpp:16-17,170,284-285. */
+static inline double taub_h(double theta) { return 2.5 * theta + sqrt(2.25 * theta * theta + 1.0); }
+
+/* rhllc_flux_stab.cpp:10-38 (EOS IDEAL :14, EOS TAUB :16-17) */
 void orc_p2c(const orc_physics_t *ph, orc_flux_t *W, orc_flux_t *U) {
-  const double Gg = ph->gamma / (ph->gamma - 1.0);
-  const double h = 1 + Gg * W->p / W->d;
+  double h;
+  if (ph->eos == ORC_EOS_TAUB) {
+    const double theta = W->p / W->d;
+    h = taub_h(theta);
+  } else {
+    const double Gg = ph->gamma / (ph->gamma - 1.0);
+    h = 1 + Gg * W->p / W->d;
+  }
   double g = dot3(W->v, W->v);
   if (g >= 1.0) {
     const double beta_fix = 0.9999;
@@ -102,8 +111,15 @@ void orc_p2c(const orc_physics_t *ph, orc_flux_t *W, orc_flux_t *U) {
   U->p = scrh - W->p;
 }
 
-/* rhllc_flux_stab.cpp:40-121 (EOS IDEAL): secant on u = Gamma*|v| at p = kMinPressure;
- * rewrites U.E on success (:115). */
+/* D h at pressure plor = p Gamma: EOS IDEAL D + plor alpha (rhllc_flux_stab.cpp:58,78,109),
+ * EOS TAUB 2.5 plor + sqrt(2.25 plor^2 + D^2) (:61,80,111). */
+static inline double fix_Dh(const orc_physics_t *ph, double D, double plor, double alpha) {
+  if (ph->eos == ORC_EOS_TAUB) return 2.5 * plor + sqrt(2.25 * plor * plor + D * D);
+  return D + plor * alpha;
+}
+
+/* rhllc_flux_stab.cpp:40-121: secant on u = Gamma*|v| at p = kMinPressure;
+ * rewrites U.E on success (:115).  alpha is Gg (EOS IDEAL, :57) or 2.5 (EOS TAUB, :60). */
 int orc_pressure_fix(const orc_physics_t *ph, orc_flux_t *U, orc_flux_t *W) {
   const double p = kMinPressure;
   const double D = U->d;
@@ -113,8 +129,8 @@ int orc_pressure_fix(const orc_physics_t *ph, orc_flux_t *U, orc_flux_t *W) {
   double u0 = umax;
   double lor = sqrt(1.0 + u0 * u0);
   double plor = p * lor;
-  const double alpha = ph->gamma / (ph->gamma - 1.0);
-  double Dh = D + plor * alpha;
+  const double alpha = ph->eos == ORC_EOS_TAUB ? 2.5 : ph->gamma / (ph->gamma - 1.0);
+  double Dh = fix_Dh(ph, D, plor, alpha);
   double f0 = m / Dh - u0;
   double u1 = (-D + sqrt(D * D + 4.0 * m * alpha * p)) / (2.0 * alpha * p);
   int done = 0;
@@ -122,7 +138,7 @@ int orc_pressure_fix(const orc_physics_t *ph, orc_flux_t *U, orc_flux_t *W) {
   for (int k = 1; k < MAX_ITER; k++) {
     lor = sqrt(1.0 + u1 * u1);
     plor = p * lor;
-    Dh = D + plor * alpha;
+    Dh = fix_Dh(ph, D, plor, alpha);
     f1 = m / Dh - u1;
     if (done == 1) break;
     du = (u1 - u0) / (f1 - f0) * f1;
@@ -136,7 +152,7 @@ int orc_pressure_fix(const orc_physics_t *ph, orc_flux_t *U, orc_flux_t *W) {
   if (done != 1) return 2;
   lor = sqrt(1.0 + u1 * u1);
   plor = p * lor;
-  Dh = D + plor * alpha;
+  Dh = fix_Dh(ph, D, plor, alpha);
   W->d = U->d / lor;
   W->p = p;
   U->p = Dh * lor - p;
@@ -145,7 +161,8 @@ int orc_pressure_fix(const orc_physics_t *ph, orc_flux_t *U, orc_flux_t *W) {
   return 0;
 }
 
-/* rhllc_flux_stab.cpp:123-220 (EOS IDEAL): Newton on f(p) = D h Gamma - E - p.
+/* rhllc_flux_stab.cpp:123-220: Newton on f(p) = D h Gamma - E - p (EOS IDEAL :166-169,
+ * EOS TAUB :170-174).
  * Return codes as there; 5 marks the DIE_IF(lor2 < 1.0) abort (:161). */
 int orc_c2p(const orc_physics_t *ph, const orc_flux_t *U, orc_flux_t *W, int *iters) {
   const double tol = 1.e-11;
@@ -176,9 +193,16 @@ int orc_c2p(const orc_physics_t *ph, const orc_flux_t *U, orc_flux_t *W, int *it
     lor = sqrt(lor2);
     tau = lor * D_1;
     theta = p * tau;
-    h = 1.0 + Gg * theta;
-    dh_dp = Gg * tau;
-    dh_dtau = Gg * p;
+    if (ph->eos == ORC_EOS_TAUB) {
+      h = taub_h(theta);
+      const double scrh = (5.0 * h - 8.0 * theta) / (2.0 * h - 5.0 * theta);
+      dh_dp = tau * scrh;
+      dh_dtau = p * scrh;
+    } else {
+      h = 1.0 + Gg * theta;
+      dh_dp = Gg * tau;
+      dh_dtau = Gg * p;
+    }
     yp = D * h * lor - E - p;
     dyp = D * lor * dh_dp - m2 * lor2 * lor / (alpha2 * alpha) * (lor * dh_dtau + D * h) - 1.0;
     dp = yp / dyp;
@@ -237,7 +261,8 @@ static inline void signal_speed(double cs2, double Vn, double V2, double *cmin, 
   *cmin = (Vn * (1.0 - cs2) - sroot) / (1.0 - V2 * cs2);
 }
 
-/* rhllc_flux_stab.cpp:240-431 (EOS IDEAL, HLLC branch of Mignone & Bodo 2006).
+/* rhllc_flux_stab.cpp:240-431 (HLLC branch of Mignone & Bodo 2006; enthalpy and sound
+ * speed of EOS IDEAL :276-283 or EOS TAUB :284-292).
  * F receives S * (flux rotated back to the global frame); returns max(lL, lR). */
 double orc_flux(const orc_physics_t *ph, const orc_flux_t *cUL, const orc_flux_t *cUR, const orc_flux_t *cWL,
                 const orc_flux_t *cWR, const double n[3], double S, orc_flux_t *Fout) {
@@ -253,10 +278,19 @@ double orc_flux(const orc_physics_t *ph, const orc_flux_t *cUL, const orc_flux_t
   const double *VL = WL.v, *VR = WR.v;
   const double dL = WL.d, dR = WR.d, pL = WL.p, pR = WR.p;
   const double VVL = dot3(VL, VL), VVR = dot3(VR, VR);
-  const double hL = 1 + Gg * pL / dL;
-  const double hR = 1 + Gg * pR / dR;
-  const double csL2 = ((G1 * pL) / (dL * hL));
-  const double csR2 = ((G1 * pR) / (dR * hR));
+  double hL, hR, csL2, csR2;
+  if (ph->eos == ORC_EOS_TAUB) {
+    const double thetaL = pL / dL, thetaR = pR / dR;
+    hL = taub_h(thetaL);
+    hR = taub_h(thetaR);
+    csL2 = thetaL / (3.0 * hL) * (5.0 * hL - 8.0 * thetaL) / (hL - thetaL);
+    csR2 = thetaR / (3.0 * hR) * (5.0 * hR - 8.0 * thetaR) / (hR - thetaR);
+  } else {
+    hL = 1 + Gg * pL / dL;
+    hR = 1 + Gg * pR / dR;
+    csL2 = ((G1 * pL) / (dL * hL));
+    csR2 = ((G1 * pR) / (dR * hR));
+  }
   double cmaxL, cmaxR, cminL, cminR;
   signal_speed(csL2, VL[0], VVL, &cminL, &cmaxL);
   signal_speed(csR2, VR[0], VVR, &cminR, &cmaxR);

@@ -55,10 +55,14 @@ typedef struct {
 
 /* Options of the physics that the reference fixes at compile time. */
 typedef struct {
-  double gamma;      /* kGamma1, lib/global_consts.h:20 (constexpr 5/3 there) */
+  double gamma;      /* kGamma1, lib/global_consts.h:20 (constexpr 5/3 there); EOS IDEAL only */
   int has_out_source;   /* e_bound_out_source ghost state below is valid */
   orc_flux_t out_source_W, out_source_U;
+  int eos;              /* ORC_EOS_IDEAL or ORC_EOS_TAUB (include/hd_config.h:16-20) */
 } orc_physics_t;
+
+#define ORC_EOS_IDEAL 0
+#define ORC_EOS_TAUB 1
 
 typedef struct {
   int64_t c2p_fail[5];   /* count by GetPhysValueStab return code 1..4 (index 0 unused) */

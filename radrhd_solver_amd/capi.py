@@ -11,6 +11,7 @@ import numpy as np
 
 PREC_FP64, PREC_MIXED = 0, 1
 GUESS_REFERENCE, GUESS_PREVIOUS, GUESS_MLP = 0, 1, 2
+EOS_IDEAL, EOS_TAUB = 0, 1
 HIST_BINS = 41
 
 
@@ -42,6 +43,7 @@ class Params(C.Structure):
         ("has_out_source", C.c_int32),
         ("out_source_W", C.c_double * 5),
         ("out_source_U", C.c_double * 5),
+        ("eos", C.c_int32),
     ]
 
 
@@ -95,10 +97,10 @@ SIGNATURES = {
     "rhllc_set_radiation": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "rhllc_time_kernels": (C.c_int, [_vp, _i32, _vp]),
     "rhllc_info": (C.c_int, [_vp, _vp, _vp, _vp]),
-    "rhllc_eval_flux": (C.c_int, [_i32, _dbl, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "rhllc_eval_c2p": (C.c_int, [_i32, _dbl, _i64, _vp, _vp, _vp, _vp, _vp]),
-    "rhllc_eval_p2c": (C.c_int, [_dbl, _i64, _vp, _vp]),
-    "rhllc_eval_pressure_fix": (C.c_int, [_dbl, _i64, _vp, _vp, _vp]),
+    "rhllc_eval_flux": (C.c_int, [_i32, _i32, _dbl, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "rhllc_eval_c2p": (C.c_int, [_i32, _i32, _dbl, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "rhllc_eval_p2c": (C.c_int, [_i32, _dbl, _i64, _vp, _vp]),
+    "rhllc_eval_pressure_fix": (C.c_int, [_i32, _dbl, _i64, _vp, _vp, _vp]),
     "rhllc_eval_rad_source": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_device_query": (C.c_int, [_i32, C.c_char_p, _i32, _vp, _vp]),
     "rhllc_copy_probe": (C.c_int, [_i32, _i64, _i32, _vp]),
@@ -142,13 +144,15 @@ class Solver:
     """Owns one device handle (``rhllc_solver``)."""
 
     def __init__(self, mesh, gamma=5.0 / 3.0, cfl=0.4, h_min=None, precision=PREC_FP64,
-                 guess=GUESS_REFERENCE, reorder=True, device=0, collect_stats=False, out_source=None):
+                 guess=GUESS_REFERENCE, reorder=True, device=0, collect_stats=False, out_source=None,
+                 eos=EOS_IDEAL):
         L = _lib()
         p = Params()
         p.gamma, p.cfl = gamma, cfl
         p.h_min = mesh.h_min() if h_min is None else h_min
         p.precision, p.c2p_guess, p.reorder, p.device = precision, guess, int(reorder), device
         p.collect_stats = int(collect_stats)
+        p.eos = eos
         if out_source is not None:
             p.has_out_source = 1
             for k in range(5):
@@ -238,41 +242,41 @@ class Solver:
 
 
 # ----------------------------------------------------------- pointwise evals
-def eval_flux(UL, UR, WL, WR, nS4, gamma=5.0 / 3.0, precision=PREC_FP64):
+def eval_flux(UL, UR, WL, WR, nS4, gamma=5.0 / 3.0, precision=PREC_FP64, eos=EOS_IDEAL):
     a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 5) for x in (UL, UR, WL, WR)]
     nS4 = np.ascontiguousarray(nS4, dtype=np.float64).reshape(-1, 4)
     n = nS4.shape[0]
     F = np.zeros((n, 5))
     sp = np.zeros(n)
-    check(_lib().rhllc_eval_flux(precision, gamma, n, *[x.ctypes.data for x in a], nS4.ctypes.data,
+    check(_lib().rhllc_eval_flux(precision, eos, gamma, n, *[x.ctypes.data for x in a], nS4.ctypes.data,
                                  F.ctypes.data, sp.ctypes.data))
     return F, sp
 
 
-def eval_c2p(U, gamma=5.0 / 3.0, guess=None, W_init=None, precision=PREC_FP64):
+def eval_c2p(U, gamma=5.0 / 3.0, guess=None, W_init=None, precision=PREC_FP64, eos=EOS_IDEAL):
     U = np.ascontiguousarray(U, dtype=np.float64).reshape(-1, 5)
     n = U.shape[0]
     W = np.zeros((n, 5)) if W_init is None else np.array(W_init, dtype=np.float64, order="C").reshape(n, 5)
     g = None if guess is None else np.ascontiguousarray(guess, dtype=np.float64)
     codes = np.zeros(n, np.int32)
     iters = np.zeros(n, np.int32)
-    check(_lib().rhllc_eval_c2p(precision, gamma, n, U.ctypes.data, _ptr(g), W.ctypes.data, codes.ctypes.data,
+    check(_lib().rhllc_eval_c2p(precision, eos, gamma, n, U.ctypes.data, _ptr(g), W.ctypes.data, codes.ctypes.data,
                                 iters.ctypes.data))
     return W, codes, iters
 
 
-def eval_p2c(W, gamma=5.0 / 3.0):
+def eval_p2c(W, gamma=5.0 / 3.0, eos=EOS_IDEAL):
     W = np.array(W, dtype=np.float64, order="C").reshape(-1, 5)
     U = np.zeros_like(W)
-    check(_lib().rhllc_eval_p2c(gamma, W.shape[0], W.ctypes.data, U.ctypes.data))
+    check(_lib().rhllc_eval_p2c(eos, gamma, W.shape[0], W.ctypes.data, U.ctypes.data))
     return W, U
 
 
-def eval_pressure_fix(U, W, gamma=5.0 / 3.0):
+def eval_pressure_fix(U, W, gamma=5.0 / 3.0, eos=EOS_IDEAL):
     U = np.array(U, dtype=np.float64, order="C").reshape(-1, 5)
     W = np.array(W, dtype=np.float64, order="C").reshape(-1, 5)
     codes = np.zeros(U.shape[0], np.int32)
-    check(_lib().rhllc_eval_pressure_fix(gamma, U.shape[0], U.ctypes.data, W.ctypes.data, codes.ctypes.data))
+    check(_lib().rhllc_eval_pressure_fix(eos, gamma, U.shape[0], U.ctypes.data, W.ctypes.data, codes.ctypes.data))
     return U, W, codes
 
 

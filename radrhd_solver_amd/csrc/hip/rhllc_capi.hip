@@ -117,7 +117,10 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
     return fail(RHLLC_ERR_ARG, "n_cells must be in [1, 2^25) (geo_face_t::id_l is 25 bits)");
   if (n_faces < 1 || n_faces > 0x7FFFFFFF) return fail(RHLLC_ERR_ARG, "n_faces out of range");
   const rhllc_params &p = *params;
-  if (!(p.gamma > 1.0)) return fail(RHLLC_ERR_ARG, "gamma must be > 1");
+  if (p.eos != RHLLC_EOS_IDEAL && p.eos != RHLLC_EOS_TAUB) return fail(RHLLC_ERR_ARG, "unknown eos");
+  if (p.eos == RHLLC_EOS_IDEAL && !(p.gamma > 1.0)) return fail(RHLLC_ERR_ARG, "gamma must be > 1");
+  if (p.eos == RHLLC_EOS_TAUB && p.c2p_guess == RHLLC_GUESS_MLP)
+    return fail(RHLLC_ERR_ARG, "the surrogate start is trained for the ideal gas; use RHLLC_EOS_IDEAL");
   if (!(p.cfl > 0.0) || !(p.h_min > 0.0)) return fail(RHLLC_ERR_ARG, "CFL and h_min must be > 0");
   if (p.precision != RHLLC_PREC_FP64 && p.precision != RHLLC_PREC_MIXED)
     return fail(RHLLC_ERR_ARG, "unknown precision");
@@ -228,8 +231,9 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
   Dev &d = h->d;
   d.n_cells = N;
   d.n_faces = F;
-  d.gamma = p.gamma;
-  d.Gg = p.gamma / (p.gamma - 1.0);
+  d.eos = p.eos;
+  d.gamma = p.eos == RHLLC_EOS_IDEAL ? p.gamma : 0.0;
+  d.Gg = p.eos == RHLLC_EOS_IDEAL ? p.gamma / (p.gamma - 1.0) : 0.0;
   d.cfl_hmin = p.cfl * p.h_min;
   for (int k = 0; k < 5; ++k) d.ghostW[k] = p.out_source_W[k], d.ghostU[k] = p.out_source_U[k];
 
@@ -623,9 +627,17 @@ hipError_t to_dev(DevBuf &b, const void *src, size_t bytes) {
 }
 }  // namespace
 
-extern "C" int rhllc_eval_flux(int32_t precision, double gamma, int64_t n, const double *UL, const double *UR,
-                               const double *WL, const double *WR, const double *nS4, double *F, double *speed) {
+namespace {
+bool bad_eos(int32_t eos, double gamma) {
+  return !(eos == RHLLC_EOS_TAUB || (eos == RHLLC_EOS_IDEAL && gamma > 1.0));
+}
+}  // namespace
+
+extern "C" int rhllc_eval_flux(int32_t precision, int32_t eos, double gamma, int64_t n, const double *UL,
+                               const double *UR, const double *WL, const double *WR, const double *nS4, double *F,
+                               double *speed) {
   if (n < 0 || !UL || !UR || !WL || !WR || !nS4 || !F || !speed) return fail(RHLLC_ERR_ARG, "bad argument");
+  if (bad_eos(eos, gamma)) return fail(RHLLC_ERR_ARG, "unknown eos, or gamma <= 1 for the ideal gas");
   DevBuf a, b, c, dd, e, f, g;
   const size_t s5 = sizeof(double) * 5 * n;
   HIP_TRY(to_dev(a, UL, s5));
@@ -635,7 +647,7 @@ extern "C" int rhllc_eval_flux(int32_t precision, double gamma, int64_t n, const
   HIP_TRY(to_dev(e, nS4, sizeof(double) * 4 * n));
   HIP_TRY(to_dev(f, nullptr, s5));
   HIP_TRY(to_dev(g, nullptr, sizeof(double) * n));
-  HIP_TRY(launch_eval_flux(precision == RHLLC_PREC_MIXED, gamma, n, (double *)a.p, (double *)b.p, (double *)c.p,
+  HIP_TRY(launch_eval_flux(precision == RHLLC_PREC_MIXED, eos, gamma, n, (double *)a.p, (double *)b.p, (double *)c.p,
                            (double *)dd.p, (double *)e.p, (double *)f.p, (double *)g.p, nullptr));
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(F, f.p, s5, hipMemcpyDeviceToHost));
@@ -643,9 +655,10 @@ extern "C" int rhllc_eval_flux(int32_t precision, double gamma, int64_t n, const
   return RHLLC_OK;
 }
 
-extern "C" int rhllc_eval_c2p(int32_t precision, double gamma, int64_t n, const double *U, const double *guess,
-                              double *W, int32_t *codes, int32_t *iters) {
+extern "C" int rhllc_eval_c2p(int32_t precision, int32_t eos, double gamma, int64_t n, const double *U,
+                              const double *guess, double *W, int32_t *codes, int32_t *iters) {
   if (n < 0 || !U || !W || !codes || !iters) return fail(RHLLC_ERR_ARG, "bad argument");
+  if (bad_eos(eos, gamma)) return fail(RHLLC_ERR_ARG, "unknown eos, or gamma <= 1 for the ideal gas");
   if (precision != RHLLC_PREC_FP64 && precision != RHLLC_PREC_MIXED) return fail(RHLLC_ERR_ARG, "bad precision");
   DevBuf a, b, c, dd, e;
   HIP_TRY(to_dev(a, U, sizeof(double) * 5 * n));
@@ -653,7 +666,7 @@ extern "C" int rhllc_eval_c2p(int32_t precision, double gamma, int64_t n, const 
   if (guess) HIP_TRY(to_dev(b, guess, sizeof(double) * n));
   HIP_TRY(to_dev(dd, nullptr, sizeof(int32_t) * n));
   HIP_TRY(to_dev(e, nullptr, sizeof(int32_t) * n));
-  HIP_TRY(launch_eval_c2p(precision == RHLLC_PREC_MIXED, gamma, n, (double *)a.p, (double *)b.p, (double *)c.p, (int32_t *)dd.p, (int32_t *)e.p,
+  HIP_TRY(launch_eval_c2p(precision == RHLLC_PREC_MIXED, eos, gamma, n, (double *)a.p, (double *)b.p, (double *)c.p, (int32_t *)dd.p, (int32_t *)e.p,
                           nullptr));
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(W, c.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost));
@@ -662,25 +675,27 @@ extern "C" int rhllc_eval_c2p(int32_t precision, double gamma, int64_t n, const 
   return RHLLC_OK;
 }
 
-extern "C" int rhllc_eval_p2c(double gamma, int64_t n, double *W, double *U) {
+extern "C" int rhllc_eval_p2c(int32_t eos, double gamma, int64_t n, double *W, double *U) {
   if (n < 0 || !W || !U) return fail(RHLLC_ERR_ARG, "bad argument");
+  if (bad_eos(eos, gamma)) return fail(RHLLC_ERR_ARG, "unknown eos, or gamma <= 1 for the ideal gas");
   DevBuf a, b;
   HIP_TRY(to_dev(a, W, sizeof(double) * 5 * n));
   HIP_TRY(to_dev(b, nullptr, sizeof(double) * 5 * n));
-  HIP_TRY(launch_eval_p2c(gamma, n, (double *)a.p, (double *)b.p, nullptr));
+  HIP_TRY(launch_eval_p2c(eos, gamma, n, (double *)a.p, (double *)b.p, nullptr));
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(W, a.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(U, b.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost));
   return RHLLC_OK;
 }
 
-extern "C" int rhllc_eval_pressure_fix(double gamma, int64_t n, double *U, double *W, int32_t *codes) {
+extern "C" int rhllc_eval_pressure_fix(int32_t eos, double gamma, int64_t n, double *U, double *W, int32_t *codes) {
   if (n < 0 || !U || !W || !codes) return fail(RHLLC_ERR_ARG, "bad argument");
+  if (bad_eos(eos, gamma)) return fail(RHLLC_ERR_ARG, "unknown eos, or gamma <= 1 for the ideal gas");
   DevBuf a, b, c;
   HIP_TRY(to_dev(a, U, sizeof(double) * 5 * n));
   HIP_TRY(to_dev(b, W, sizeof(double) * 5 * n));
   HIP_TRY(to_dev(c, nullptr, sizeof(int32_t) * n));
-  HIP_TRY(launch_eval_fix(gamma, n, (double *)a.p, (double *)b.p, (int32_t *)c.p, nullptr));
+  HIP_TRY(launch_eval_fix(eos, gamma, n, (double *)a.p, (double *)b.p, (int32_t *)c.p, nullptr));
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(U, a.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(W, b.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost));

@@ -33,6 +33,7 @@ struct Scalars {
 // wide loads.
 struct Dev {
   int32_t n_cells, n_faces;
+  int32_t eos;  // Eos (rhllc_physics.h): kIdeal or kTaub
   double gamma, Gg, cfl_hmin;
   // faces
   const int2 *face_lr;      // (left cell, right cell); right < 0: fixed ghost state
@@ -84,13 +85,14 @@ hipError_t launch_recover_W(const Dev &d, hipStream_t s);
 
 hipError_t launch_copy(const void *src, void *dst, int64_t n16, hipStream_t s);
 
-// Pointwise evaluations (n states) for parity tests.
-hipError_t launch_eval_flux(bool fp32, double gamma, int64_t n, const double *UL, const double *UR, const double *WL,
-                            const double *WR, const double *nS4, double *F, double *speed, hipStream_t s);
-hipError_t launch_eval_c2p(bool mixed, double gamma, int64_t n, const double *U, const double *guess, double *W,
-                           int32_t *codes, int32_t *iters, hipStream_t s);
-hipError_t launch_eval_p2c(double gamma, int64_t n, double *W, double *U, hipStream_t s);
-hipError_t launch_eval_fix(double gamma, int64_t n, double *U, double *W, int32_t *codes, hipStream_t s);
+// Pointwise evaluations (n states) for parity tests; eos is an Eos value.
+hipError_t launch_eval_flux(bool fp32, int eos, double gamma, int64_t n, const double *UL, const double *UR,
+                            const double *WL, const double *WR, const double *nS4, double *F, double *speed,
+                            hipStream_t s);
+hipError_t launch_eval_c2p(bool mixed, int eos, double gamma, int64_t n, const double *U, const double *guess,
+                           double *W, int32_t *codes, int32_t *iters, hipStream_t s);
+hipError_t launch_eval_p2c(int eos, double gamma, int64_t n, double *W, double *U, hipStream_t s);
+hipError_t launch_eval_fix(int eos, double gamma, int64_t n, double *U, double *W, int32_t *codes, hipStream_t s);
 hipError_t launch_eval_rad(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
                            const double *ks, double *G, hipStream_t s);
 

@@ -54,9 +54,10 @@ __device__ __forceinline__ void load_W32(const Dev &d, int c, State<double> &W) 
   W.v[0] = r.v[0], W.v[1] = r.v[1], W.v[2] = r.v[2], W.p = r.p;
   W.d = r.D * sqrt(1.0 - dot3(W.v, W.v));
 }
+template <int EOS>
 __device__ __forceinline__ void store_W32(const Dev &d, int c, const State<double> &W) {
   float4 va, sd;
-  mixed_record(d.gamma, d.Gg, W, va, sd);
+  mixed_record<EOS>(d.gamma, d.Gg, W, va, sd);
   d.Wva[c] = va;
   d.Wsd[c] = sd;
 }
@@ -80,7 +81,7 @@ __device__ __forceinline__ void fold_speed(const Dev &d, double speed) {
 }
 
 // One face: gather, HLLC flux, flux store; returns max(lL, lR).
-template <bool MIXED>
+template <bool MIXED, int EOS>
 __device__ __forceinline__ double face_flux(const Dev &d, int i) {
   double speed = 0.0;  // rhllc_3d_stab.cpp:18 -- per-thread max starts at 0
   const int2 lr = d.face_lr[i];
@@ -96,7 +97,7 @@ __device__ __forceinline__ double face_flux(const Dev &d, int i) {
       UR.d = d.ghostU[0], UR.v[0] = d.ghostU[1], UR.v[1] = d.ghostU[2], UR.v[2] = d.ghostU[3], UR.p = d.ghostU[4];
       WR.d = d.ghostW[0], WR.v[0] = d.ghostW[1], WR.v[1] = d.ghostW[2], WR.v[2] = d.ghostW[3], WR.p = d.ghostW[4];
     }
-    const double s = hllc_flux<double>(d.gamma, d.Gg, UL, UR, WL, WR, nS.x, nS.y, nS.z, nS.w, F);
+    const double s = hllc_flux<double, Ieee, EOS>(d.gamma, d.Gg, UL, UR, WL, WR, nS.x, nS.y, nS.z, nS.w, F);
     speed = smax(speed, s);
     d.F4[i] = make_double4(F.d, F.v[0], F.v[1], F.v[2]);
     d.FE[i] = F.p;
@@ -113,7 +114,7 @@ __device__ __forceinline__ double face_flux(const Dev &d, int i) {
       State<double> g;
       g.d = d.ghostW[0], g.v[0] = d.ghostW[1], g.v[1] = d.ghostW[2], g.v[2] = d.ghostW[3], g.p = d.ghostW[4];
       float4 va, sd;
-      mixed_record(d.gamma, d.Gg, g, va, sd);
+      mixed_record<EOS>(d.gamma, d.Gg, g, va, sd);
       WR = mixed_in(va, sd);
     }
     const double s = mixed_face_flux(WL, WR, Sn.x, Sn.y, Sn.z, F);
@@ -127,14 +128,14 @@ __device__ __forceinline__ double face_flux(const Dev &d, int i) {
 
 // kFacesPerThread faces per thread, block-strided so loads and stores stay
 // coalesced.
-template <bool MIXED>
+template <bool MIXED, int EOS>
 __global__ __launch_bounds__(kBlock) void k_face(Dev d) {
   const int base = (int)(xcd_block(blockIdx.x, gridDim.x) * (kBlock * kFacesPerThread) + threadIdx.x);
   double speed = 0.0;
 #pragma unroll
   for (int k = 0; k < kFacesPerThread; ++k) {
     const int i = base + k * kBlock;
-    if (i < d.n_faces) speed = smax(speed, face_flux<MIXED>(d, i));
+    if (i < d.n_faces) speed = smax(speed, face_flux<MIXED, EOS>(d, i));
   }
   fold_speed(d, speed);
 }
@@ -241,27 +242,28 @@ __device__ __forceinline__ void update_U(const Dev &d, int c, double tau, State<
 // (ballots) and, with STATS, by all threads of the block (histogram).
 //   first fallback: the previous pressure (when the start was the surrogate's)
 //   then the reference start (all variants but kFp64Ref), then PhysPressureFix
-template <int VARIANT, bool RAD, bool STATS>
+template <int VARIANT, bool RAD, bool STATS, int EOS>
 __device__ __forceinline__ void finish_cell(const Dev &d, int c, bool live, State<double> U, double guess,
                                             bool from_mlp, double tau) {
   constexpr bool MIXED = VARIANT >= kMixedPrev;
+  using A = typename std::conditional<MIXED, Fast, Ieee>::type;
   int iters = 0, code = 0;
   bool fell_back = false, fixed = false, fix_failed = false;
   double resid = 0.0;
   State<double> W;
   if (live) {
     // c2p writes W only when it succeeds.
-    code = MIXED ? c2p<Fast>(d.Gg, U, W, guess, iters) : c2p(d.Gg, U, W, guess, iters);
+    code = c2p<A, EOS>(d.Gg, U, W, guess, iters);
     if (code && from_mlp) {
       int it2 = 0;
       fell_back = true;
-      code = c2p<Fast>(d.Gg, U, W, (double)d.Wva[c].w, it2);
+      code = c2p<A, EOS>(d.Gg, U, W, (double)d.Wva[c].w, it2);
       iters += it2;
     }
     if (code && VARIANT != kFp64Ref) {
       int it2 = 0;
       fell_back = true;
-      code = MIXED ? c2p<Fast>(d.Gg, U, W, -1.0, it2) : c2p(d.Gg, U, W, -1.0, it2);
+      code = c2p<A, EOS>(d.Gg, U, W, -1.0, it2);
       iters += it2;
     }
     if (code) {
@@ -271,7 +273,7 @@ __device__ __forceinline__ void finish_cell(const Dev &d, int c, bool live, Stat
       else
         load_W32(d, c, W);
       fixed = true;
-      fix_failed = pressure_fix(d.Gg, U, W) != 0;
+      fix_failed = pressure_fix<EOS>(d.Gg, U, W) != 0;
     }
     if (RAD) {
       // radRHD_main.cpp:117-132: U += (ds*tau)*G(W), ds = 1, then c2p (code ignored).
@@ -289,20 +291,17 @@ __device__ __forceinline__ void finish_cell(const Dev &d, int c, bool live, Stat
       U.v[1] += ds * tau * G[2];
       U.v[2] += ds * tau * G[3];
       int it3 = 0;
-      if (MIXED)
-        c2p<Fast>(d.Gg, U, W, W.p, it3);
-      else
-        c2p(d.Gg, U, W, -1.0, it3);
+      c2p<A, EOS>(d.Gg, U, W, MIXED ? W.p : -1.0, it3);
       iters += it3;
     }
-    if (STATS) resid = c2p_residual(d.Gg, U, W.p);
+    if (STATS) resid = c2p_residual<EOS>(d.Gg, U, W.p);
     d.U4[c] = make_double4(U.d, U.v[0], U.v[1], U.v[2]);
     d.UE[c] = U.p;
     if (!MIXED) {
       d.W4[c] = make_double4(W.d, W.v[0], W.v[1], W.v[2]);
       d.Wp[c] = W.p;
     } else {
-      store_W32(d, c, W);
+      store_W32<EOS>(d, c, W);
     }
   }
   if (__ballot(live && (code != 0 || fixed || fell_back)) != 0) {  // rare: skip eight ballots otherwise
@@ -344,7 +343,7 @@ __device__ __forceinline__ void finish_cell(const Dev &d, int c, bool live, Stat
 // kMixedMlp: cells whose tau changed by more than kMlpGate are not finished
 // here; they are appended to d.mlp_list (one atomic per wave) and finished by
 // k_cell_mlp, which starts their c2p from the surrogate on full waves.
-template <int VARIANT, bool RAD, bool STATS>
+template <int VARIANT, bool RAD, bool STATS, int EOS>
 __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
   constexpr bool MIXED = VARIANT >= kMixedPrev;
   const int c = (int)(xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x);
@@ -374,13 +373,14 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
   double guess = -1.0;
   if (VARIANT == kFp64Prev && live) guess = d.Wp[c];
   if (MIXED && live) guess = (double)p32;
-  finish_cell<VARIANT, RAD, STATS>(d, c, live, U, guess, false, tau);
+  finish_cell<VARIANT, RAD, STATS, EOS>(d, c, live, U, guess, false, tau);
 }
 
 // Cells deferred by k_cell<kMixedMlp>: list entries [0, mlp_count), 64 per
 // wave so the surrogate's MFMA pass runs on full waves.  A fixed grid loops
 // over the list (the count is only known on the device); every block runs
 // the same number of iterations, and all of them end when the list does.
+// The surrogate is trained for the ideal gas only.
 template <bool RAD, bool STATS>
 __global__ __launch_bounds__(kBlock) void k_cell_mlp(Dev d) {
   const unsigned int n = d.sc->mlp_count;
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_mlp(Dev d) {
     U.d = 1.0, U.v[0] = 0.0, U.v[1] = 0.0, U.v[2] = 0.0, U.p = 2.0;  // dummy lanes
     if (live) load_U(d, c, U);
     const double g = mlp_guess(d, live, U);
-    finish_cell<kMixedMlp, RAD, STATS>(d, c, live, U, g, true, tau);
+    finish_cell<kMixedMlp, RAD, STATS, kIdeal>(d, c, live, U, g, true, tau);
   }
 }
 
@@ -425,12 +425,13 @@ __global__ __launch_bounds__(1024) void k_dt(Dev d) {
 }
 
 // HllcPhysToConv (rhllc_utils.cpp:25-33): U = p2c(W), W clamped in place.
+template <int EOS>
 __global__ __launch_bounds__(kBlock) void k_p2c_init(Dev d) {
   const int c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= d.n_cells) return;
   State<double> W, U;
   load_W(d, c, W);
-  p2c<double>(d.Gg, W, U);
+  p2c<double, Ieee, EOS>(d.Gg, W, U);
   d.W4[c] = make_double4(W.d, W.v[0], W.v[1], W.v[2]);
   d.Wp[c] = W.p;
   d.U4[c] = make_double4(U.d, U.v[0], U.v[1], U.v[2]);
@@ -439,15 +440,17 @@ __global__ __launch_bounds__(kBlock) void k_p2c_init(Dev d) {
 
 // Mixed path: the FP32 cell records from the FP64 primitives (after an
 // upload or HllcPhysToConv).
+template <int EOS>
 __global__ __launch_bounds__(kBlock) void k_seed_mixed(Dev d) {
   const int c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= d.n_cells) return;
   State<double> W;
   load_W(d, c, W);
-  store_W32(d, c, W);
+  store_W32<EOS>(d, c, W);
 }
 
 // Mixed path: FP64 primitives from the FP64 conserved state (for download).
+template <int EOS>
 __global__ __launch_bounds__(kBlock) void k_recover_W(Dev d) {
   const int c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= d.n_cells) return;
@@ -455,7 +458,7 @@ __global__ __launch_bounds__(kBlock) void k_recover_W(Dev d) {
   load_U(d, c, U);
   load_W32(d, c, W);
   int it = 0;
-  c2p(d.Gg, U, W, W.p, it);  // W keeps the FP32 values if c2p fails
+  c2p<Ieee, EOS>(d.Gg, U, W, W.p, it);  // W keeps the FP32 values if c2p fails
   d.W4[c] = make_double4(W.d, W.v[0], W.v[1], W.v[2]);
   d.Wp[c] = W.p;
 }
@@ -482,49 +485,64 @@ static inline unsigned int face_blocks(int64_t n) {
 
 hipError_t launch_p2c_init(const Dev &d, hipStream_t s) {
   if (d.n_cells == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_p2c_init, dim3(blocks_for(d.n_cells)), dim3(kBlock), 0, s, d);
+  if (d.eos == kTaub)
+    hipLaunchKernelGGL(k_p2c_init<kTaub>, dim3(blocks_for(d.n_cells)), dim3(kBlock), 0, s, d);
+  else
+    hipLaunchKernelGGL(k_p2c_init<kIdeal>, dim3(blocks_for(d.n_cells)), dim3(kBlock), 0, s, d);
   return hipGetLastError();
 }
 
 hipError_t launch_face(const Dev &d, bool mixed, hipStream_t s) {
   if (d.n_faces == 0) return hipSuccess;
-  if (mixed)
-    hipLaunchKernelGGL(k_face<true>, dim3(face_blocks(d.n_faces)), dim3(kBlock), 0, s, d);
-  else
-    hipLaunchKernelGGL(k_face<false>, dim3(face_blocks(d.n_faces)), dim3(kBlock), 0, s, d);
+  const dim3 g(face_blocks(d.n_faces)), b(kBlock);
+  if (d.eos == kTaub) {
+    if (mixed)
+      hipLaunchKernelGGL((k_face<true, kTaub>), g, b, 0, s, d);
+    else
+      hipLaunchKernelGGL((k_face<false, kTaub>), g, b, 0, s, d);
+  } else {
+    if (mixed)
+      hipLaunchKernelGGL((k_face<true, kIdeal>), g, b, 0, s, d);
+    else
+      hipLaunchKernelGGL((k_face<false, kIdeal>), g, b, 0, s, d);
+  }
   return hipGetLastError();
 }
 
-template <int V>
+template <int V, int EOS>
 static void launch_cell_v(const Dev &d, bool rad, bool stats, hipStream_t s) {
   const dim3 g(blocks_for(d.n_cells)), b(kBlock);
   if (rad) {
     if (stats)
-      hipLaunchKernelGGL((k_cell<V, true, true>), g, b, 0, s, d);
+      hipLaunchKernelGGL((k_cell<V, true, true, EOS>), g, b, 0, s, d);
     else
-      hipLaunchKernelGGL((k_cell<V, true, false>), g, b, 0, s, d);
+      hipLaunchKernelGGL((k_cell<V, true, false, EOS>), g, b, 0, s, d);
   } else {
     if (stats)
-      hipLaunchKernelGGL((k_cell<V, false, true>), g, b, 0, s, d);
+      hipLaunchKernelGGL((k_cell<V, false, true, EOS>), g, b, 0, s, d);
     else
-      hipLaunchKernelGGL((k_cell<V, false, false>), g, b, 0, s, d);
+      hipLaunchKernelGGL((k_cell<V, false, false, EOS>), g, b, 0, s, d);
   }
 }
 
+// The surrogate start (kMixedMlp) exists for the ideal gas only; rhllc_create
+// rejects it with the Taub-Mathews EOS.
 hipError_t launch_cell(const Dev &d, int variant, bool rad, bool stats, hipStream_t s) {
   if (d.n_cells == 0) return hipSuccess;
+  const bool taub = d.eos == kTaub;
   switch (variant) {
     case kFp64Ref:
-      launch_cell_v<kFp64Ref>(d, rad, stats, s);
+      taub ? launch_cell_v<kFp64Ref, kTaub>(d, rad, stats, s) : launch_cell_v<kFp64Ref, kIdeal>(d, rad, stats, s);
       break;
     case kFp64Prev:
-      launch_cell_v<kFp64Prev>(d, rad, stats, s);
+      taub ? launch_cell_v<kFp64Prev, kTaub>(d, rad, stats, s) : launch_cell_v<kFp64Prev, kIdeal>(d, rad, stats, s);
       break;
     case kMixedPrev:
-      launch_cell_v<kMixedPrev>(d, rad, stats, s);
+      taub ? launch_cell_v<kMixedPrev, kTaub>(d, rad, stats, s) : launch_cell_v<kMixedPrev, kIdeal>(d, rad, stats, s);
       break;
     case kMixedMlp:
-      launch_cell_v<kMixedMlp>(d, rad, stats, s);
+      if (taub) return hipErrorInvalidValue;
+      launch_cell_v<kMixedMlp, kIdeal>(d, rad, stats, s);
       break;
     default:
       return hipErrorInvalidValue;
@@ -559,12 +577,18 @@ hipError_t launch_dt(const Dev &d, hipStream_t s) {
 
 hipError_t launch_seed_mixed(const Dev &d, hipStream_t s) {
   if (d.n_cells == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_seed_mixed, dim3(blocks_for(d.n_cells)), dim3(kBlock), 0, s, d);
+  if (d.eos == kTaub)
+    hipLaunchKernelGGL(k_seed_mixed<kTaub>, dim3(blocks_for(d.n_cells)), dim3(kBlock), 0, s, d);
+  else
+    hipLaunchKernelGGL(k_seed_mixed<kIdeal>, dim3(blocks_for(d.n_cells)), dim3(kBlock), 0, s, d);
   return hipGetLastError();
 }
 hipError_t launch_recover_W(const Dev &d, hipStream_t s) {
   if (d.n_cells == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_recover_W, dim3(blocks_for(d.n_cells)), dim3(kBlock), 0, s, d);
+  if (d.eos == kTaub)
+    hipLaunchKernelGGL(k_recover_W<kTaub>, dim3(blocks_for(d.n_cells)), dim3(kBlock), 0, s, d);
+  else
+    hipLaunchKernelGGL(k_recover_W<kIdeal>, dim3(blocks_for(d.n_cells)), dim3(kBlock), 0, s, d);
   return hipGetLastError();
 }
 
@@ -581,7 +605,7 @@ __device__ __forceinline__ void st5(double *p, const State<double> &s) {
   p[0] = s.d, p[1] = s.v[0], p[2] = s.v[1], p[3] = s.v[2], p[4] = s.p;
 }
 
-template <bool FP32>
+template <bool FP32, int EOS>
 __global__ void k_eval_flux(double gamma, int64_t n, const double *UL, const double *UR, const double *WL,
                             const double *WR, const double *nS4, double *F, double *speed) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -590,8 +614,8 @@ __global__ void k_eval_flux(double gamma, int64_t n, const double *UL, const dou
   State<double> ul, ur, wl, wr, f;
   ld5(UL + 5 * i, ul), ld5(UR + 5 * i, ur), ld5(WL + 5 * i, wl), ld5(WR + 5 * i, wr);
   if (!FP32) {
-    speed[i] = hllc_flux<double>(gamma, Gg, ul, ur, wl, wr, nS4[4 * i], nS4[4 * i + 1], nS4[4 * i + 2],
-                                 nS4[4 * i + 3], f);
+    speed[i] = hllc_flux<double, Ieee, EOS>(gamma, Gg, ul, ur, wl, wr, nS4[4 * i], nS4[4 * i + 1], nS4[4 * i + 2],
+                                            nS4[4 * i + 3], f);
     st5(F + 5 * i, f);
   } else {
     // The mixed face path: FP32 primitives and geometry, FP64 fast arithmetic,
@@ -599,8 +623,8 @@ __global__ void k_eval_flux(double gamma, int64_t n, const double *UL, const dou
     // The cell records the mixed cell kernel would store for these states.
     State<double> ff;
     float4 la, ls, ra, rs;
-    mixed_record(gamma, Gg, wl, la, ls);
-    mixed_record(gamma, Gg, wr, ra, rs);
+    mixed_record<EOS>(gamma, Gg, wl, la, ls);
+    mixed_record<EOS>(gamma, Gg, wr, ra, rs);
     const double *g = nS4 + 4 * i;
     speed[i] = mixed_face_flux(mixed_in(la, ls), mixed_in(ra, rs), (float)(g[0] * g[3]), (float)(g[1] * g[3]),
                                (float)(g[2] * g[3]), ff);
@@ -609,6 +633,7 @@ __global__ void k_eval_flux(double gamma, int64_t n, const double *UL, const dou
   }
 }
 
+template <int EOS>
 __global__ void k_eval_c2p(bool mixed, double gamma, int64_t n, const double *U, const double *guess, double *W,
                            int32_t *codes, int32_t *iters) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -618,28 +643,30 @@ __global__ void k_eval_c2p(bool mixed, double gamma, int64_t n, const double *U,
   ld5(W + 5 * i, w);
   int it = 0;
   const double Gg = gamma / (gamma - 1.0), g0 = guess ? guess[i] : -1.0;
-  codes[i] = mixed ? c2p<Fast>(Gg, u, w, g0, it) : c2p(Gg, u, w, g0, it);
+  codes[i] = mixed ? c2p<Fast, EOS>(Gg, u, w, g0, it) : c2p<Ieee, EOS>(Gg, u, w, g0, it);
   iters[i] = it;
   st5(W + 5 * i, w);
 }
 
+template <int EOS>
 __global__ void k_eval_p2c(double gamma, int64_t n, double *W, double *U) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   State<double> w, u;
   ld5(W + 5 * i, w);
-  p2c<double>(gamma / (gamma - 1.0), w, u);
+  p2c<double, Ieee, EOS>(gamma / (gamma - 1.0), w, u);
   st5(W + 5 * i, w);
   st5(U + 5 * i, u);
 }
 
+template <int EOS>
 __global__ void k_eval_fix(double gamma, int64_t n, double *U, double *W, int32_t *codes) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   State<double> u, w;
   ld5(U + 5 * i, u);
   ld5(W + 5 * i, w);
-  codes[i] = pressure_fix(gamma / (gamma - 1.0), u, w);
+  codes[i] = pressure_fix<EOS>(gamma / (gamma - 1.0), u, w);
   st5(U + 5 * i, u);
   st5(W + 5 * i, w);
 }
@@ -666,29 +693,49 @@ hipError_t launch_copy(const void *src, void *dst, int64_t n16, hipStream_t s) {
 
 static inline dim3 eval_grid(int64_t n) { return dim3((unsigned int)((n + 255) / 256)); }
 
-hipError_t launch_eval_flux(bool fp32, double gamma, int64_t n, const double *UL, const double *UR, const double *WL,
-                            const double *WR, const double *nS4, double *F, double *speed, hipStream_t s) {
-  if (n == 0) return hipSuccess;
+template <int EOS>
+static void eval_flux_e(bool fp32, double gamma, int64_t n, const double *UL, const double *UR, const double *WL,
+                        const double *WR, const double *nS4, double *F, double *speed, hipStream_t s) {
   if (fp32)
-    hipLaunchKernelGGL(k_eval_flux<true>, eval_grid(n), dim3(256), 0, s, gamma, n, UL, UR, WL, WR, nS4, F, speed);
+    hipLaunchKernelGGL((k_eval_flux<true, EOS>), eval_grid(n), dim3(256), 0, s, gamma, n, UL, UR, WL, WR, nS4, F,
+                       speed);
   else
-    hipLaunchKernelGGL(k_eval_flux<false>, eval_grid(n), dim3(256), 0, s, gamma, n, UL, UR, WL, WR, nS4, F, speed);
+    hipLaunchKernelGGL((k_eval_flux<false, EOS>), eval_grid(n), dim3(256), 0, s, gamma, n, UL, UR, WL, WR, nS4, F,
+                       speed);
+}
+hipError_t launch_eval_flux(bool fp32, int eos, double gamma, int64_t n, const double *UL, const double *UR,
+                            const double *WL, const double *WR, const double *nS4, double *F, double *speed,
+                            hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (eos == kTaub)
+    eval_flux_e<kTaub>(fp32, gamma, n, UL, UR, WL, WR, nS4, F, speed, s);
+  else
+    eval_flux_e<kIdeal>(fp32, gamma, n, UL, UR, WL, WR, nS4, F, speed, s);
   return hipGetLastError();
 }
-hipError_t launch_eval_c2p(bool mixed, double gamma, int64_t n, const double *U, const double *guess, double *W,
-                           int32_t *codes, int32_t *iters, hipStream_t s) {
+hipError_t launch_eval_c2p(bool mixed, int eos, double gamma, int64_t n, const double *U, const double *guess,
+                           double *W, int32_t *codes, int32_t *iters, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_eval_c2p, eval_grid(n), dim3(256), 0, s, mixed, gamma, n, U, guess, W, codes, iters);
+  if (eos == kTaub)
+    hipLaunchKernelGGL(k_eval_c2p<kTaub>, eval_grid(n), dim3(256), 0, s, mixed, gamma, n, U, guess, W, codes, iters);
+  else
+    hipLaunchKernelGGL(k_eval_c2p<kIdeal>, eval_grid(n), dim3(256), 0, s, mixed, gamma, n, U, guess, W, codes, iters);
   return hipGetLastError();
 }
-hipError_t launch_eval_p2c(double gamma, int64_t n, double *W, double *U, hipStream_t s) {
+hipError_t launch_eval_p2c(int eos, double gamma, int64_t n, double *W, double *U, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_eval_p2c, eval_grid(n), dim3(256), 0, s, gamma, n, W, U);
+  if (eos == kTaub)
+    hipLaunchKernelGGL(k_eval_p2c<kTaub>, eval_grid(n), dim3(256), 0, s, gamma, n, W, U);
+  else
+    hipLaunchKernelGGL(k_eval_p2c<kIdeal>, eval_grid(n), dim3(256), 0, s, gamma, n, W, U);
   return hipGetLastError();
 }
-hipError_t launch_eval_fix(double gamma, int64_t n, double *U, double *W, int32_t *codes, hipStream_t s) {
+hipError_t launch_eval_fix(int eos, double gamma, int64_t n, double *U, double *W, int32_t *codes, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_eval_fix, eval_grid(n), dim3(256), 0, s, gamma, n, U, W, codes);
+  if (eos == kTaub)
+    hipLaunchKernelGGL(k_eval_fix<kTaub>, eval_grid(n), dim3(256), 0, s, gamma, n, U, W, codes);
+  else
+    hipLaunchKernelGGL(k_eval_fix<kIdeal>, eval_grid(n), dim3(256), 0, s, gamma, n, U, W, codes);
   return hipGetLastError();
 }
 hipError_t launch_eval_rad(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,

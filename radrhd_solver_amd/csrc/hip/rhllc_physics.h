@@ -12,8 +12,13 @@
 //   rotation     GetRotationMatrix         lib/geometry/src/linear_alg.cpp:3-42
 //   hllc_flux    rhllc::GetFluxStab        solvers/rhllc/src/rhllc_flux_stab.cpp:231-431
 //
-// EOS is the ideal gas with a runtime adiabatic index (the reference's
-// kGamma1 is a constexpr 5/3, lib/global_consts.h:20-21).
+// EOS (template parameter, include/hd_config.h:16-20 selects it at compile
+// time in the reference):
+//   kIdeal  ideal gas, h = 1 + Gg p/rho, Gg = gamma/(gamma - 1), with a runtime
+//           gamma (the reference's kGamma1 is a constexpr 5/3,
+//           lib/global_consts.h:20-21);
+//   kTaub   Taub-Mathews, h = 2.5 theta + sqrt(2.25 theta^2 + 1), theta = p/rho
+//           (the reference's EOS TAUB branches; its shipped default).
 //
 // Sums of three products follow the reference's Eigen 3.4 evaluation on
 // x86-64 (2-double packets): dot products (a0+a1)+a2; rows 0 and 1 of T*v
@@ -27,6 +32,8 @@
 namespace rhllc_dev {
 
 constexpr int kMaxIter = 40;  // rhllc_flux_stab.cpp:8
+
+enum Eos : int { kIdeal = 0, kTaub = 1 };
 
 template <typename R>
 struct State {
@@ -143,9 +150,15 @@ __device__ __forceinline__ void rot_back(const Rot<R> &T, const R *v, R *r) {
 }
 
 // Conserved from primitive; clamps |v| >= 1 to 0.9999 and rewrites W.v as the reference does.
-template <typename R, typename A = Ieee>
+template <typename R, typename A = Ieee, int EOS = kIdeal>
 __device__ __forceinline__ void p2c(R Gg, State<R> &W, State<R> &U) {
-  const R h = R(1) + A::div(Gg * W.p, W.d);
+  R h;
+  if (EOS == kIdeal) {
+    h = R(1) + A::div(Gg * W.p, W.d);
+  } else {
+    const R theta = A::div(W.p, W.d);
+    h = R(2.5) * theta + A::sqrt_(R(2.25) * theta * theta + R(1));
+  }
   R g = dot3(W.v, W.v);
   if (g >= R(1)) {
     const R beta_fix = R(0.9999);
@@ -162,6 +175,12 @@ __device__ __forceinline__ void p2c(R Gg, State<R> &W, State<R> &U) {
 }
 
 // Secant fallback at p = 1e-12 (kMinPressure); rewrites U.E on success.  Returns 0 or 2.
+// D h at pressure p lor: ideal D + plor Gg; Taub-Mathews 2.5 plor + sqrt(2.25 plor^2 + D^2).
+template <int EOS>
+__device__ __forceinline__ double fix_Dh(double Gg, double D, double plor) {
+  return EOS == kIdeal ? D + plor * Gg : 2.5 * plor + sqrt(2.25 * plor * plor + D * D);
+}
+template <int EOS = kIdeal>
 __device__ __forceinline__ int pressure_fix(double Gg, State<double> &U, State<double> &W) {
   const double p = 1e-12;
   const double D = U.d;
@@ -171,8 +190,8 @@ __device__ __forceinline__ int pressure_fix(double Gg, State<double> &U, State<d
   double u0 = umax;
   double lor = sqrt(1.0 + u0 * u0);
   double plor = p * lor;
-  const double alpha = Gg;
-  double Dh = D + plor * alpha;
+  const double alpha = EOS == kIdeal ? Gg : 2.5;
+  double Dh = fix_Dh<EOS>(Gg, D, plor);
   double f0 = m / Dh - u0;
   double u1 = (-D + sqrt(D * D + 4.0 * m * alpha * p)) / (2.0 * alpha * p);
   int done = 0;
@@ -180,7 +199,7 @@ __device__ __forceinline__ int pressure_fix(double Gg, State<double> &U, State<d
   for (int k = 1; k < kMaxIter; k++) {
     lor = sqrt(1.0 + u1 * u1);
     plor = p * lor;
-    Dh = D + plor * alpha;
+    Dh = fix_Dh<EOS>(Gg, D, plor);
     f1 = m / Dh - u1;
     if (done == 1) break;
     du = (u1 - u0) / (f1 - f0) * f1;
@@ -194,7 +213,7 @@ __device__ __forceinline__ int pressure_fix(double Gg, State<double> &U, State<d
   if (done != 1) return 2;
   lor = sqrt(1.0 + u1 * u1);
   plor = p * lor;
-  Dh = D + plor * alpha;
+  Dh = fix_Dh<EOS>(Gg, D, plor);
   W.d = U.d / lor;
   W.p = p;
   U.p = Dh * lor - p;
@@ -220,7 +239,7 @@ __device__ __forceinline__ int pressure_fix(double Gg, State<double> &U, State<d
 //    pressure is within 2.3e-11 p of the root and |f|/(E + p) <= ~1e-11 --
 //    the reference's own stop (|dp| < 1e-11 E) usually costs one more
 //    iteration to confirm what the previous one already reached.
-template <typename A = Ieee>
+template <typename A = Ieee, int EOS = kIdeal>
 __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<double> &W, double guess, int &iters) {
   const double tol = 1.e-11;
   const double eps2 = 1.e-12;
@@ -272,9 +291,16 @@ __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<doub
     }
     tau = lor * D_1;
     theta = p * tau;
-    h = 1.0 + Gg * theta;
-    dh_dp = Gg * tau;
-    dh_dtau = Gg * p;
+    if (EOS == kIdeal) {
+      h = 1.0 + Gg * theta;
+      dh_dp = Gg * tau;
+      dh_dtau = Gg * p;
+    } else {
+      h = 2.5 * theta + A::sqrt_(2.25 * theta * theta + 1.0);
+      const double scrh = A::div(5.0 * h - 8.0 * theta, 2.0 * h - 5.0 * theta);
+      dh_dp = tau * scrh;
+      dh_dtau = p * scrh;
+    }
     yp = D * h * lor - E - p;
     dyp = D * lor * dh_dp - mlor3 * (lor * dh_dtau + D * h) - 1.0;
     dp = A::div(yp, dyp);
@@ -310,12 +336,14 @@ __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<doub
 
 // |f(p)| / (E + p) of the c2p equation at the recovered state (used for the
 // residual check of the mixed path; not part of the reference).
+template <int EOS = kIdeal>
 __device__ __forceinline__ double c2p_residual(double Gg, const State<double> &U, double p) {
   const double D = U.d, E = U.p;
   const double m2 = dot3(U.v, U.v);
   const double alpha = E + p;
   const double lor = sqrt(1.0 / (1.0 - m2 / (alpha * alpha)));
-  const double h = 1.0 + Gg * p * lor / D;
+  const double theta = p * lor / D;
+  const double h = EOS == kIdeal ? 1.0 + Gg * theta : 2.5 * theta + sqrt(2.25 * theta * theta + 1.0);
   return fabs(D * h * lor - E - p) / alpha;
 }
 
@@ -330,7 +358,7 @@ __device__ __forceinline__ void signal_speed(R cs2, R Vn, R V2, R &cmin, R &cmax
 
 // HLLC flux through a face with unit normal n (pointing out of the left cell)
 // and area S.  F receives S * flux in the global frame; returns max(lL, lR).
-template <typename R, typename A = Ieee>
+template <typename R, typename A = Ieee, int EOS = kIdeal>
 __device__ __forceinline__ R hllc_flux(R G1, R Gg, const State<R> &cUL, const State<R> &cUR, const State<R> &cWL,
                                        const State<R> &cWR, R n0, R n1, R n2, R S, State<R> &Fout) {
   const Rot<R> T = rotation<R, A>(n0, n1, n2);
@@ -342,10 +370,19 @@ __device__ __forceinline__ R hllc_flux(R G1, R Gg, const State<R> &cUL, const St
   const R *VL = WL.v, *VR = WR.v;
   const R dL = WL.d, dR = WR.d, pL = WL.p, pR = WR.p;
   const R VVL = dot3(VL, VL), VVR = dot3(VR, VR);
-  const R hL = R(1) + A::div(Gg * pL, dL);
-  const R hR = R(1) + A::div(Gg * pR, dR);
-  const R csL2 = A::div(G1 * pL, dL * hL);
-  const R csR2 = A::div(G1 * pR, dR * hR);
+  R hL, hR, csL2, csR2;
+  if (EOS == kIdeal) {
+    hL = R(1) + A::div(Gg * pL, dL);
+    hR = R(1) + A::div(Gg * pR, dR);
+    csL2 = A::div(G1 * pL, dL * hL);
+    csR2 = A::div(G1 * pR, dR * hR);
+  } else {
+    const R thL = A::div(pL, dL), thR = A::div(pR, dR);
+    hL = R(2.5) * thL + A::sqrt_(R(2.25) * thL * thL + R(1));
+    hR = R(2.5) * thR + A::sqrt_(R(2.25) * thR * thR + R(1));
+    csL2 = A::div(A::div(thL, R(3) * hL) * (R(5) * hL - R(8) * thL), hL - thL);
+    csR2 = A::div(A::div(thR, R(3) * hR) * (R(5) * hR - R(8) * thR), hR - thR);
+  }
   R cmaxL, cmaxR, cminL, cminR;
   signal_speed<R, A>(csL2, VL[0], VVL, cminL, cmaxL);
   signal_speed<R, A>(csR2, VR[0], VVR, cminR, cmaxR);
@@ -430,6 +467,7 @@ struct MixedIn {
 };
 
 // The record of an FP64 primitive state (|v| < 1).
+template <int EOS = kIdeal>
 __device__ __forceinline__ void mixed_record(double gamma, double Gg, const State<double> &W, float4 &va,
                                              float4 &sd) {
 #pragma clang fp contract(fast)
@@ -438,11 +476,20 @@ __device__ __forceinline__ void mixed_record(double gamma, double Gg, const Stat
   const double rho = (float)W.d, p = va.w;
   const double v[3] = {va.x, va.y, va.z};
   const double lor2 = A::rcp(1.0 - dot3(v, v));
-  const double rh = rho + Gg * p;  // rho h
+  double rh, cs2;  // rho h, sound speed^2
+  if (EOS == kIdeal) {
+    rh = rho + Gg * p;
+    cs2 = A::div(gamma * p, rh);
+  } else {
+    const double th = A::div(p, rho);
+    const double h = 2.5 * th + A::sqrt_(2.25 * th * th + 1.0);
+    rh = rho * h;
+    cs2 = A::div(th * (5.0 * h - 8.0 * th), 3.0 * h * (h - th));
+  }
   uint32_t es, eD;
   sd.x = rhllc_ext::split<8>(rh * lor2, es);
   sd.y = rhllc_ext::split<8>(rho * A::sqrt_(lor2), eD);
-  sd.z = (float)A::div(gamma * p, rh);
+  sd.z = (float)cs2;
   sd.w = rhllc_ext::bits_f32(es | (eD << 8));
 }
 

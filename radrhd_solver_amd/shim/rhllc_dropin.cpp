@@ -73,6 +73,11 @@ struct DeviceGrid {
 DeviceGrid g_dev;
 
 int precision_code(const std::string &s) { return s == "mixed" ? RHLLC_PREC_MIXED : RHLLC_PREC_FP64; }
+int eos_code(const std::string &s) {
+  if (s == "taub") return RHLLC_EOS_TAUB;
+  if (s == "ideal") return RHLLC_EOS_IDEAL;
+  die("hllc_cfg.json: \"eos\" must be \"taub\" or \"ideal\", got", s);
+}
 int guess_code(const std::string &s) {
   if (s == "previous") return RHLLC_GUESS_PREVIOUS;
   if (s == "mlp") return RHLLC_GUESS_MLP;
@@ -90,6 +95,7 @@ void ensure_device(grid_t &grid) {
   for (size_t i = 0; i < F; ++i) std::memcpy(&gf[40 * i], &grid.faces[i].geo, 40);
   rhllc_params p;
   std::memset(&p, 0, sizeof(p));
+  p.eos = eos_code(_hllc_cfg.eos);
   p.gamma = _hllc_cfg.gamma;
   p.cfl = _hllc_cfg.CFL;
   p.h_min = _hllc_cfg.h_min;
@@ -193,7 +199,7 @@ void HllcPhysToConv(std::vector<elem_t> &cells) {
   const size_t n = cells.size();
   std::vector<double> W(5 * n), U(5 * n);
   for (size_t i = 0; i < n; ++i) std::memcpy(&W[5 * i], &cells[i].phys_val, 40);
-  if (rhllc_eval_p2c(_hllc_cfg.gamma, (int64_t)n, W.data(), U.data()) != RHLLC_OK)
+  if (rhllc_eval_p2c(eos_code(_hllc_cfg.eos), _hllc_cfg.gamma, (int64_t)n, W.data(), U.data()) != RHLLC_OK)
     die("HllcPhysToConv", rhllc_last_error());
   for (size_t i = 0; i < n; ++i) {
     std::memcpy(static_cast<void *>(&cells[i].phys_val), &W[5 * i], 40);
@@ -331,6 +337,7 @@ int Read(const std::string &file, hllc_value_t &st) {
   if (load(file, kv)) return e_completion_fail;
   bool ok = need(kv, "T", st.T, file) && need(kv, "CFL", st.CFL, file) && need(kv, "h_min", st.h_min, file) &&
             need(kv, "save_timer", st.save_timer, file) && need(kv, "tau", st.tau, file);
+  opt(kv, "eos", st.eos);
   opt(kv, "gamma", st.gamma);
   opt(kv, "precision", st.precision);
   opt(kv, "c2p_start", st.c2p_start);

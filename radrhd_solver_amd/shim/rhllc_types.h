@@ -96,11 +96,14 @@ static_assert(sizeof(geo_cell_t) == 56, "geo_cell_t layout");
 static_assert(sizeof(elem_t) == 136, "elem_t layout");
 
 // lib/json/include/json_struct.h:15-26 (all keys required).  The drop-in also
-// reads optional keys the reference ignores: "gamma" (EOS index, default
-// 5/3 = the reference's kGamma1), "precision" ("fp64" | "mixed") and
-// "c2p_start" ("reference" | "previous" | "mlp") with "mlp_model".
+// reads optional keys the reference ignores: "eos" ("taub", the reference's
+// shipped EOS, include/hd_config.h:20, or "ideal"), "gamma" (index of the
+// ideal gas, default 5/3 = the reference's kGamma1), "precision" ("fp64" |
+// "mixed") and "c2p_start" ("reference" | "previous" | "mlp") with
+// "mlp_model".
 struct hllc_value_t {
   Type T = 0, CFL = 0, h_min = 0, save_timer = 0, tau = 0;
+  std::string eos = "taub";
   Type gamma = 5.0 / 3.0;
   std::string precision = "fp64";
   std::string c2p_start = "reference";

@@ -15,8 +15,11 @@ compiled reference returns.  The fixtures are data only (inputs + outputs):
   shock_tube_200.json  config 0 (200x10x10 hexes, 120k tets) after 200 steps: per-field sums and hashes
   brick.vtk.xz, sphere.vtk.xz  the reference's test meshes (tests/data), compressed
   brick_cone.json  the reference's RHLLC test on brick.vtk to T = 0.5: geometry and state hashes
+  taub.npz, brick_cone_taub.json  the pointwise functions, a tiny trajectory and the brick test
+                 from the reference built with EOS TAUB (its shipped setting)
 
-``python make_golden.py --vtk`` rewrites only the last three.
+``python make_golden.py --vtk`` rewrites only the brick/sphere files,
+``--taub`` only the EOS TAUB files.
 """
 import hashlib
 import json
@@ -114,12 +117,85 @@ def vtk_fixtures():
         json.dump(rec, f, indent=1)
 
 
+def taub_fixtures():
+    """The same pointwise functions and a tiny trajectory from the reference
+    compiled with EOS TAUB (its shipped include/hd_config.h:20), and the
+    brick.vtk test of vtk_fixtures() with EOS TAUB."""
+    T = O.EOS_TAUB
+    rng = np.random.default_rng(SEED + 100)
+    W = prims(rng, 500)
+    W[:25, 1:4] *= 1.0 / np.linalg.norm(W[:25, 1:4], axis=1)[:, None] * 1.0001
+    Wc, U = O.ref_p2c(W, eos=T)
+    out = {"p2c_W_in": W, "p2c_W_out": Wc, "p2c_U": U}
+    W = prims(rng, 2000)
+    _, U = O.ref_p2c(W, eos=T)
+    U[:100, 4] = np.sqrt(U[:100, 0] ** 2 + (U[:100, 1:4] ** 2).sum(1)) * (1 - 1e-3)  # Q < 0: code 1
+    U[100:200, 4] *= 0.999999  # near-admissible states
+    W_init = np.full((U.shape[0], 5), -7.0)
+    Wo, codes = O.ref_c2p(U, W_init, eos=T)
+    out.update(c2p_U=U, c2p_W_init=W_init, c2p_W_out=Wo, c2p_codes=codes)
+    bad = codes != 0
+    Uf, Wf, cf = O.ref_pressure_fix(U[bad], Wo[bad], eos=T)
+    out.update(fix_U_in=U[bad], fix_W_in=Wo[bad], fix_U_out=Uf, fix_W_out=Wf, fix_codes=cf)
+    k = 3000
+    WL = prims(rng, k, rho=(-2, 2), pr=(-3, 2), wmax=10)
+    WR = prims(rng, k, rho=(-2, 2), pr=(-3, 2), wmax=10)
+    WL[:300, 1:4] = [0.99, 0, 0]
+    WR[:300, 1:4] = [0.99, 0, 0]
+    WL[300:600, 1:4] = [-0.99, 0, 0]
+    WR[300:600, 1:4] = [-0.99, 0, 0]
+    _, UL = O.ref_p2c(WL, eos=T)
+    _, UR = O.ref_p2c(WR, eos=T)
+    nn = unit(rng, k)
+    nn[600:606] = np.eye(3).tolist() + (-np.eye(3)).tolist()
+    nS = np.column_stack([nn, rng.uniform(0.01, 2.0, k)])
+    F, sp = O.ref_flux(UL, UR, WL, WR, nS, eos=T)
+    out.update(flux_UL=UL, flux_UR=UR, flux_WL=WL, flux_WR=WR, flux_nS=nS, flux_F=F, flux_speed=sp)
+    m = M.kuhn_box(8, 2, 2, length=(1.0, 0.25, 0.25), seed=SEED)
+    W0 = M.init_state(m, M.INIT_SHOCK_TUBE)
+    h = m.h_min()
+    Wt, Ut, taus, speeds = O.ref_run(m, W0, 0.4, h, 200, eos=T)
+    out.update(traj_geo_cells=m.cells.view(np.uint8).reshape(-1), traj_geo_faces=m.faces.view(np.uint8).reshape(-1),
+               traj_W0=W0, traj_cfl=0.4, traj_h_min=h, traj_W=Wt, traj_U=Ut, traj_taus=taus, traj_speeds=speeds)
+    np.savez(os.path.join(HERE, "taub.npz"), **out)
+
+    # brick.vtk with EOS TAUB, the configuration the reference ships
+    with open(os.path.join(HERE, "brick_cone.json")) as f:
+        ideal = json.load(f)
+    from oracle import mesh_geo as G
+
+    P, Tt = G.read_legacy_vtk(os.path.join(REF_DATA, "brick.vtk"))
+    nb, nrm, ar, vol, cen = G.build_data_from_vtk(P, Tt, "Cone")
+    cells, faces = O.ref_convert_bin_to_geo(nb, nrm, ar, vol, cen)
+    m = M.Mesh(np.frombuffer(bytearray(cells.tobytes()), M.GEO_CELL_DTYPE),
+               np.frombuffer(bytearray(faces.tobytes()), M.GEO_FACE_DTYPE))
+    W0 = np.tile(np.array(ideal["W_init"]), (m.n_cells, 1))
+    _, _, taus, _ = O.ref_run(m, W0, ideal["cfl"], ideal["h_min"], 2000, eos=T)
+    t, steps = 0.0, 0
+    while t < ideal["T"]:
+        t += taus[steps]
+        steps += 1
+    Wt, Ut, taus, speeds = O.ref_run(m, W0, ideal["cfl"], ideal["h_min"], steps, eos=T)
+    t = 0.0
+    for tau in taus:
+        t += float(tau)
+    rec = {"source": ideal["source"] + ", reference built with EOS TAUB", "steps": steps, "t": t,
+           "taus_sha256": sha(taus), "last_speed": float(speeds[-1]), "U_sha256": sha(Ut), "W_sha256": sha(Wt),
+           "U_abs_sum": [float(np.abs(Ut[:, j]).sum()) for j in range(5)]}
+    with open(os.path.join(HERE, "brick_cone_taub.json"), "w") as f:
+        json.dump(rec, f, indent=1)
+
+
 def main():
     if not O.have_ref():
         sys.exit("oracle/_ref/libref_rhllc.so is not built (make -C oracle ref)")
     if sys.argv[1:] == ["--vtk"]:
         vtk_fixtures()
         print("wrote VTK fixtures to", HERE)
+        return
+    if sys.argv[1:] == ["--taub"]:
+        taub_fixtures()
+        print("wrote EOS TAUB fixtures to", HERE)
         return
     rng = np.random.default_rng(SEED)
     out = {}
@@ -202,6 +278,7 @@ def main():
     with open(os.path.join(HERE, "shock_tube_200.json"), "w") as f:
         json.dump(rec, f, indent=1)
     vtk_fixtures()
+    taub_fixtures()
     print("wrote fixtures to", HERE)
 
 

@@ -90,3 +90,27 @@ def test_planted_mistake_is_caught():
     F_bad = F.copy()
     F_bad[:, 1] -= d["WL"][:, 4] * d["nS"][:, 0] * d["nS"][:, 3] * 1e-15 + 1e-300
     assert not np.array_equal(F_bad, d["F"])
+
+
+# ------------------------------------------------ EOS TAUB (the reference's shipped EOS)
+def test_taub_pointwise_bitwise():
+    d = load("taub.npz")
+    T = O.EOS_TAUB
+    Wc, U = O.p2c(d["p2c_W_in"], eos=T)
+    assert np.array_equal(Wc, d["p2c_W_out"]) and np.array_equal(U, d["p2c_U"])
+    W, codes, _ = O.c2p(d["c2p_U"], W_init=d["c2p_W_init"], eos=T)
+    assert np.array_equal(codes, d["c2p_codes"]) and np.array_equal(W, d["c2p_W_out"])
+    assert (d["c2p_codes"] != 0).sum() >= 100
+    U2, W2, c2 = O.pressure_fix(d["fix_U_in"], d["fix_W_in"], eos=T)
+    assert np.array_equal(c2, d["fix_codes"]) and np.array_equal(U2, d["fix_U_out"]) and np.array_equal(W2, d["fix_W_out"])
+    F, sp = O.flux(d["flux_UL"], d["flux_UR"], d["flux_WL"], d["flux_WR"], d["flux_nS"], eos=T)
+    assert np.array_equal(F, d["flux_F"]) and np.array_equal(sp, d["flux_speed"])
+
+
+def test_taub_tiny_trajectory_200_steps_bitwise():
+    d = load("taub.npz")
+    m = M.Mesh(d["traj_geo_cells"].view(M.GEO_CELL_DTYPE).copy(), d["traj_geo_faces"].view(M.GEO_FACE_DTYPE).copy())
+    st = O.State(m, d["traj_W0"], eos=O.EOS_TAUB)
+    _, taus, speeds = st.run(float(d["traj_cfl"]), float(d["traj_h_min"]), 200)
+    assert np.array_equal(taus, d["traj_taus"]) and np.array_equal(speeds, d["traj_speeds"])
+    assert np.array_equal(st.U, d["traj_U"]) and np.array_equal(st.W, d["traj_W"])

@@ -39,7 +39,7 @@ def test_rhllc_solver_executable_matches_oracle(tmp_path):
     W0 = M.init_state(m, M.INIT_SHOCK_TUBE)
     h = m.h_min()
     T = 60 * 0.4 * h
-    dirs = M.write_case(str(tmp_path), m, W0, T=T, CFL=0.4, save_timer=1e9)
+    dirs = M.write_case(str(tmp_path), m, W0, T=T, CFL=0.4, save_timer=1e9, eos="ideal")
     st, nsteps = _oracle_loop(m, W0, 0.4, h, T)
     assert nsteps > 20
     r = subprocess.run([SOLVER_EXE, str(tmp_path / "directories_cfg.json")], capture_output=True, text=True, timeout=120)

@@ -80,11 +80,11 @@ def test_eval_c2p_p2c_fix_bitwise():
     assert np.array_equal(fo, fg) and np.array_equal(Uo2, Ug2) and np.array_equal(Wo2, Wg2)
 
 
-def _run_both(mesh, W0, cfl, steps, gamma=5.0 / 3.0, graph=True, reorder=True):
+def _run_both(mesh, W0, cfl, steps, gamma=5.0 / 3.0, graph=True, reorder=True, eos=capi.EOS_IDEAL):
     h = mesh.h_min()
-    st = O.State(mesh, W0, gamma)
+    st = O.State(mesh, W0, gamma, eos=eos)
     t_o, taus_o, speeds_o = st.run(cfl, h, steps)
-    s = capi.Solver(mesh, gamma=gamma, cfl=cfl, h_min=h, reorder=reorder)
+    s = capi.Solver(mesh, gamma=gamma, cfl=cfl, h_min=h, reorder=reorder, eos=eos)
     s.upload_W(W0)
     s.run(steps, graph=graph)
     s.sync()
@@ -176,3 +176,80 @@ def test_step_radiation_source_bitwise():
     W, U = s.download()
     s.close()
     assert np.array_equal(st.U, U) and np.array_equal(st.W, W)
+
+
+# ------------------------------------------------ EOS TAUB (the reference's shipped EOS)
+def test_eval_taub_bitwise_against_oracle_and_reference_vectors():
+    import os
+
+    T = capi.EOS_TAUB
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "taub.npz"), allow_pickle=False)
+    Wc, U = capi.eval_p2c(d["p2c_W_in"], eos=T)
+    assert np.array_equal(Wc, d["p2c_W_out"]) and np.array_equal(U, d["p2c_U"])
+    W, codes, _ = capi.eval_c2p(d["c2p_U"], W_init=d["c2p_W_init"], eos=T)
+    assert np.array_equal(codes, d["c2p_codes"]) and np.array_equal(W, d["c2p_W_out"])
+    U2, W2, c2 = capi.eval_pressure_fix(d["fix_U_in"], d["fix_W_in"], eos=T)
+    assert np.array_equal(c2, d["fix_codes"]) and np.array_equal(U2, d["fix_U_out"]) and np.array_equal(W2, d["fix_W_out"])
+    F, sp = capi.eval_flux(d["flux_UL"], d["flux_UR"], d["flux_WL"], d["flux_WR"], d["flux_nS"], eos=T)
+    assert np.array_equal(F, d["flux_F"]) and np.array_equal(sp, d["flux_speed"])
+    # and against the oracle on fresh states
+    rng = np.random.default_rng(SEED + 7)
+    n = 20000
+    WL, WR = random_prims(rng, n), random_prims(rng, n)
+    _, UL = O.p2c(WL, eos=T)
+    _, UR = O.p2c(WR, eos=T)
+    nS = random_faces(rng, n)
+    Fo, so = O.flux(UL, UR, WL, WR, nS, eos=T)
+    Fg, sg = capi.eval_flux(UL, UR, WL, WR, nS, eos=T)
+    assert np.array_equal(Fo, Fg) and np.array_equal(so, sg)
+    Wo, co, io = O.c2p(UL, eos=T)
+    Wg, cg, ig = capi.eval_c2p(UL, eos=T)
+    assert np.array_equal(co, cg) and np.array_equal(io, ig) and np.array_equal(Wo, Wg)
+
+
+def test_step_taub_bitwise():
+    """Shock tube and blast with the Taub-Mathews EOS: GPU FP64 step bitwise the oracle
+    (itself bitwise the reference built with EOS TAUB), and the reference's own
+    200-step trajectory from tests/golden/taub.npz."""
+    import os
+
+    T = capi.EOS_TAUB
+    m = M.kuhn_box(40, 4, 4, length=(1.0, 0.1, 0.1), seed=SEED)
+    W0 = M.init_state(m, M.INIT_SHOCK_TUBE)
+    st, (t_o, taus, speeds), (Wg, Ug, stats) = _run_both(m, W0, 0.4, 150, eos=T)
+    assert np.array_equal(st.U, Ug) and np.array_equal(st.W, Wg)
+    assert stats["max_speed"] == speeds[-1] and stats["t"] == t_o
+    m = M.kuhn_box(10, 10, 10, seed=SEED)
+    W0 = M.init_state(m, M.INIT_BLAST, params=[0.5, 0.5, 0.5, 0.25, 1.0, 1000.0, 1.0])
+    st, _, (Wg, Ug, _) = _run_both(m, W0, 0.4, 60, eos=T)
+    assert np.array_equal(st.U, Ug) and np.array_equal(st.W, Wg)
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "taub.npz"), allow_pickle=False)
+    m = M.Mesh(d["traj_geo_cells"].view(M.GEO_CELL_DTYPE).copy(), d["traj_geo_faces"].view(M.GEO_FACE_DTYPE).copy())
+    s = capi.Solver(m, cfl=float(d["traj_cfl"]), h_min=float(d["traj_h_min"]), eos=T)
+    s.upload_W(d["traj_W0"])
+    s.run(200)
+    s.sync()
+    Wg, Ug = s.download()
+    s.close()
+    assert np.array_equal(Ug, d["traj_U"]) and np.array_equal(Wg, d["traj_W"])
+
+
+def test_mixed_taub_blast_l1():
+    T = capi.EOS_TAUB
+    m = M.kuhn_box(16, 16, 16, seed=SEED)
+    W0 = M.init_state(m, M.INIT_BLAST)
+    out = {}
+    for prec, guess in ((capi.PREC_FP64, capi.GUESS_REFERENCE), (capi.PREC_MIXED, capi.GUESS_PREVIOUS)):
+        s = capi.Solver(m, cfl=0.4, precision=prec, guess=guess, eos=T)
+        s.upload_W(W0)
+        s.run(60)
+        s.sync()
+        out[prec] = (s.download()[1], s.stats())
+        s.close()
+    U64, U32 = out[capi.PREC_FP64][0], out[capi.PREC_MIXED][0]
+    assert out[capi.PREC_MIXED][1]["fix_fail"] == 0 and sum(out[capi.PREC_MIXED][1]["c2p_fail"]) == 0
+    for k in range(5):
+        ref = np.abs(U64[:, k]).sum()
+        assert np.abs(U32[:, k] - U64[:, k]).sum() <= 1e-5 * ref, k
+    with pytest.raises(capi.RhllcError):
+        capi.Solver(m, precision=capi.PREC_MIXED, guess=capi.GUESS_MLP, eos=T)

@@ -6,7 +6,9 @@ codes, initial state and inflow, CFL 0.7, h_min 1e-5, run to T = 0.5
 (302 steps).  tests/golden/brick_cone.json holds the hashes of the compiled
 reference's state at T = 0.5 (tests/golden/make_golden.py).  The FP64 GPU
 step must reproduce it bit for bit, through the C ABI and through the
-drop-in executables (rhllc_make_geo, then rhllc_solver).
+drop-in executables (rhllc_make_geo, then rhllc_solver); the drop-in's default
+EOS TAUB run matches tests/golden/brick_cone_taub.json, the same test from the
+reference built with its shipped EOS.
 """
 import glob
 import hashlib
@@ -89,14 +91,25 @@ def _read(path, width):
         return np.frombuffer(f.read(8 * n * width), np.float64).reshape(n, width)
 
 
-def test_brick_drop_in_executables(brick_vtk, gold, tmp_path):
+@pytest.mark.parametrize("eos", ["ideal", None])
+def test_brick_drop_in_executables(brick_vtk, gold, tmp_path, eos):
     """rhllc_make_geo <base> brick.vtk, then rhllc_solver with the test's
-    configuration and no init file (the Cone default state)."""
+    configuration and no init file (the Cone default state).  Without an "eos"
+    key the drop-in runs the reference's shipped EOS TAUB, checked against the
+    reference built that way (tests/golden/brick_cone_taub.json)."""
+    if eos is None:
+        with open(os.path.join(GOLD, "brick_cone_taub.json")) as f:
+            expect = json.load(f)
+    else:
+        expect = gold
     base = str(tmp_path) + os.sep
     r = subprocess.run([MAKE_GEO_EXE, base, brick_vtk], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+    cfg = {"T": gold["T"], "CFL": gold["cfl"], "h_min": gold["h_min"], "save_timer": 0.1, "tau": 0.0}
+    if eos is not None:
+        cfg["eos"] = eos
     with open(base + "hllc_cfg.json", "w") as f:
-        json.dump({"T": gold["T"], "CFL": gold["cfl"], "h_min": gold["h_min"], "save_timer": 0.1, "tau": 0.0}, f)
+        json.dump(cfg, f)
     with open(base + "solve_cfg.json", "w") as f:
         json.dump({"class_vtk": 0, "max_number_of_iter": 1, "accuracy": 1e-5, "start_point": 0}, f)
     os.makedirs(base + "Solve")
@@ -115,6 +128,6 @@ def test_brick_drop_in_executables(brick_vtk, gold, tmp_path):
     assert ks[0] == 0 and len(ks) >= 3
     last = base + f"Solve/Solve{ks[-1]}"
     W = np.column_stack([_read(last + "density.bin", 1), _read(last + "velocity.bin", 3), _read(last + "pressure.bin", 1)])
-    assert sha(W) == gold["W_sha256"]
+    assert sha(W) == expect["W_sha256"]
     rho0 = _read(base + "Solve/Solve0density.bin", 1)
     assert (rho0 == gold["W_init"][0]).all()

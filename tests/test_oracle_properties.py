@@ -260,3 +260,47 @@ def test_rad_source_is_the_covariant_four_force():
     # at rest: G = (rho ka Ur, rho (ka + ks) F)
     G = O.rad_source([2.0, 0, 0, 0, 1.0], 0.5, [0.1, -0.2, 0.05], 3.0, 1.5)
     assert np.allclose(G, [2.0 * 3.0 * 0.5, 2 * 4.5 * 0.1, -2 * 4.5 * 0.2, 2 * 4.5 * 0.05], rtol=1e-14)
+
+
+# ------------------------------------------------ Taub-Mathews EOS (closed forms)
+def test_taub_enthalpy_identity_and_sound_speed():
+    """Taub-Mathews: (h - theta)(h - 4 theta) = 1, and c_s^2 = (dp/drho)_s / h along an
+    isentrope (d eps = theta d rho / rho, eps = h - 1 - theta), both through the
+    oracle's own p2c and flux (the signal speed of a gas at rest is c_s)."""
+    rng = np.random.default_rng(11)
+    n = 400
+    rho = 10 ** rng.uniform(-3, 2, n)
+    theta = 10 ** rng.uniform(-5, 2, n)
+    W = np.column_stack([rho, np.zeros((n, 3)), rho * theta])
+    _, U = O.p2c(W, eos=O.EOS_TAUB)
+    h = (U[:, 4] + W[:, 4]) / rho  # E = rho h - p at rest
+    # h - 4 theta ~ 1/(3 theta) cancels for large theta: rounding of h enters ~h^2 amplified
+    assert (np.abs((h - theta) * (h - 4 * theta) - 1.0) <= 1e-14 * (1 + h * h)).all()
+    nS = np.tile([1.0, 0.0, 0.0, 1.0], (n, 1))
+    _, cs = O.flux(U, U, W, W, nS, eos=O.EOS_TAUB)
+    eps = h - 1 - theta
+    drho = 1e-6 * rho
+    eps2 = eps + theta * drho / rho
+    # invert eps(theta) = 1.5 theta + sqrt(2.25 theta^2 + 1) - 1 by Newton
+    th2 = theta.copy()
+    for _ in range(50):
+        f = 1.5 * th2 + np.sqrt(2.25 * th2 * th2 + 1) - 1 - eps2
+        th2 -= f / (1.5 + 2.25 * th2 / np.sqrt(2.25 * th2 * th2 + 1))
+    dpdrho = ((rho + drho) * th2 - rho * theta) / drho
+    np.testing.assert_allclose(cs ** 2, dpdrho / h, rtol=2e-5)
+
+
+def test_taub_c2p_round_trip():
+    rng = np.random.default_rng(12)
+    n = 2000
+    r = 10 ** rng.uniform(-3, 2, n)
+    p = r * 10 ** rng.uniform(-5, 2, n)
+    Wl = np.exp(rng.uniform(0, np.log(30), n))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    W = np.column_stack([r, d * np.sqrt(1 - 1 / Wl**2)[:, None], p])
+    _, U = O.p2c(W, eos=O.EOS_TAUB)
+    W2, codes, _ = O.c2p(U, eos=O.EOS_TAUB)
+    assert (codes == 0).all()
+    np.testing.assert_allclose(W2[:, 4], W[:, 4], rtol=1e-7)
+    np.testing.assert_allclose(W2[:, 0], W[:, 0], rtol=1e-9)
