@@ -253,3 +253,27 @@ def test_mixed_taub_blast_l1():
         assert np.abs(U32[:, k] - U64[:, k]).sum() <= 1e-5 * ref, k
     with pytest.raises(capi.RhllcError):
         capi.Solver(m, precision=capi.PREC_MIXED, guess=capi.GUESS_MLP, eos=T)
+
+
+def test_mixed_radiation_l1():
+    """Mixed step with the radiation source (the compact M1 form of the mixed path)
+    against the FP64 step with the reference's tensor form: per-field L1 <= 1e-5."""
+    m = M.kuhn_box(16, 16, 16, seed=SEED)
+    W0 = M.init_state(m, M.INIT_BLAST)
+    Ur, Fr, ka, ks = M.init_radiation(m.n_cells, seed=SEED)
+    Fr[:50] = 0.0  # the vanishing-flux branch
+    Fr[50:100] *= 3.0 / np.maximum(np.linalg.norm(Fr[50:100], axis=1, keepdims=True), 1e-30) * Ur[50:100, None]  # |F| > U
+    out = {}
+    for prec, guess in ((capi.PREC_FP64, capi.GUESS_REFERENCE), (capi.PREC_MIXED, capi.GUESS_PREVIOUS)):
+        s = capi.Solver(m, cfl=0.4, precision=prec, guess=guess)
+        s.upload_W(W0)
+        s.set_radiation(Ur, Fr, ka, ks)
+        s.run(40)
+        s.sync()
+        out[prec] = (s.download()[1], s.stats())
+        s.close()
+    U64, U32 = out[capi.PREC_FP64][0], out[capi.PREC_MIXED][0]
+    assert out[capi.PREC_MIXED][1]["fix_fail"] == 0
+    for k in range(5):
+        ref = np.abs(U64[:, k]).sum()
+        assert np.abs(U32[:, k] - U64[:, k]).sum() <= 1e-5 * ref, k
