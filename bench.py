@@ -127,20 +127,8 @@ def cpu_baseline(mesh, W0, rad, steps_req):
 
 
 def _cpu_step(st, tau, rad):
-    import ctypes as C
-
-    import oracle as O
-
-    Ur, Fr, ka, ks = (np.ascontiguousarray(a, dtype=np.float32) for a in rad)
-    L = O.lib()
-    L.orc_step_rad.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_double] + [
-        C.c_void_p
-    ] * 4 + [C.c_int, C.c_void_p]
-    L.orc_step_rad.restype = C.c_double
-    return L.orc_step_rad(
-        C.byref(st.ph), st.n_cells, st.cells.ctypes.data, st.n_faces, st.faces.ctypes.data, tau,
-        Ur.ctypes.data, Fr.ctypes.data, ka.ctypes.data, ks.ctypes.data, 0, C.byref(st.stats),
-    )
+    Ur, Fr, ka, ks = rad
+    return st.step_rad(tau, Ur, Fr, ka, ks)
 
 
 def load_traffic(key):

@@ -129,11 +129,29 @@ int rhllc_reset_stats(rhllc_solver *h);
 int rhllc_download_face_flux(rhllc_solver *h, double *F);
 
 /* Radiation fields for the explicit 4-momentum source, applied after the
- * hydro update as radRHD_main.cpp:117-132 does (M1 Eddington tensor, thermal
- * emission off).  Per cell in the caller's numbering, FP32.  NULL Ur removes
- * the source. */
+ * hydro update as radRHD_main.cpp:117-132 does (M1 Eddington tensor).  Per
+ * cell in the caller's numbering, FP32.  NULL Ur removes the source.  With an
+ * opacity table set (below) kappa_a and kappa_s are not used and may be NULL. */
 int rhllc_set_radiation(rhllc_solver *h, const float *Ur, const float *Fr3, const float *kappa_a,
                         const float *kappa_s);
+
+/* The thermal emission term of GetRadSource (radRHD_utils.cpp:89-90),
+ * q = -kappa_a rho B_Plank(T) / kRadiation with T = GetTemperature(rho, p)
+ * (lib/physics/include/gas_state.h:22-27) in the reference's CGS unit system
+ * (lib/global_consts.h).  Off by default: with code-unit states of order one
+ * (T ~ 1e13 K) the term dwarfs every other. */
+int rhllc_set_radiation_emission(rhllc_solver *h, int32_t on);
+
+/* Opacities from the reference's cooling-function table (TableFunc over
+ * ln D x ln T, tab_func/rcf_H.dat; read with rhllc_table_read of
+ * include/rhllc_mesh.h).  header = {min_x, max_x, step_x, min_y, max_y,
+ * step_y}; data = nx*ny values of ln Lambda at [i*ny + j]; nx >= 1, ny >= 2.
+ * With a table the source is the coupling path of the multi-rank step:
+ * AddRadFlux (rhllc_3d_mpi.cpp:509-533) with GetRadSourceOpt
+ * (radRHD_utils.cpp:103-158) and each cell's full_phys_data_t::Init
+ * (lib/grids/src/solvers_struct.cpp:264-281) of the updated state, thermal
+ * emission included.  NULL data removes the table. */
+int rhllc_set_opacity_table(rhllc_solver *h, int32_t nx, int32_t ny, const double header[6], const double *data);
 
 /* Average device time (ms) of each kernel of the step, measured with HIP
  * events around each launch on the handle's stream over nsteps eager steps:
@@ -156,7 +174,11 @@ int rhllc_eval_c2p(int32_t precision, int32_t eos, double gamma, int64_t n, cons
 int rhllc_eval_p2c(int32_t eos, double gamma, int64_t n, double *W, double *U);
 int rhllc_eval_pressure_fix(int32_t eos, double gamma, int64_t n, double *U, double *W, int32_t *codes);
 int rhllc_eval_rad_source(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
-                          const double *ks, double *G);
+                          const double *ks, int32_t emission, double *G);
+/* GetRadSourceOpt with the table's opacities: G (4 per state) and the cell
+ * data (T, vel, lorenz, alpha, betta: 5 per state). */
+int rhllc_eval_rad_source_opt(int32_t nx, int32_t ny, const double header[6], const double *table, int64_t n,
+                              const double *W, const double *Ur, const double *Fr3, double *G, double *cell);
 
 /* Calibration copy: reps launches of a plain vectorised copy kernel over
  * nbytes (read nbytes + write nbytes per launch); *gbps receives the mean

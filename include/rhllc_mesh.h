@@ -160,6 +160,19 @@ int rhllc_bin_to_geo(int64_t n_cells, const int32_t *neighbors, const double *no
                      const double *volumes, const double *centers, int64_t *n_faces, void *geo_cells,
                      void *geo_faces);
 
+/*
+ * The cooling-function table (tab_func/rcf_H.dat), as
+ * files_sys::txt::ReadTableFunc (lib/files_sys/src/reader_txt.cpp:59-118)
+ * reads it: "... = Nd" and "... = Nt" lines, two header lines, then Nd*Nt
+ * triples (ln D, ln T, ln Lambda).  *nx = Nd, *ny = Nt; header = {min_x,
+ * max_x, step_x, min_y, max_y, step_y} over the ln D and ln T axes, step =
+ * second value - first; data (Nd*Nt doubles at [i*Nt + j]) is filled when not
+ * NULL, so a first call with data = NULL gives the sizes.  Values are those
+ * of the reference's reader bit for bit.  Fails on a missing file, a bad
+ * count line (Nd or Nt below 2) or fewer than Nd*Nt triples.
+ */
+int rhllc_table_read(const char *path, int32_t *nx, int32_t *ny, double header[6], double *data);
+
 /* Read/write "int32 n, then n records" files (reader_bin.h:113-135,232-252). */
 int64_t rhllc_file_count(const char *path);
 int rhllc_file_read_records(const char *path, size_t rec_bytes, int64_t n, void *out);

@@ -16,6 +16,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import subprocess
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -93,8 +94,16 @@ def lib():
         L.orc_edd_tensor.argtypes = [i32, i32, dbl, vp]
         L.orc_edd_tensor.restype = dbl
         L.orc_rad_source.argtypes = [vp, dbl, vp, dbl, dbl, i32, vp]
-        L.orc_run_rad.argtypes = [vp, i64, vp, i64, vp, dbl, dbl, i32, vp, vp, vp, vp, i32, vp, vp, vp]
+        L.orc_run_rad.argtypes = [vp, i64, vp, i64, vp, dbl, dbl, i32, vp, vp, vp, vp, i32, vp, vp, vp, vp]
         L.orc_run_rad.restype = dbl
+        L.orc_step_rad.argtypes = [vp, i64, vp, i64, vp, dbl, vp, vp, vp, vp, i32, vp, vp]
+        L.orc_step_rad.restype = dbl
+        L.orc_table_read.argtypes = [C.c_char_p, vp, vp, vp, vp]
+        L.orc_table_read.restype = i32
+        L.orc_table_lookup.argtypes = [vp, dbl, dbl]
+        L.orc_table_lookup.restype = dbl
+        L.orc_cell_rad.argtypes = [vp, vp, vp]
+        L.orc_rad_source_opt.argtypes = [vp, vp, dbl, vp, vp]
         L.orc_num_threads.restype = i32
         _orc = L
     return _orc
@@ -127,6 +136,9 @@ def ref_lib(eos=EOS_IDEAL):
         L.ref_convert_bin_to_geo.argtypes = [i64, vp, vp, vp, vp, vp, i64, vp, vp]
         L.ref_convert_bin_to_geo.restype = i64
         L.ref_cone_states.argtypes = [vp, vp, vp]
+        L.ref_table_read.argtypes = [C.c_char_p, vp, vp, vp, vp]
+        L.ref_table_read.restype = i32
+        L.ref_table_lookup.argtypes = [i32, i32, vp, vp, i64, vp, vp, vp]
         L.ref_set_log_dir(("/tmp/").encode())
         _ref[eos] = L
     return _ref[eos]
@@ -227,16 +239,31 @@ class State:
         )
         return t, taus, speeds
 
-    def run_rad(self, cfl, h_min, steps, Ur, Fr, ka, ks, emission=0):
+    def run_rad(self, cfl, h_min, steps, Ur, Fr, ka=None, ks=None, emission=0, table=None):
+        """Coupled steps; with `table` (a Table) the opacities come from each cell's state."""
         taus = np.zeros(steps)
         speeds = np.zeros(steps)
+        if ka is None:
+            ka = ks = np.zeros(self.n_cells, np.float32)
         Ur, Fr, ka, ks = (np.ascontiguousarray(a, dtype=np.float32) for a in (Ur, Fr, ka, ks))
+        tab = None if table is None else C.byref(table.c())
         t = lib().orc_run_rad(
             C.byref(self.ph), self.n_cells, self.cells.ctypes.data, self.n_faces, self.faces.ctypes.data,
             cfl, h_min, steps, Ur.ctypes.data, Fr.ctypes.data, ka.ctypes.data, ks.ctypes.data, emission,
-            taus.ctypes.data, speeds.ctypes.data, C.byref(self.stats),
+            tab, taus.ctypes.data, speeds.ctypes.data, C.byref(self.stats),
         )
         return t, taus, speeds
+
+    def step_rad(self, tau, Ur, Fr, ka=None, ks=None, emission=0, table=None):
+        """One coupled step (hydro, source, c2p); returns the step's max signal speed."""
+        if ka is None:
+            ka = ks = np.zeros(self.n_cells, np.float32)
+        Ur, Fr, ka, ks = (np.ascontiguousarray(a, dtype=np.float32) for a in (Ur, Fr, ka, ks))
+        tab = None if table is None else C.byref(table.c())
+        return lib().orc_step_rad(
+            C.byref(self.ph), self.n_cells, self.cells.ctypes.data, self.n_faces, self.faces.ctypes.data, tau,
+            Ur.ctypes.data, Fr.ctypes.data, ka.ctypes.data, ks.ctypes.data, emission, tab, C.byref(self.stats),
+        )
 
     def step(self, tau):
         return lib().orc_step(
@@ -250,6 +277,62 @@ def rad_source(W, Ur, Fr, ka, ks, emission=0):
     Fr = np.ascontiguousarray(Fr, dtype=np.float64).reshape(3)
     G = np.zeros(4)
     lib().orc_rad_source(W.ctypes.data, float(Ur), Fr.ctypes.data, float(ka), float(ks), emission, G.ctypes.data)
+    return G
+
+
+class _CTable(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32)] + [
+        (k, C.c_double) for k in ("min_x", "max_x", "step_x", "min_y", "max_y", "step_y")
+    ] + [("data", C.c_void_p)]
+
+
+@dataclass
+class Table:
+    """TableFunc (ln Lambda over ln D x ln T): hdr = (min_x, max_x, step_x, min_y, max_y, step_y)."""
+
+    hdr: np.ndarray
+    data: np.ndarray  # (nx, ny) float64
+
+    def c(self):
+        self.data = np.ascontiguousarray(self.data, dtype=np.float64)
+        nx, ny = self.data.shape
+        self._c = _CTable(nx, ny, *[float(v) for v in self.hdr], self.data.ctypes.data)
+        return self._c
+
+
+def table_read(path):
+    """ReadTableFunc restated (orc_table_read)."""
+    nx, ny = C.c_int32(), C.c_int32()
+    hdr = np.zeros(6)
+    if lib().orc_table_read(str(path).encode(), C.byref(nx), C.byref(ny), hdr.ctypes.data, None):
+        raise ValueError(f"cannot read table {path}")
+    data = np.zeros((nx.value, ny.value))
+    lib().orc_table_read(str(path).encode(), C.byref(nx), C.byref(ny), hdr.ctypes.data, data.ctypes.data)
+    return Table(hdr, data)
+
+
+def table_lookup(t, x, y):
+    ct = t.c()
+    x, y = np.broadcast_arrays(np.asarray(x, np.float64), np.asarray(y, np.float64))
+    return np.array([lib().orc_table_lookup(C.byref(ct), float(a), float(b)) for a, b in zip(x.ravel(), y.ravel())])
+
+
+def cell_rad(t, W):
+    """full_phys_data_t::Init per state: (T, vel, lorenz, alpha, betta), shape (n, 5)."""
+    W = np.ascontiguousarray(np.atleast_2d(W), dtype=np.float64)
+    ct = t.c()
+    out = np.zeros((len(W), 5))
+    for i in range(len(W)):
+        lib().orc_cell_rad(C.byref(ct), W[i].ctypes.data, out[i].ctypes.data)
+    return out
+
+
+def rad_source_opt(W, cell, Ur, Fr):
+    W = np.ascontiguousarray(W, dtype=np.float64).reshape(5)
+    cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(5)
+    Fr = np.ascontiguousarray(Fr, dtype=np.float64).reshape(3)
+    G = np.zeros(4)
+    lib().orc_rad_source_opt(W.ctypes.data, cell.ctypes.data, float(Ur), Fr.ctypes.data, G.ctypes.data)
     return G
 
 
@@ -285,6 +368,29 @@ def ref_cone_states():
     W0, Ws, Us = np.zeros(5), np.zeros(5), np.zeros(5)
     ref_lib().ref_cone_states(W0.ctypes.data, Ws.ctypes.data, Us.ctypes.data)
     return W0, Ws, Us
+
+
+def ref_table_read(path):
+    """files_sys::txt::ReadTableFunc of the compiled reference."""
+    nx, ny = C.c_int32(), C.c_int32()
+    hdr = np.zeros(6)
+    if ref_lib().ref_table_read(str(path).encode(), C.byref(nx), C.byref(ny), hdr.ctypes.data, None):
+        raise ValueError(f"cannot read table {path}")
+    data = np.zeros((nx.value, ny.value))
+    ref_lib().ref_table_read(str(path).encode(), C.byref(nx), C.byref(ny), hdr.ctypes.data, data.ctypes.data)
+    return Table(hdr, data)
+
+
+def ref_table_lookup(t, x, y):
+    """TableFunc::operator() of the compiled reference at the points (x, y)."""
+    data = np.ascontiguousarray(t.data, dtype=np.float64)
+    hdr = np.ascontiguousarray(t.hdr, dtype=np.float64)
+    x, y = np.broadcast_arrays(np.asarray(x, np.float64), np.asarray(y, np.float64))
+    x, y = np.ascontiguousarray(x.ravel()), np.ascontiguousarray(y.ravel())
+    out = np.zeros(len(x))
+    ref_lib().ref_table_lookup(data.shape[0], data.shape[1], hdr.ctypes.data, data.ctypes.data, len(x),
+                               x.ctypes.data, y.ctypes.data, out.ctypes.data)
+    return out
 
 
 def ref_p2c(W, eos=EOS_IDEAL):

@@ -5,7 +5,8 @@
 // (solvers/rhllc/src/rhllc_flux_stab.cpp, rhllc_3d_stab.cpp, rhllc_utils.cpp,
 // lib/geometry/src/linear_alg.cpp, lib/grids/src/solvers_struct.cpp,
 // lib/json/src/json_struct.cpp, solvers/rhllc/src/rhllc_flux.cpp,
-// lib/geometry/src/convert_bin_to_geo.cpp) into
+// lib/geometry/src/convert_bin_to_geo.cpp, lib/files_sys/src/reader_txt.cpp
+// and the serial lib/mpi_extension/src/*.cpp it calls) into
 // oracle/_ref/libref_rhllc.so.  Tests use it to pin the C restatement in
 // oracle/rhllc_oracle.c; it is never part of the product.
 //
@@ -27,6 +28,7 @@
 #include "rhllc_flux_stab.h"
 #include "rhllc_utils.h"
 #include "linear_alg.h"
+#include "reader_txt.h"
 #include "solvers_struct.h"
 
 extern "C" {
@@ -179,6 +181,28 @@ void ref_cone_states(double *W_init, double *W_src, double *U_src) {
     dst[k][0] = src[k]->d, dst[k][1] = src[k]->v[0], dst[k][2] = src[k]->v[1], dst[k][3] = src[k]->v[2];
     dst[k][4] = src[k]->p;
   }
+}
+
+// files_sys::txt::ReadTableFunc (lib/files_sys/src/reader_txt.cpp:59-118):
+// hdr = {min_x, max_x, step_x, min_y, max_y, step_y}; data (nx*ny) when not NULL.
+int ref_table_read(const char *path, int32_t *nx, int32_t *ny, double *hdr, double *data) {
+  TableFunc t;
+  if (files_sys::txt::ReadTableFunc(path, t) != e_completion_success) return 1;
+  *nx = t.Nx, *ny = t.Ny;
+  hdr[0] = t.min_x, hdr[1] = t.max_x, hdr[2] = t.step_x;
+  hdr[3] = t.min_y, hdr[4] = t.max_y, hdr[5] = t.step_y;
+  if (data) std::memcpy(data, t.data.data(), sizeof(double) * t.data.size());
+  return 0;
+}
+
+// TableFunc::operator() (lib/grids/src/solvers_struct.cpp:10-58) at n points.
+void ref_table_lookup(int32_t nx, int32_t ny, const double *hdr, const double *data, int64_t n, const double *x,
+                      const double *y, double *out) {
+  TableFunc t(nx, ny);
+  t.min_x = hdr[0], t.max_x = hdr[1], t.step_x = hdr[2];
+  t.min_y = hdr[3], t.max_y = hdr[4], t.step_y = hdr[5];
+  std::memcpy(t.data.data(), data, sizeof(double) * nx * ny);
+  for (int64_t i = 0; i < n; ++i) out[i] = t(x[i], y[i]);
 }
 
 }  // extern "C"

@@ -19,8 +19,15 @@
  *                                            + consistency F(U,U) and mirror symmetry
  *   orc_step / orc_run Hllc3dStab + loop  -- pinned: compiled reference trajectories (200 steps)
  *                                            + free-stream preservation, periodic conservation
- *   orc_edd_tensor / orc_rad_source       -- pinned: compiled reference (RAD_RHD build) vectors
- *                                            when available; closed forms (isotropic limit)
+ *   orc_edd_tensor / orc_rad_source       -- pinned: closed form (the covariant four-force with
+ *                                            Levermore's M1 tensor; emission = q u^mu); the
+ *                                            reference's radRHD_utils.cpp does not compile as shipped
+ *   orc_table_read / orc_table_lookup     -- pinned: compiled reference (ReadTableFunc,
+ *                                            TableFunc::operator(); tests/golden/table.npz)
+ *                                            + closed forms on a synthetic table
+ *   orc_cell_rad / orc_rad_source_opt     -- pinned: closed forms (ideal-gas T, Lorentz factor,
+ *                                            alpha = Lambda kDist / (4 sigma T^4), Thomson betta)
+ *                                            and equality with orc_rad_source at those opacities
  *
  * Summation order of Eigen's 3-vector expressions (the reference uses
  * Eigen 3.4 fixed-size types, vectorised with 2-double packets on x86-64):
@@ -31,7 +38,10 @@
  */
 #include "rhllc_oracle.h"
 
+#include <float.h>
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
@@ -592,17 +602,155 @@ void orc_rad_source(const orc_flux_t *W, double Ur, const double Fr[3], double a
                rhogamma * (tot_op * Fr[i] + scat_op * 2.0 * uF * u[i]);
 }
 
+/* lib/files_sys/src/reader_txt.cpp:59-118.  Two lines "... = N" (Nd, then
+ * Nt), two more lines skipped, then Nd*Nt triples (ln D, ln T, ln Lambda)
+ * read as whitespace-separated numbers; the ln D of each row's first triple
+ * and the ln T column of the last row give the axes; step = a[1] - a[0] and
+ * min/max over each axis. */
+static int header_count(FILE *f, int32_t *n) {
+  char line[4096];
+  if (!fgets(line, sizeof line, f)) return 1;
+  const char *eq = strchr(line, '=');
+  if (!eq) return 1;
+  char *end = NULL;
+  const long v = strtol(eq + 1, &end, 10);
+  if (end == eq + 1 || v < 2 || v > (1 << 24)) return 1;
+  *n = (int32_t)v;
+  return 0;
+}
+
+int orc_table_read(const char *path, int32_t *nx, int32_t *ny, double hdr[6], double *data) {
+  FILE *f = fopen(path, "r");
+  if (!f) return 1;
+  int32_t nd = 0, nt = 0;
+  char line[4096];
+  int bad = header_count(f, &nd) || header_count(f, &nt) || !fgets(line, sizeof line, f) ||
+            !fgets(line, sizeof line, f);
+  double *dens = bad ? NULL : (double *)malloc(sizeof(double) * nd);
+  double *temp = bad ? NULL : (double *)malloc(sizeof(double) * nt);
+  if (!bad && (!dens || !temp)) bad = 1;
+  for (int32_t i = 0; !bad && i < nd; ++i)
+    for (int32_t j = 0; !bad && j < nt; ++j) {
+      double a, b, c;
+      if (fscanf(f, "%lf %lf %lf", &a, &b, &c) != 3) {
+        bad = 1;
+        break;
+      }
+      if (j == 0) dens[i] = a;
+      temp[j] = b;
+      if (data) data[(int64_t)i * nt + j] = c;
+    }
+  fclose(f);
+  if (!bad) {
+    *nx = nd, *ny = nt;
+    hdr[2] = dens[1] - dens[0];
+    hdr[5] = temp[1] - temp[0];
+    hdr[1] = -DBL_MAX, hdr[0] = DBL_MAX;
+    for (int32_t i = 0; i < nd; ++i) hdr[1] = fmax(hdr[1], dens[i]), hdr[0] = fmin(hdr[0], dens[i]);
+    hdr[4] = -DBL_MAX, hdr[3] = DBL_MAX;
+    for (int32_t j = 0; j < nt; ++j) hdr[4] = fmax(hdr[4], temp[j]), hdr[3] = fmin(hdr[3], temp[j]);
+  }
+  free(dens);
+  free(temp);
+  return bad;
+}
+
+/* lib/grids/src/solvers_struct.cpp:10-58.  Indices below 0 (round(min) < min)
+ * are clamped to 0 here; the reference reads before the row there. */
+double orc_table_lookup(const orc_table_t *t, double x, double y) {
+  if (x < t->min_x) x = t->min_x;
+  if (y < t->min_y) y = t->min_y;
+  if (x > t->max_x) x = t->max_x;
+  int i = (int)((round(x) - t->min_x) / t->step_x);
+  if (i > t->nx - 1) i = t->nx - 1;
+  if (i < 0) i = 0;
+  if (y > t->max_y) {
+    const double y1 = exp(t->data[(int64_t)t->ny * i + (t->ny - 1)]);
+    const double y0 = exp(t->data[(int64_t)t->ny * i + (t->ny - 2)]);
+    const double x1 = exp(t->max_y);
+    const double x0 = exp(t->max_y - t->step_y);
+    const double k = (y0 - y1) / (x0 - x1);
+    const double m = (-x1 * y0 + x0 * y1) / (x0 - x1);
+    return log(k * exp(y) + m);
+  }
+  int j = (int)((round(y) - t->min_y) / t->step_y);
+  if (j > t->ny - 1) j = t->ny - 1;
+  if (j < 0) j = 0;
+  return t->data[(int64_t)t->ny * i + j];
+}
+
+/* LOG(kDensity), LOG(kStefanBoltzmann4), LOG(kDist): the values that the
+ * reference's header generator writes (resources/parser_header.cpp, 16
+ * significant digits) into log_global_consts.h. */
+#define LOG_kDensity 4.904385174253596
+#define LOG_kStefanBoltzmann4 -8.391375953193897
+#define LOG_kDist 29.93360620892259
+static const double kSigma_thomson = 6.65210 * 1e-25;
+
+/* lib/grids/src/solvers_struct.cpp:264-281 */
+void orc_cell_rad(const orc_table_t *t, const orc_flux_t *W, double cell[5]) {
+  const double T = temperature(W->d, W->p);
+  const double logT = log(T);
+  const double vel2 = dot3(W->v, W->v);
+  const double vel = sqrt(vel2);
+  const double lorenz = 1. / sqrt(1. - vel2);
+  const double L = orc_table_lookup(t, log(W->d) + LOG_kDensity, logT);
+  const double log_alpha = L - (LOG_kStefanBoltzmann4 + 4 * logT) + LOG_kDist;
+  cell[0] = T;
+  cell[1] = vel;
+  cell[2] = lorenz;
+  cell[3] = exp(log_alpha);
+  cell[4] = (kSigma_thomson / kM_hydrogen * kDist) * W->d * kDensity;
+}
+
+/* radRHD_utils.cpp:103-158 */
+void orc_rad_source_opt(const orc_flux_t *W, const double cell[5], double Ur, const double Fr[3], double G[4]) {
+  const double abs_op = cell[3], scat_op = cell[4];
+  const double tot_op = abs_op + scat_op;
+  const double rho = W->d;
+  const double *v = W->v;
+  const double gamma = cell[2];
+  const double gamma2 = gamma * gamma;
+  const double u2 = gamma2 * cell[1] * cell[1];
+  double u[3] = {gamma * v[0], gamma * v[1], gamma * v[2]};
+  double uuD = 0.;
+  const double uF = dot3(u, Fr);
+  double uD[3];
+  for (int i = 0; i < 3; i++) {
+    uD[i] = 0.;
+    for (int j = 0; j < 3; j++) {
+      const double D = orc_edd_tensor(j, i, Ur, Fr);
+      uD[i] += u[j] * D;
+      uuD += u[i] * u[j] * D;
+    }
+  }
+  const double rhogamma = rho * gamma;
+  const double q = -abs_op * rho * b_planck(cell[0]) / kRadiation;
+  G[0] = q * gamma + rhogamma * (abs_op - scat_op * (u2 + uuD)) * Ur - rho * (abs_op - scat_op * (u2 + gamma2)) * uF;
+  for (int i = 0; i < 3; i++)
+    G[i + 1] = q * u[i] - rho * (tot_op * uD[i] + scat_op * u[i] * (gamma2 + uuD)) * Ur +
+               rhogamma * (tot_op * Fr[i] + scat_op * 2.0 * uF * u[i]);
+}
+
 /* radRHD_main.cpp:99-132 (serial coupled loop, minus the transfer solve):
- * Hllc3dStab, then U += (ds*tau)*G, then GetPhysValueStab (code ignored). */
+ * Hllc3dStab, then U += (ds*tau)*G, then GetPhysValueStab (code ignored).
+ * With a table: radRHD_mpi_main.cpp:112-122, AddRadFlux then HllcConvToPhys. */
 double orc_step_rad(const orc_physics_t *ph, int64_t n_cells, orc_elem_t *cells, int64_t n_faces, orc_face_t *faces,
                     double tau, const float *Ur, const float *Fr3, const float *ka, const float *ks, int emission,
-                    orc_stats_t *st) {
+                    const orc_table_t *tab, orc_stats_t *st) {
   const double speed = orc_step(ph, n_cells, cells, n_faces, faces, tau, st);
-#pragma omp parallel for schedule(static)
+  int64_t fix_fail = 0;
+#pragma omp parallel for schedule(static) reduction(+ : fix_fail)
   for (int64_t c = 0; c < n_cells; ++c) {
     double G[4];
     const double Fr[3] = {Fr3[3 * c], Fr3[3 * c + 1], Fr3[3 * c + 2]};
-    orc_rad_source(&cells[c].phys_val, (double)Ur[c], Fr, (double)ka[c], (double)ks[c], emission, G);
+    if (tab) {
+      double cell[5];
+      orc_cell_rad(tab, &cells[c].phys_val, cell);
+      orc_rad_source_opt(&cells[c].phys_val, cell, (double)Ur[c], Fr, G);
+    } else {
+      orc_rad_source(&cells[c].phys_val, (double)Ur[c], Fr, (double)ka[c], (double)ks[c], emission, G);
+    }
     const double ds = 1;
     orc_flux_t *U = &cells[c].conv_val;
     U->p += ds * tau * G[0];
@@ -610,20 +758,23 @@ double orc_step_rad(const orc_physics_t *ph, int64_t n_cells, orc_elem_t *cells,
     U->v[1] += ds * tau * G[2];
     U->v[2] += ds * tau * G[3];
     int it = 0;
-    orc_c2p(ph, U, &cells[c].phys_val, &it);
+    if (orc_c2p(ph, U, &cells[c].phys_val, &it) && tab)
+      if (orc_pressure_fix(ph, U, &cells[c].phys_val)) fix_fail += 1;
   }
+  st->fix_fail += fix_fail;
   return speed;
 }
 
 double orc_run_rad(const orc_physics_t *ph, int64_t n_cells, orc_elem_t *cells, int64_t n_faces, orc_face_t *faces,
                    double cfl, double h_min, int steps, const float *Ur, const float *Fr3, const float *ka,
-                   const float *ks, int emission, double *taus, double *speeds, orc_stats_t *st) {
+                   const float *ks, int emission, const orc_table_t *tab, double *taus, double *speeds,
+                   orc_stats_t *st) {
   phys_to_conv(ph, n_cells, cells);
   double speed = 1;
   double tau = cfl * h_min / speed;
   double t = 0;
   for (int k = 0; k < steps; ++k) {
-    speed = orc_step_rad(ph, n_cells, cells, n_faces, faces, tau, Ur, Fr3, ka, ks, emission, st);
+    speed = orc_step_rad(ph, n_cells, cells, n_faces, faces, tau, Ur, Fr3, ka, ks, emission, tab, st);
     if (taus) taus[k] = tau;
     if (speeds) speeds[k] = speed;
     t += tau;

@@ -112,14 +112,37 @@ void orc_load(int64_t n_cells, const void *geo_cells, const double *W, int64_t n
 double orc_edd_tensor(int i, int j, double U, const double F[3]);
 void orc_rad_source(const orc_flux_t *W, double Ur, const double Fr[3], double ka, double ks, int emission,
                     double G[4]);
+/* TableFunc (lib/grids/include/solvers_struct.h:24-44): ln Lambda over
+ * (ln D, ln T), data[i*ny + j]; ReadTableFunc (lib/files_sys/src/reader_txt.cpp:59-118)
+ * fills it from tab_func/rcf_H.dat. */
+typedef struct {
+  int32_t nx, ny;
+  double min_x, max_x, step_x, min_y, max_y, step_y;
+  const double *data;
+} orc_table_t;
+/* ReadTableFunc: returns 0 and fills *nx, *ny, hdr = {min_x, max_x, step_x,
+ * min_y, max_y, step_y} and data (when not NULL, nx*ny values); nonzero on a
+ * missing or malformed file. */
+int orc_table_read(const char *path, int32_t *nx, int32_t *ny, double hdr[6], double *data);
+/* TableFunc::operator() (lib/grids/src/solvers_struct.cpp:10-58). */
+double orc_table_lookup(const orc_table_t *t, double x, double y);
+/* full_phys_data_t::Init(const flux_t *) (lib/grids/src/solvers_struct.cpp:264-281):
+ * cell[5] = (T, vel, lorenz, alpha, betta). */
+void orc_cell_rad(const orc_table_t *t, const orc_flux_t *W, double cell[5]);
+/* GetRadSourceOpt (radRHD_utils.cpp:103-158) with the M1 Eddington tensor. */
+void orc_rad_source_opt(const orc_flux_t *W, const double cell[5], double Ur, const double Fr[3], double G[4]);
 /* Coupled step of radRHD_main.cpp:99-132: hydro step, then U += tau*G(W),
- * then c2p (return code ignored, as there).  Ur..ks are per cell. */
+ * then c2p (return code ignored, as there).  Ur..ks are per cell.  With a
+ * table (tab != NULL; ka and ks unused) the source step is AddRadFlux
+ * (rhllc_3d_mpi.cpp:509-533: GetRadSourceOpt of each cell's Init) followed by
+ * HllcConvToPhys (rhllc_3d_mpi.cpp:488-505: c2p, PhysPressureFix on failure). */
 double orc_step_rad(const orc_physics_t *ph, int64_t n_cells, orc_elem_t *cells, int64_t n_faces,
                     orc_face_t *faces, double tau, const float *Ur, const float *Fr3, const float *ka,
-                    const float *ks, int emission, orc_stats_t *st);
+                    const float *ks, int emission, const orc_table_t *tab, orc_stats_t *st);
 double orc_run_rad(const orc_physics_t *ph, int64_t n_cells, orc_elem_t *cells, int64_t n_faces,
                    orc_face_t *faces, double cfl, double h_min, int steps, const float *Ur, const float *Fr3,
-                   const float *ka, const float *ks, int emission, double *taus, double *speeds, orc_stats_t *st);
+                   const float *ka, const float *ks, int emission, const orc_table_t *tab, double *taus,
+                   double *speeds, orc_stats_t *st);
 
 int orc_num_threads(void);
 

@@ -48,6 +48,7 @@ def load_mesh_lib() -> C.CDLL:
         lib.rhllc_vtk_close.restype = None
         lib.rhllc_tet_geometry.argtypes = [i64, vp, i64, vp, i32, vp, vp, vp, vp, vp]
         lib.rhllc_bin_to_geo.argtypes = [i64, vp, vp, vp, vp, vp, vp, vp, vp]
+        lib.rhllc_table_read.argtypes = [C.c_char_p, vp, vp, vp, vp]
         lib.rhllc_mesh_last_error.restype = C.c_char_p
         _mesh = lib
     return _mesh

@@ -17,7 +17,7 @@ from ._lib import HIP_LIB, MESH_LIB, PKG_DIR
 CSRC = os.path.join(PKG_DIR, "csrc")
 MESH_SRC = [
     os.path.join(CSRC, "mesh", f)
-    for f in ("kuhn_box.cpp", "mesh_tools.cpp", "morton.cpp", "vtk_legacy.cpp", "tet_geometry.cpp")
+    for f in ("kuhn_box.cpp", "mesh_tools.cpp", "morton.cpp", "vtk_legacy.cpp", "tet_geometry.cpp", "table_txt.cpp")
 ]
 HIP_SRC = [os.path.join(CSRC, "hip", f) for f in ("rhllc_kernels.hip", "rhllc_capi.hip", "mlp_kernels.hip", "mlp_capi.hip")]
 HIP_FLAGS = [

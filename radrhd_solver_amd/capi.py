@@ -95,13 +95,16 @@ SIGNATURES = {
     "rhllc_reset_stats": (C.c_int, [_vp]),
     "rhllc_download_face_flux": (C.c_int, [_vp, _vp]),
     "rhllc_set_radiation": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "rhllc_set_radiation_emission": (C.c_int, [_vp, _i32]),
+    "rhllc_set_opacity_table": (C.c_int, [_vp, _i32, _i32, _vp, _vp]),
     "rhllc_time_kernels": (C.c_int, [_vp, _i32, _vp]),
     "rhllc_info": (C.c_int, [_vp, _vp, _vp, _vp]),
     "rhllc_eval_flux": (C.c_int, [_i32, _i32, _dbl, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_eval_c2p": (C.c_int, [_i32, _i32, _dbl, _i64, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_eval_p2c": (C.c_int, [_i32, _dbl, _i64, _vp, _vp]),
     "rhllc_eval_pressure_fix": (C.c_int, [_i32, _dbl, _i64, _vp, _vp, _vp]),
-    "rhllc_eval_rad_source": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "rhllc_eval_rad_source": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp]),
+    "rhllc_eval_rad_source_opt": (C.c_int, [_i32, _i32, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_device_query": (C.c_int, [_i32, C.c_char_p, _i32, _vp, _vp]),
     "rhllc_copy_probe": (C.c_int, [_i32, _i64, _i32, _vp]),
     "rhllc_mlp_train": (C.c_int, [_vp, _vp, _vp]),
@@ -217,13 +220,28 @@ class Solver:
         check(_lib().rhllc_download_face_flux(self.h, F.ctypes.data))
         return F
 
-    def set_radiation(self, Ur, Fr, ka, ks):
+    def set_radiation(self, Ur, Fr, ka=None, ks=None):
+        """Per-cell radiation moments (and opacities, which an opacity table replaces)."""
         if Ur is None:
             check(_lib().rhllc_set_radiation(self.h, None, None, None, None))
             return
-        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (Ur, Fr, ka, ks)]
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float32) for a in (Ur, Fr, ka, ks)]
         self._rad = arrs
-        check(_lib().rhllc_set_radiation(self.h, *[a.ctypes.data for a in arrs]))
+        check(_lib().rhllc_set_radiation(self.h, *[None if a is None else a.ctypes.data for a in arrs]))
+
+    def set_radiation_emission(self, on):
+        check(_lib().rhllc_set_radiation_emission(self.h, 1 if on else 0))
+
+    def set_opacity_table(self, table):
+        """table: (header[6], data[nx, ny]) as read_table returns, or None to remove it."""
+        if table is None:
+            check(_lib().rhllc_set_opacity_table(self.h, 0, 0, None, None))
+            return
+        hdr, data = table
+        hdr = np.ascontiguousarray(hdr, dtype=np.float64).reshape(6)
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        check(_lib().rhllc_set_opacity_table(self.h, data.shape[0], data.shape[1], hdr.ctypes.data,
+                                             data.ctypes.data))
 
     def set_mlp(self, model):
         model = np.ascontiguousarray(model, dtype=np.float32)
@@ -280,15 +298,32 @@ def eval_pressure_fix(U, W, gamma=5.0 / 3.0, eos=EOS_IDEAL):
     return U, W, codes
 
 
-def eval_rad_source(W, Ur, Fr, ka, ks):
+def eval_rad_source(W, Ur, Fr, ka, ks, emission=False):
     W = np.ascontiguousarray(W, dtype=np.float64).reshape(-1, 5)
     n = W.shape[0]
     Ur, ka, ks = (np.ascontiguousarray(x, dtype=np.float64).reshape(n) for x in (Ur, ka, ks))
     Fr = np.ascontiguousarray(Fr, dtype=np.float64).reshape(n, 3)
     G = np.zeros((n, 4))
     check(_lib().rhllc_eval_rad_source(n, W.ctypes.data, Ur.ctypes.data, Fr.ctypes.data, ka.ctypes.data,
-                                       ks.ctypes.data, G.ctypes.data))
+                                       ks.ctypes.data, 1 if emission else 0, G.ctypes.data))
     return G
+
+
+def eval_rad_source_opt(table, W, Ur, Fr):
+    """GetRadSourceOpt with the table's opacities: (G[n, 4], cell[n, 5] = (T, vel, lorenz, alpha, betta))."""
+    hdr, data = table
+    hdr = np.ascontiguousarray(hdr, dtype=np.float64).reshape(6)
+    data = np.ascontiguousarray(data, dtype=np.float64)
+    W = np.ascontiguousarray(W, dtype=np.float64).reshape(-1, 5)
+    n = W.shape[0]
+    Ur = np.ascontiguousarray(Ur, dtype=np.float64).reshape(n)
+    Fr = np.ascontiguousarray(Fr, dtype=np.float64).reshape(n, 3)
+    G = np.zeros((n, 4))
+    cell = np.zeros((n, 5))
+    check(_lib().rhllc_eval_rad_source_opt(data.shape[0], data.shape[1], hdr.ctypes.data, data.ctypes.data, n,
+                                           W.ctypes.data, Ur.ctypes.data, Fr.ctypes.data, G.ctypes.data,
+                                           cell.ctypes.data))
+    return G, cell
 
 
 def mlp_train(steps=20000, batch=65536, lr=2e-3, gamma=5.0 / 3.0, seed=20261202, norm_samples=1 << 20, device=0):

@@ -12,7 +12,7 @@
 //    roundings do not, and at Lorentz factors ~10 c2p amplifies an error in E
 //    by W^2 E/p (~1e3-1e4) into the pressure.  D in FP32 leaves the errors
 //    unchanged, since its rounding reaches D and E alike (pack below);
-//  * the cell record's s = rho h W^2 and D (8-bit counts): they must match
+//  * the cell record's s - D (s = rho h W^2) and D (8-bit counts): they must match
 //    the record's FP32 v and p far better than FP32 rounding does.
 // Host and device use the same functions.
 #pragma once
@@ -62,20 +62,21 @@ struct Packed {
   uint2 b;
 };
 
-// Face flux record.  The energy slot holds the flux of tau = E - D, and
-// unpack returns E as (FP32 D) + tau: the rounding of the D flux then enters
-// D and E alike and leaves E - D alone.  In a cold gas (p / rho ~ 1e-6) tau is
-// a millionth of E, and an E rebuilt from separately rounded D and E fluxes
-// carried the FP32 D rounding into tau amplified ~1e6 (transverse-momentum L1
-// 4e-4 against FP64; 3e-6 with tau).
-__host__ __device__ __forceinline__ Packed pack(double d, double mx, double my, double mz, double E) {
+// Face flux record.  The energy slot holds the flux of tau = E - D (the
+// caller forms it without going through the E flux), and unpack returns E as
+// (FP32 D) + tau: the rounding of the D flux then enters D and E alike and
+// leaves E - D alone.  In a cold gas (p / rho ~ 1e-6) tau is a millionth of
+// E, and an E rebuilt from separately rounded D and E fluxes carried the FP32
+// D rounding into tau amplified ~1e6 (transverse-momentum L1 4e-4 against
+// FP64; 3e-6 with tau).
+__host__ __device__ __forceinline__ Packed pack(double d, double mx, double my, double mz, double tau) {
   Packed o;
   uint32_t e1, e2, e3, e4;
   o.a.x = (float)d;
   o.a.y = split<8>(mx, e1);
   o.a.z = split<8>(my, e2);
   o.a.w = split<8>(mz, e3);
-  o.b.x = f32_bits(split<8>(E - d, e4));
+  o.b.x = f32_bits(split<8>(tau, e4));
   o.b.y = e1 | (e2 << 8) | (e3 << 16) | (e4 << 24);
   return o;
 }

@@ -57,6 +57,7 @@ struct rhllc_solver {
   Scalars *host_sc = nullptr;  // pinned staging of the device scalars
   float4 *rad_FU = nullptr;
   float2 *rad_k = nullptr;
+  double *tab = nullptr;  // opacity table (nx*ny), owned
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_src = nullptr;
   float *mlp_P = nullptr;
@@ -84,6 +85,7 @@ struct rhllc_solver {
     for (void *p : allocs) (void)hipFree(p);
     if (rad_FU) (void)hipFree(rad_FU);
     if (rad_k) (void)hipFree(rad_k);
+    if (tab) (void)hipFree(tab);
     if (mlp_P) (void)hipFree(mlp_P);
     if (mlp_W2b) (void)hipFree(mlp_W2b);
     if (host_sc) (void)hipHostFree(host_sc);
@@ -525,14 +527,16 @@ extern "C" int rhllc_set_radiation(rhllc_solver *h, const float *Ur, const float
     h->rad = false;
     return RHLLC_OK;
   }
-  if (!Fr3 || !kappa_a || !kappa_s) return fail(RHLLC_ERR_ARG, "null radiation array");
+  if (!Fr3) return fail(RHLLC_ERR_ARG, "null radiation array");
+  if (!kappa_a != !kappa_s) return fail(RHLLC_ERR_ARG, "kappa_a and kappa_s must both be given or both be NULL");
+  if (!kappa_a && !h->tab) return fail(RHLLC_ERR_ARG, "no opacities: give kappa_a/kappa_s or set an opacity table");
   const int32_t N = h->n_cells;
   std::vector<float4> fu(N);
   std::vector<float2> kk(N);
   for (int32_t i = 0; i < N; ++i) {
     const int32_t c = h->cell_new_to_old[i];
     fu[i] = make_float4(Fr3[3 * (size_t)c], Fr3[3 * (size_t)c + 1], Fr3[3 * (size_t)c + 2], Ur[c]);
-    kk[i] = make_float2(kappa_a[c], kappa_s[c]);
+    kk[i] = kappa_a ? make_float2(kappa_a[c], kappa_s[c]) : make_float2(0.f, 0.f);
   }
   if (!h->rad_FU) HIP_TRY(hipMalloc((void **)&h->rad_FU, sizeof(float4) * N));
   if (!h->rad_k) HIP_TRY(hipMalloc((void **)&h->rad_k, sizeof(float2) * N));
@@ -541,6 +545,56 @@ extern "C" int rhllc_set_radiation(rhllc_solver *h, const float *Ur, const float
   h->d.rad_FU = h->rad_FU;
   h->d.rad_k = h->rad_k;
   h->rad = true;
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_set_radiation_emission(rhllc_solver *h, int32_t on) {
+  if (!h) return fail(RHLLC_ERR_ARG, "null handle");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->drop_graph();  // the graph holds the kernel arguments by value
+  h->d.rad_emission = on != 0;
+  return RHLLC_OK;
+}
+
+// Table axes as the lookup uses them: finite, min <= max, positive steps.
+static int check_table(int32_t nx, int32_t ny, const double *hdr) {
+  if (nx < 1 || ny < 2 || (int64_t)nx * ny > (int64_t)1 << 26) return fail(RHLLC_ERR_ARG, "bad table size");
+  for (int k = 0; k < 6; ++k)
+    if (!std::isfinite(hdr[k])) return fail(RHLLC_ERR_ARG, "non-finite table header");
+  if (!(hdr[0] <= hdr[1]) || !(hdr[2] > 0) || !(hdr[3] <= hdr[4]) || !(hdr[5] > 0))
+    return fail(RHLLC_ERR_ARG, "table header needs min <= max and step > 0 on both axes");
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_set_opacity_table(rhllc_solver *h, int32_t nx, int32_t ny, const double header[6],
+                                       const double *data) {
+  if (!h) return fail(RHLLC_ERR_ARG, "null handle");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->drop_graph();
+  if (!data) {
+    if (h->tab) HIP_TRY(hipFree(h->tab));
+    h->tab = nullptr;
+    h->d.tab = nullptr;
+    return RHLLC_OK;
+  }
+  if (!header) return fail(RHLLC_ERR_ARG, "null table header");
+  if (int e = check_table(nx, ny, header)) return e;
+  double *t = nullptr;
+  HIP_TRY(hipMalloc((void **)&t, sizeof(double) * nx * ny));
+  const hipError_t e = hipMemcpy(t, data, sizeof(double) * nx * ny, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(t);
+    HIP_TRY(e);
+  }
+  if (h->tab) HIP_TRY(hipFree(h->tab));
+  h->tab = t;
+  Dev &d = h->d;
+  d.tab = t;
+  d.tab_nx = nx, d.tab_ny = ny;
+  d.tab_min_x = header[0], d.tab_max_x = header[1], d.tab_step_x = header[2];
+  d.tab_min_y = header[3], d.tab_max_y = header[4], d.tab_step_y = header[5];
   return RHLLC_OK;
 }
 
@@ -704,7 +758,7 @@ extern "C" int rhllc_eval_pressure_fix(int32_t eos, double gamma, int64_t n, dou
 }
 
 extern "C" int rhllc_eval_rad_source(int64_t n, const double *W, const double *Ur, const double *Fr3,
-                                     const double *ka, const double *ks, double *G) {
+                                     const double *ka, const double *ks, int32_t emission, double *G) {
   if (n < 0 || !W || !Ur || !Fr3 || !ka || !ks || !G) return fail(RHLLC_ERR_ARG, "bad argument");
   DevBuf a, b, c, dd, e, f;
   HIP_TRY(to_dev(a, W, sizeof(double) * 5 * n));
@@ -714,9 +768,29 @@ extern "C" int rhllc_eval_rad_source(int64_t n, const double *W, const double *U
   HIP_TRY(to_dev(e, ks, sizeof(double) * n));
   HIP_TRY(to_dev(f, nullptr, sizeof(double) * 4 * n));
   HIP_TRY(launch_eval_rad(n, (double *)a.p, (double *)b.p, (double *)c.p, (double *)dd.p, (double *)e.p,
-                          (double *)f.p, nullptr));
+                          emission, (double *)f.p, nullptr));
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(G, f.p, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_eval_rad_source_opt(int32_t nx, int32_t ny, const double header[6], const double *table,
+                                         int64_t n, const double *W, const double *Ur, const double *Fr3, double *G,
+                                         double *cell) {
+  if (n < 0 || !header || !table || !W || !Ur || !Fr3 || !G || !cell) return fail(RHLLC_ERR_ARG, "bad argument");
+  if (int e = check_table(nx, ny, header)) return e;
+  DevBuf t, a, b, c, g, o;
+  HIP_TRY(to_dev(t, table, sizeof(double) * nx * ny));
+  HIP_TRY(to_dev(a, W, sizeof(double) * 5 * n));
+  HIP_TRY(to_dev(b, Ur, sizeof(double) * n));
+  HIP_TRY(to_dev(c, Fr3, sizeof(double) * 3 * n));
+  HIP_TRY(to_dev(g, nullptr, sizeof(double) * 4 * n));
+  HIP_TRY(to_dev(o, nullptr, sizeof(double) * 5 * n));
+  HIP_TRY(launch_eval_rad_opt(nx, ny, header, (double *)t.p, n, (double *)a.p, (double *)b.p, (double *)c.p,
+                              (double *)g.p, (double *)o.p, nullptr));
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(G, g.p, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(cell, o.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost));
   return RHLLC_OK;
 }
 

@@ -52,10 +52,16 @@ struct Dev {
   double4 *W4;   // (rho, vx, vy, vz), FP64 path
   double *Wp;    // p (both paths: the FP64 pressure)
   float4 *Wva;   // mixed path: (vx, vy, vz, p) in FP32 (mixed_record)
-  float4 *Wsd;   // mixed path: (s, D, cs2, extension counts of s and D)
+  float4 *Wsd;   // mixed path: (s - D, D, cs2, extension counts of s - D and D), s = rho h W^2
   // radiation source (optional): (Fx, Fy, Fz, Ur) and (kappa_a, kappa_s)
   const float4 *rad_FU;
   const float2 *rad_k;
+  int32_t rad_emission;  // thermal emission term of the source (0: off)
+  // Opacity table (ln Lambda over ln D x ln T, TableFunc of the reference):
+  // when set, each cell's opacities come from its state and rad_k is unused.
+  const double *tab;
+  int32_t tab_nx, tab_ny;
+  double tab_min_x, tab_max_x, tab_step_x, tab_min_y, tab_max_y, tab_step_y;
   // fixed ghost state for boundary code -3
   double ghostW[5], ghostU[5];
   // surrogate c2p guess (RHLLC_GUESS_MLP): FP32 model and bf16 W2
@@ -94,6 +100,9 @@ hipError_t launch_eval_c2p(bool mixed, int eos, double gamma, int64_t n, const d
 hipError_t launch_eval_p2c(int eos, double gamma, int64_t n, double *W, double *U, hipStream_t s);
 hipError_t launch_eval_fix(int eos, double gamma, int64_t n, double *U, double *W, int32_t *codes, hipStream_t s);
 hipError_t launch_eval_rad(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
-                           const double *ks, double *G, hipStream_t s);
+                           const double *ks, int emission, double *G, hipStream_t s);
+// hdr = {min_x, max_x, step_x, min_y, max_y, step_y}; host pointer, read at launch.
+hipError_t launch_eval_rad_opt(int nx, int ny, const double *hdr, const double *table, int64_t n, const double *W,
+                               const double *Ur, const double *Fr3, double *G, double *cell, hipStream_t s);
 
 }  // namespace rhllc_dev

@@ -277,22 +277,40 @@ __device__ __forceinline__ void finish_cell(const Dev &d, int c, bool live, Stat
     }
     if (RAD) {
       // radRHD_main.cpp:117-132: U += (ds*tau)*G(W), ds = 1, then c2p (code ignored).
+      // With the opacity table: AddRadFlux (rhllc_3d_mpi.cpp:509-533) with
+      // GetRadSourceOpt and the cell's full_phys_data_t of the updated W.
       const float4 fu = d.rad_FU[c];
-      const float2 kk = d.rad_k[c];
       const double Fr[3] = {(double)fu.x, (double)fu.y, (double)fu.z};
       double G[4];
-      if (MIXED)
-        rad_source<Fast>(W, (double)fu.w, Fr, (double)kk.x, (double)kk.y, G);
-      else
-        rad_source(W, (double)fu.w, Fr, (double)kk.x, (double)kk.y, G);
+      if (d.tab) {
+        const OpacityTable t{d.tab,       d.tab_nx,    d.tab_ny,     d.tab_min_x, d.tab_max_x,
+                             d.tab_step_x, d.tab_min_y, d.tab_max_y, d.tab_step_y};
+        rad_source_opt<A>(W, cell_rad<A>(t, W), (double)fu.w, Fr, G);
+      } else {
+        const float2 kk = d.rad_k[c];
+        rad_source<A>(W, (double)fu.w, Fr, (double)kk.x, (double)kk.y, G, d.rad_emission != 0);
+      }
       const double ds = 1;
       U.p += ds * tau * G[0];
       U.v[0] += ds * tau * G[1];
       U.v[1] += ds * tau * G[2];
       U.v[2] += ds * tau * G[3];
       int it3 = 0;
-      c2p<A, EOS>(d.Gg, U, W, MIXED ? W.p : -1.0, it3);
+      int code2 = c2p<A, EOS>(d.Gg, U, W, MIXED ? W.p : -1.0, it3);
       iters += it3;
+      if (d.tab && code2) {
+        // HllcConvToPhys (rhllc_3d_mpi.cpp:488-505): reference start, then
+        // PhysPressureFix of the state that c2p left (the pre-source W).
+        if (MIXED) {
+          it3 = 0;
+          code2 = c2p<A, EOS>(d.Gg, U, W, -1.0, it3);
+          iters += it3;
+        }
+        if (code2) {
+          fixed = true;
+          fix_failed = fix_failed || pressure_fix<EOS>(d.Gg, U, W) != 0;
+        }
+      }
     }
     if (STATS) resid = c2p_residual<EOS>(d.Gg, U, W.p);
     d.U4[c] = make_double4(U.d, U.v[0], U.v[1], U.v[2]);
@@ -672,12 +690,24 @@ __global__ void k_eval_fix(double gamma, int64_t n, double *U, double *W, int32_
 }
 
 __global__ void k_eval_rad(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
-                           const double *ks, double *G) {
+                           const double *ks, int emission, double *G) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   State<double> w;
   ld5(W + 5 * i, w);
-  rad_source(w, Ur[i], Fr3 + 3 * i, ka[i], ks[i], G + 4 * i);
+  rad_source(w, Ur[i], Fr3 + 3 * i, ka[i], ks[i], G + 4 * i, emission != 0);
+}
+
+__global__ void k_eval_rad_opt(OpacityTable t, int64_t n, const double *W, const double *Ur, const double *Fr3,
+                               double *G, double *cell) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  State<double> w;
+  ld5(W + 5 * i, w);
+  const CellRad c = cell_rad(t, w);
+  rad_source_opt(w, c, Ur[i], Fr3 + 3 * i, G + 4 * i);
+  double *o = cell + 5 * i;
+  o[0] = c.T, o[1] = c.vel, o[2] = c.lorenz, o[3] = c.alpha, o[4] = c.betta;
 }
 
 // Plain copy, 16 B per thread per iteration (calibration of byte counters).
@@ -739,9 +769,17 @@ hipError_t launch_eval_fix(int eos, double gamma, int64_t n, double *U, double *
   return hipGetLastError();
 }
 hipError_t launch_eval_rad(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
-                           const double *ks, double *G, hipStream_t s) {
+                           const double *ks, int emission, double *G, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_eval_rad, eval_grid(n), dim3(256), 0, s, n, W, Ur, Fr3, ka, ks, G);
+  hipLaunchKernelGGL(k_eval_rad, eval_grid(n), dim3(256), 0, s, n, W, Ur, Fr3, ka, ks, emission, G);
+  return hipGetLastError();
+}
+
+hipError_t launch_eval_rad_opt(int nx, int ny, const double *hdr, const double *table, int64_t n, const double *W,
+                               const double *Ur, const double *Fr3, double *G, double *cell, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const OpacityTable t{table, nx, ny, hdr[0], hdr[1], hdr[2], hdr[3], hdr[4], hdr[5]};
+  hipLaunchKernelGGL(k_eval_rad_opt, eval_grid(n), dim3(256), 0, s, t, n, W, Ur, Fr3, G, cell);
   return hipGetLastError();
 }
 

@@ -453,17 +453,22 @@ __device__ __forceinline__ R hllc_flux(R G1, R Gg, const State<R> &cUL, const St
 // c_t, so the global momentum flux is c_t m + (F_n - c_t m_n) n, with F_n the
 // normal component.  Only v.n and m.n of each side are needed.
 // What the face kernel reads of a cell on the mixed path, stored by the cell
-// kernel as two float4: (vx, vy, vz, p) in FP32, and (s, D, cs2, e) with
-// s = rho h W^2 (m = s v, E = s - p), D = rho W, cs2 = gamma p/(rho h).
-// s and D are formed in FP64 from the FP32-rounded primitives and stored as
-// FP32 plus 8-bit extensions packed in e (flux_ext.h), so they match the
-// stored v and p to 2^-31: at Lorentz factors ~10 an s rounded on its own
-// (inconsistent with v, p by 6e-8) raised the transverse-momentum L1 of a
-// shear layer from 1e-6 to 2e-5.  Forming these once per cell, instead of once
-// per face side, removes the p2c and sound-speed divides and roots from the
-// face kernel.  rho is not stored: rho = D sqrt(1 - v^2).
+// kernel as two float4: (vx, vy, vz, p) in FP32, and (sigma, D, cs2, e) with
+// sigma = s - D, s = rho h W^2 (m = s v, E = s - p), D = rho W,
+// cs2 = gamma p/(rho h).  sigma and D are formed in FP64 from the FP32-rounded
+// primitives and stored as FP32 plus 8-bit extensions packed in e
+// (flux_ext.h), so they match the stored v and p to 2^-31: at Lorentz factors
+// ~10 an s rounded on its own (inconsistent with v, p by 6e-8) raised the
+// transverse-momentum L1 of a shear layer from 1e-6 to 2e-5.  The record
+// holds sigma rather than s because E - D = sigma - p is the thermal and
+// kinetic energy: in a gas at T ~ 1e4 K (p / rho c^2 ~ 1e-9) it is below the
+// 2^-31 resolution of s, and an s-based record lost the pressure (momentum L1
+// 3e-3 against FP64 after 20 steps).  Forming these once per cell, instead of
+// once per face side, removes the p2c and sound-speed divides and roots from
+// the face kernel.  rho is not stored: rho = D sqrt(1 - v^2).
 struct MixedIn {
   double v[3], p, s, D, cs2;
+  double sigma;  // s - D
 };
 
 // The record of an FP64 primitive state (|v| < 1).
@@ -475,20 +480,25 @@ __device__ __forceinline__ void mixed_record(double gamma, double Gg, const Stat
   va = make_float4((float)W.v[0], (float)W.v[1], (float)W.v[2], (float)W.p);
   const double rho = (float)W.d, p = va.w;
   const double v[3] = {va.x, va.y, va.z};
-  const double lor2 = A::rcp(1.0 - dot3(v, v));
-  double rh, cs2;  // rho h, sound speed^2
+  const double vv = dot3(v, v);
+  const double lor2 = A::rcp(1.0 - vv);
+  const double lor = A::sqrt_(lor2);
+  double rhm1, cs2;  // rho (h - 1), sound speed^2
   if (EOS == kIdeal) {
-    rh = rho + Gg * p;
-    cs2 = A::div(gamma * p, rh);
+    rhm1 = Gg * p;
+    cs2 = A::div(gamma * p, rho + rhm1);
   } else {
     const double th = A::div(p, rho);
-    const double h = 2.5 * th + A::sqrt_(2.25 * th * th + 1.0);
-    rh = rho * h;
+    const double r = A::sqrt_(2.25 * th * th + 1.0);
+    const double h = 2.5 * th + r;
+    rhm1 = 2.5 * p + rho * A::div(2.25 * th * th, r + 1.0);  // rho (h - 1) without the cancellation
     cs2 = A::div(th * (5.0 * h - 8.0 * th), 3.0 * h * (h - th));
   }
+  // sigma = s - D = rho W (h W - 1) = W (rho (h - 1) W + rho (W - 1)), W - 1 = W^2 v^2 / (W + 1)
+  const double sigma = lor * (rhm1 * lor + rho * A::div(lor2 * vv, lor + 1.0));
   uint32_t es, eD;
-  sd.x = rhllc_ext::split<8>(rh * lor2, es);
-  sd.y = rhllc_ext::split<8>(rho * A::sqrt_(lor2), eD);
+  sd.x = rhllc_ext::split<8>(sigma, es);
+  sd.y = rhllc_ext::split<8>(rho * lor, eD);
   sd.z = (float)cs2;
   sd.w = rhllc_ext::bits_f32(es | (eD << 8));
 }
@@ -497,8 +507,9 @@ __device__ __forceinline__ MixedIn mixed_in(const float4 &va, const float4 &sd) 
   MixedIn o;
   const uint32_t e = rhllc_ext::f32_bits(sd.w);
   o.v[0] = va.x, o.v[1] = va.y, o.v[2] = va.z, o.p = va.w;
-  o.s = rhllc_ext::join<8>(sd.x, e);
+  o.sigma = rhllc_ext::join<8>(sd.x, e);
   o.D = rhllc_ext::join<8>(sd.y, e >> 8);
+  o.s = o.D + o.sigma;
   o.cs2 = sd.z;
   return o;
 }
@@ -506,6 +517,7 @@ __device__ __forceinline__ MixedIn mixed_in(const float4 &va, const float4 &sd) 
 struct MixedSide {
   double p, v[3];           // primitives (v after the clamp)
   double D, E, s;           // conserved D, E and s (m = s v)
+  double sigma;             // s - D
   double vn, mn;            // v.n, m.n
   double cmin, cmax;        // signal speeds
 };
@@ -525,7 +537,8 @@ __device__ __forceinline__ void mixed_side(const MixedIn &w, const double *n, Mi
   }
   o.s = w.s;
   o.D = w.D;
-  o.E = w.s - w.p;
+  o.sigma = w.sigma;
+  o.E = w.D + (w.sigma - w.p);  // E - D = sigma - p, formed from the two small terms
   o.vn = dot3(o.v, n);
   o.mn = o.s * o.vn;
   const double cs2 = w.cs2;
@@ -542,7 +555,8 @@ __device__ __forceinline__ void mixed_side(const MixedIn &w, const double *n, Mi
 // relies on: m = m_n n + m_t holds only for |n| = 1).  The contact solve is
 // evaluated for every face and its result taken only in the subsonic case
 // (lL < 0 < lR), so the face needs no branch.
-// F receives S * flux; returns max(lL, lR) as hllc_flux does.
+// F receives S * flux with F.p the flux of E - D (not of E); returns
+// max(lL, lR) as hllc_flux does.
 __device__ __forceinline__ double mixed_face_flux(const MixedIn &WL, const MixedIn &WR, float sx, float sy,
                                                   float sz, State<double> &F) {
 #pragma clang fp contract(fast)
@@ -573,26 +587,30 @@ __device__ __forceinline__ double mixed_face_flux(const MixedIn &WL, const Mixed
   const bool left = sup ? lL >= 0.0 : us >= 0.0;
   const double vn = left ? L.vn : R.vn, mn = left ? L.mn : R.mn, p = left ? L.p : R.p;
   const double E = left ? L.E : R.E, D = left ? L.D : R.D, s = left ? L.s : R.s;
+  const double sig = left ? L.sigma : R.sigma;
   const double v0 = left ? L.v[0] : R.v[0], v1 = left ? L.v[1] : R.v[1], v2 = left ? L.v[2] : R.v[2];
   const double lam = left ? lL : lR;
   const double dif = A::rcp(lam - us);
   const double ms = (lam * (E + ps) - mn) * us * dif;  // star normal momentum
   const double ct = sup ? vn : vn + lam * ((lam - vn) * dif - 1.0);
   const double Fn = sup ? mn * vn + p : mn * vn + lam * (ms - mn) + p;
-  const double FE = sup ? mn : ms;  // F_E = m*_n in the star region
   const double cm = ct * s;         // c_t m = c_t s v
   const double cn = Fn - ct * mn;   // normal remainder (the pressure-like part)
   F.d = D * ct * S;
   F.v[0] = (cm * v0 + cn * n[0]) * S;
   F.v[1] = (cm * v1 + cn * n[1]) * S;
   F.v[2] = (cm * v2 + cn * n[2]) * S;
-  F.p = FE * S;
+  // F_E - F_D from the small terms only: sigma v_n outside the fan, and in the
+  // star region m*_n - D* u* = u* (lam (sigma - p + p*) - sigma v_n) / (lam - u*)
+  // (D* = D (lam - v_n) / (lam - u*)); forming F_E and F_D and subtracting
+  // would leave E - D with the rounding of D.  (F_E = m*_n in the star region.)
+  const double Ftau = sup ? sig * vn : us * dif * (lam * (sig - p + ps) - sig * vn);
+  F.p = Ftau * S;
   return smax(lL, lR);
 }
 
-// Radiation 4-momentum source, solvers/RadRHD/src/radRHD_utils.cpp:8-101 with
-// the M1 Eddington tensor (ISOTROPIC_INTENSITY == 1 branch); the thermal
-// emission term is omitted (q = 0).  EddTensor(a, b) is evaluated once per
+// Radiation 4-momentum source, solvers/RadRHD/src/radRHD_utils.cpp:8-158 with
+// the M1 Eddington tensor (ISOTROPIC_INTENSITY == 1 branch).  EddTensor(a, b) is evaluated once per
 // component from shared subexpressions; every component keeps the reference's
 // operation order, so the values are bitwise those of nine separate calls
 // (A = Ieee; the mixed path runs the same sequence with the Fast policy).
@@ -636,16 +654,49 @@ __device__ __forceinline__ void edd_matrix(double U, const double *F, double D[3
     }
 }
 
-template <typename A = Ieee>
-__device__ __forceinline__ void rad_source(const State<double> &W, double Ur, const double *Fr, double abs_op,
-                                           double scat_op, double *G) {
+// Physical constants of the source, lib/global_consts.h:12-83 (CGS branch)
+// evaluated as the reference's constexpr chain, and the three logarithms its
+// header generator writes into log_global_consts.h (resources/parser_header.cpp,
+// 16 significant digits), which full_phys_data_t::Init uses through LOG().
+namespace radc {
+constexpr double kPI = 3.1415926535897932384626433832795;
+constexpr double kC_Light = 3 * 1e10;
+constexpr double kM_hydrogen = 1.6735575 * 1e-24;
+constexpr double k_boltzmann = 1.3807 * 1e-16;
+constexpr double kStefanBoltzmann = 5.670374419 * 1e-5;
+constexpr double kSigma_thomson = 6.65210 * 1e-25;
+constexpr double kGravity = 6.6726 * 1e-8;
+constexpr double kDist = 1e13;
+constexpr double kVelocity = kC_Light;
+constexpr double kMass = (kVelocity * kVelocity * kDist) / kGravity;
+constexpr double kTime = kDist / kVelocity;
+constexpr double kDensity = kMass / (kDist * kDist * kDist);
+constexpr double kPressure = kMass / (kDist * kTime * kTime);
+constexpr double kRadiation = kMass / (kTime * kTime * kTime);
+constexpr double log_kDensity = 4.904385174253596;
+constexpr double log_kStefanBoltzmann4 = -8.391375953193897;
+constexpr double log_kDist = 29.93360620892259;
+}  // namespace radc
+
+// GetTemperature (lib/physics/include/gas_state.h:22-27): T in K of (rho, p) in code units.
+__device__ __forceinline__ double gas_temperature(double rho, double prs) {
+  constexpr double g = (radc::kM_hydrogen / radc::k_boltzmann) * radc::kPressure / radc::kDensity;
+  return g * prs / rho;
+}
+// B_Plank(T) (lib/physics/src/plunk.cpp:87-90): sigma T^4 / pi.
+__device__ __forceinline__ double b_planck(double T) {
+  const double T2 = T * T;
+  return (T2 * T2) * radc::kStefanBoltzmann / radc::kPI;
+}
+
+// The common tail of GetRadSource and GetRadSourceOpt (radRHD_utils.cpp:
+// 74-100 and 131-157) once gamma, gamma^2 and u^2 are known.
+template <typename A>
+__device__ __forceinline__ void rad_source_tail(const State<double> &W, double Ur, const double *Fr, double abs_op,
+                                                double scat_op, double q, double gamma, double gamma2, double u2,
+                                                double *G) {
   const double tot_op = abs_op + scat_op;
   const double rho = W.d;
-  double gamma = dot3(W.v, W.v);
-  double u2 = gamma;
-  const double gamma2 = A::rcp(1.0 - gamma);
-  u2 *= gamma2;
-  gamma = A::sqrt_(gamma2);
   const double u[3] = {gamma * W.v[0], gamma * W.v[1], gamma * W.v[2]};
   double Dm[3][3];
   edd_matrix<A>(Ur, Fr, Dm);
@@ -663,12 +714,92 @@ __device__ __forceinline__ void rad_source(const State<double> &W, double Ur, co
     }
   }
   const double rhogamma = rho * gamma;
-  const double q = 0.0;
   G[0] = q * gamma + rhogamma * (abs_op - scat_op * (u2 + uuD)) * Ur - rho * (abs_op - scat_op * (u2 + gamma2)) * uF;
 #pragma unroll
   for (int i = 0; i < 3; i++)
     G[i + 1] = q * u[i] - rho * (tot_op * uD[i] + scat_op * u[i] * (gamma2 + uuD)) * Ur +
                rhogamma * (tot_op * Fr[i] + scat_op * 2.0 * uF * u[i]);
+}
+
+// GetRadSource (radRHD_utils.cpp:44-101): given opacities; the thermal
+// emission q = -kappa_a rho B(T(rho, p)) / kRadiation when `emission`.
+template <typename A = Ieee>
+__device__ __forceinline__ void rad_source(const State<double> &W, double Ur, const double *Fr, double abs_op,
+                                           double scat_op, double *G, bool emission = false) {
+  double gamma = dot3(W.v, W.v);
+  double u2 = gamma;
+  const double gamma2 = A::rcp(1.0 - gamma);
+  u2 *= gamma2;
+  gamma = A::sqrt_(gamma2);
+  const double q = emission ? -abs_op * W.d * b_planck(gas_temperature(W.d, W.p)) / radc::kRadiation : 0.0;
+  rad_source_tail<A>(W, Ur, Fr, abs_op, scat_op, q, gamma, gamma2, u2, G);
+}
+
+// The reference's TableFunc (lib/grids/include/solvers_struct.h:24-44) over
+// (ln D, ln T): ln Lambda at [i*ny + j].
+struct OpacityTable {
+  const double *data;
+  int nx, ny;
+  double min_x, max_x, step_x, min_y, max_y, step_y;
+};
+
+// TableFunc::operator() (lib/grids/src/solvers_struct.cpp:10-58): the
+// arguments are clamped below and x above; the node is taken at round(x),
+// round(y) (nearest integers, then truncated to a node index); above max_y the
+// last two nodes of the row are extended linearly in exp(y).  The indices are
+// also clamped at 0, which the reference leaves to its tables (round(min) >= min).
+__device__ __forceinline__ double table_lookup(const OpacityTable &t, double x, double y) {
+  if (x < t.min_x) x = t.min_x;
+  if (y < t.min_y) y = t.min_y;
+  if (x > t.max_x) x = t.max_x;
+  int i = (int)((round(x) - t.min_x) / t.step_x);
+  i = i < t.nx - 1 ? i : t.nx - 1;
+  i = i > 0 ? i : 0;
+  if (y > t.max_y) {
+    const double y1 = exp(t.data[t.ny * i + (t.ny - 1)]);
+    const double y0 = exp(t.data[t.ny * i + (t.ny - 2)]);
+    const double x1 = exp(t.max_y);
+    const double x0 = exp(t.max_y - t.step_y);
+    const double k = (y0 - y1) / (x0 - x1);
+    const double m = (-x1 * y0 + x0 * y1) / (x0 - x1);
+    return log(k * exp(y) + m);
+  }
+  int j = (int)((round(y) - t.min_y) / t.step_y);
+  j = j < t.ny - 1 ? j : t.ny - 1;
+  j = j > 0 ? j : 0;
+  return t.data[t.ny * i + j];
+}
+
+// full_phys_data_t::Init(const flux_t *) (lib/grids/src/solvers_struct.cpp:264-281):
+// temperature, speed, Lorentz factor and the opacities of a cell's state.
+struct CellRad {
+  double T, vel, lorenz, alpha, betta;
+};
+template <typename A = Ieee>
+__device__ __forceinline__ CellRad cell_rad(const OpacityTable &t, const State<double> &W) {
+  CellRad c;
+  c.T = gas_temperature(W.d, W.p);
+  const double logT = log(c.T);
+  const double vel2 = dot3(W.v, W.v);
+  c.vel = A::sqrt_(vel2);
+  c.lorenz = A::rcp(A::sqrt_(1. - vel2));
+  const double L = table_lookup(t, log(W.d) + radc::log_kDensity, logT);
+  const double log_alpha = L - (radc::log_kStefanBoltzmann4 + 4 * logT) + radc::log_kDist;
+  c.alpha = exp(log_alpha);
+  c.betta = (radc::kSigma_thomson / radc::kM_hydrogen * radc::kDist) * W.d * radc::kDensity;
+  return c;
+}
+
+// GetRadSourceOpt (radRHD_utils.cpp:103-158): the cell's own opacities and
+// temperature, gamma from its Lorentz factor, thermal emission always on.
+template <typename A = Ieee>
+__device__ __forceinline__ void rad_source_opt(const State<double> &W, const CellRad &c, double Ur, const double *Fr,
+                                               double *G) {
+  const double gamma = c.lorenz;
+  const double gamma2 = gamma * gamma;
+  const double u2 = gamma2 * c.vel * c.vel;
+  const double q = -c.alpha * W.d * b_planck(c.T) / radc::kRadiation;
+  rad_source_tail<A>(W, Ur, Fr, c.alpha, c.betta, q, gamma, gamma2, u2, G);
 }
 
 }  // namespace rhllc_dev

@@ -179,6 +179,20 @@ def read_vtk(path: str):
     return points, tets
 
 
+def read_table(path: str):
+    """(header[6], data[nx, ny]) of a cooling-function table (tab_func/rcf_H.dat format);
+    header = (min_x, max_x, step_x, min_y, max_y, step_y) over (ln D, ln T)."""
+    lib = load_mesh_lib()
+    nx, ny = C.c_int32(), C.c_int32()
+    hdr = np.zeros(6)
+    if lib.rhllc_table_read(os.fsencode(path), C.byref(nx), C.byref(ny), hdr.ctypes.data, None) != 0:
+        raise ValueError(lib.rhllc_mesh_last_error().decode())
+    data = np.zeros((nx.value, ny.value))
+    if lib.rhllc_table_read(os.fsencode(path), C.byref(nx), C.byref(ny), hdr.ctypes.data, data.ctypes.data) != 0:
+        raise ValueError(lib.rhllc_mesh_last_error().decode())
+    return hdr, data
+
+
 @dataclass
 class TetGeometry:
     """The per-cell arrays of the reference's preprocessing (BuildDataFromVTK,

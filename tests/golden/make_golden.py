@@ -17,9 +17,11 @@ compiled reference returns.  The fixtures are data only (inputs + outputs):
   brick_cone.json  the reference's RHLLC test on brick.vtk to T = 0.5: geometry and state hashes
   taub.npz, brick_cone_taub.json  the pointwise functions, a tiny trajectory and the brick test
                  from the reference built with EOS TAUB (its shipped setting)
+  rcf_H.dat.xz, table.npz  the reference's cooling-function table (tab_func), compressed, and
+                 what its ReadTableFunc and TableFunc::operator() return on it
 
 ``python make_golden.py --vtk`` rewrites only the brick/sphere files,
-``--taub`` only the EOS TAUB files.
+``--taub`` only the EOS TAUB files, ``--table`` only the table files.
 """
 import hashlib
 import json
@@ -54,7 +56,8 @@ def unit(rng, n):
     return x / np.linalg.norm(x, axis=1)[:, None]
 
 
-REF_DATA = "/root/reference/tests/data"
+REF_ROOT = "/root/reference"
+REF_DATA = os.path.join(REF_ROOT, "tests", "data")
 
 
 def sha(a) -> str:
@@ -186,12 +189,49 @@ def taub_fixtures():
         json.dump(rec, f, indent=1)
 
 
+def table_fixtures():
+    """The reference's cooling-function table (tab_func/rcf_H.dat), xz-compressed, and
+    table.npz: what the reference's own ReadTableFunc (lib/files_sys/src/reader_txt.cpp:59-118)
+    reads from it (header, data hash) and what its TableFunc::operator()
+    (lib/grids/src/solvers_struct.cpp:10-58) returns at seeded points: inside the table,
+    on and between its nodes, below both minima, above max_x, and above max_y (the
+    extrapolation branch)."""
+    import lzma
+
+    src = os.path.join(REF_ROOT, "tab_func", "rcf_H.dat")
+    with open(src, "rb") as f:
+        raw = f.read()
+    with open(os.path.join(HERE, "rcf_H.dat.xz"), "wb") as f:
+        f.write(lzma.compress(raw, preset=9 | lzma.PRESET_EXTREME))
+    t = O.ref_table_read(src)
+    rng = np.random.default_rng(SEED + 31)
+    mnx, mxx, sx, mny, mxy, sy = t.hdr
+    n = 4000
+    x = rng.uniform(mnx - 3, mxx + 3, n)
+    y = rng.uniform(mny - 2, mxy + 3, n)
+    k = np.arange(200)
+    x[:200] = mnx + sx * (k % t.data.shape[0])  # node values of the axes
+    y[:200] = mny + sy * (k % t.data.shape[1])
+    x[200:300] = np.round(rng.uniform(mnx, mxx, 100)) + 0.5  # half-integers: round() ties
+    y[200:300] = np.round(rng.uniform(mny, mxy, 100)) + 0.5
+    y[300:400] = mxy + rng.uniform(0, 1e-3, 100)  # just above max_y
+    y[400:420] = mxy  # exactly max_y
+    x[420:440], y[420:440] = mnx, mny
+    out = O.ref_table_lookup(t, x, y)
+    np.savez(os.path.join(HERE, "table.npz"), hdr=t.hdr, shape=np.array(t.data.shape), data_sha256=sha(t.data),
+             x=x, y=y, value=out)
+
+
 def main():
     if not O.have_ref():
         sys.exit("oracle/_ref/libref_rhllc.so is not built (make -C oracle ref)")
     if sys.argv[1:] == ["--vtk"]:
         vtk_fixtures()
         print("wrote VTK fixtures to", HERE)
+        return
+    if sys.argv[1:] == ["--table"]:
+        table_fixtures()
+        print("wrote table fixtures to", HERE)
         return
     if sys.argv[1:] == ["--taub"]:
         taub_fixtures()
@@ -279,6 +319,7 @@ def main():
         json.dump(rec, f, indent=1)
     vtk_fixtures()
     taub_fixtures()
+    table_fixtures()
     print("wrote fixtures to", HERE)
 
 

@@ -1,0 +1,160 @@
+"""GPU parity of the radiation coupling: the thermal emission term and the
+table-opacity source (full_phys_data_t::Init + GetRadSourceOpt, as AddRadFlux
+applies it), pointwise and in the coupled step, against the oracle.
+
+Tolerances: everything without a transcendental function is bitwise.  The
+table path evaluates log and exp (ln T, ln rho, exp(log alpha), and log/exp
+in the table's extrapolation branch); the device's and glibc's double-precision
+log/exp may differ in the last bit, so alpha and G are compared to 1e-13
+relative, and the coupled trajectories to 1e-10 relative per field.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from radrhd_solver_amd import capi
+from radrhd_solver_amd import mesh as M
+
+from rad_cases import code_state, physical_blast, table_path
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261202
+
+
+@pytest.fixture(scope="module")
+def table(tmp_path_factory):
+    """(header, data) by the product's reader, and the same arrays as an oracle Table."""
+    hdr, data = M.read_table(table_path(tmp_path_factory.mktemp("tab")))
+    return (hdr, data), O.Table(hdr, data)
+
+
+def _rad_inputs(rng, n):
+    rho = 10 ** rng.uniform(-15, -7, n)
+    T = 10 ** rng.uniform(2.5, 6.5, n)  # below min_y (ln T < 6.89), inside, above max_y (ln T > 13.84)
+    W = code_state(rho, T, 0.0)
+    v = rng.normal(size=(n, 3))
+    W[:, 1:4] = v * (rng.uniform(0, 0.9, (n, 1)) / np.linalg.norm(v, axis=1, keepdims=True))
+    W[:16, 1:4] = 0.0
+    Ur = W[:, 4] * 10 ** rng.uniform(-3, 1, n)
+    F = rng.normal(size=(n, 3))
+    F *= (Ur * np.array([0.0, 0.3, 0.99, 1.5])[np.arange(n) % 4] / np.linalg.norm(F, axis=1))[:, None]
+    return W, Ur, F
+
+
+def test_eval_emission_bitwise():
+    rng = np.random.default_rng(SEED + 1)
+    n = 4000
+    W, Ur, F = _rad_inputs(rng, n)
+    ka, ks = 10 ** rng.uniform(-3, 3, (2, n))
+    G = capi.eval_rad_source(W, Ur, F, ka, ks, emission=True)
+    Go = np.array([O.rad_source(W[i], Ur[i], F[i], ka[i], ks[i], emission=1) for i in range(n)])
+    assert np.array_equal(G, Go)
+    G0 = capi.eval_rad_source(W, Ur, F, ka, ks, emission=False)
+    assert not np.array_equal(G, G0)
+
+
+def test_eval_rad_source_opt(table):
+    (hdr, data), t = table
+    rng = np.random.default_rng(SEED + 2)
+    n = 6000
+    W, Ur, F = _rad_inputs(rng, n)
+    G, cell = capi.eval_rad_source_opt((hdr, data), W, Ur, F)
+    co = O.cell_rad(t, W)
+    Go = np.array([O.rad_source_opt(W[i], co[i], Ur[i], F[i]) for i in range(n)])
+    # T, |v|, Lorentz factor and Thomson opacity: no transcendental function -> bitwise
+    for k in (0, 1, 2, 4):
+        assert np.array_equal(cell[:, k], co[:, k]), k
+    rel = np.abs(cell[:, 3] - co[:, 3]) / co[:, 3]
+    print("alpha: max rel diff", rel.max(), "bitwise fraction", (cell[:, 3] == co[:, 3]).mean())
+    assert rel.max() <= 1e-13
+    scale = np.abs(Go).max(axis=1, keepdims=True)
+    assert (np.abs(G - Go) <= 1e-13 * scale).all()
+    lnT = np.log(co[:, 0])
+    assert (lnT > hdr[4]).sum() > 100 and (lnT < hdr[3]).sum() > 100  # both table edges exercised
+
+
+def _run_gpu(m, W0, Ur, Fr, steps, table=None, emission=False, precision=capi.PREC_FP64,
+             guess=capi.GUESS_REFERENCE, ka=None, ks=None):
+    s = capi.Solver(m, cfl=0.4, h_min=m.h_min(), precision=precision, guess=guess)
+    s.upload_W(W0)
+    if table is not None:
+        s.set_opacity_table(table)
+    s.set_radiation_emission(emission)
+    s.set_radiation(Ur, Fr, ka, ks)
+    s.run(steps)
+    s.sync()
+    W, U = s.download()
+    stats = s.stats()
+    s.close()
+    return W, U, stats
+
+
+def test_step_emission_bitwise():
+    """radRHD_main.cpp:117-132 with GetRadSource's thermal emission, on physical states."""
+    m = M.kuhn_box(12, 12, 12, seed=SEED)
+    W0, Ur, Fr = physical_blast(m, seed=5, ur_scale=3.0)
+    rng = np.random.default_rng(SEED + 3)
+    ka = (10 ** rng.uniform(2, 6, m.n_cells)).astype(np.float32)
+    ks = (10 ** rng.uniform(0, 3, m.n_cells)).astype(np.float32)
+    st = O.State(m, W0)
+    st.run_rad(0.4, m.h_min(), 20, Ur, Fr, ka, ks, emission=1)
+    W, U, _ = _run_gpu(m, W0, Ur, Fr, 20, emission=True, ka=ka, ks=ks)
+    assert np.array_equal(st.U, U) and np.array_equal(st.W, W)
+
+
+def test_step_table_vs_oracle(table):
+    """AddRadFlux + HllcConvToPhys with the table's opacities: 20 coupled steps."""
+    (hdr, data), t = table
+    m = M.kuhn_box(12, 12, 12, seed=SEED)
+    W0, Ur, Fr = physical_blast(m, seed=5, ur_scale=30.0)
+    st = O.State(m, W0)
+    st.run_rad(0.4, m.h_min(), 20, Ur, Fr, table=t)
+    W, U, stats = _run_gpu(m, W0, Ur, Fr, 20, table=(hdr, data))
+    assert stats["fix_fail"] == 0
+    hyd = O.State(m, W0)
+    hyd.run(0.4, m.h_min(), 20)
+    moved = np.abs(st.W[:, 4] - hyd.W[:, 4]).max() / np.abs(hyd.W[:, 4]).max()
+    assert moved > 1e-3  # the source changes the pressure visibly
+    for name, a, b in (("U", U, st.U), ("W", W, st.W)):
+        for k in range(5):
+            ref = np.abs(b[:, k]).max()
+            d = np.abs(a[:, k] - b[:, k]).max()
+            print(name, k, "max abs diff / max |ref|", d / ref if ref else d)
+            assert d <= 1e-10 * ref, (name, k)
+
+
+def test_mixed_table_l1(table):
+    """The mixed step with table opacities against the FP64 step: per-field L1 <= 1e-5."""
+    (hdr, data), _ = table
+    m = M.kuhn_box(12, 12, 12, seed=SEED)
+    W0, Ur, Fr = physical_blast(m, seed=6, ur_scale=30.0)
+    _, U64, _ = _run_gpu(m, W0, Ur, Fr, 20, table=(hdr, data))
+    _, U32, st32 = _run_gpu(m, W0, Ur, Fr, 20, table=(hdr, data), precision=capi.PREC_MIXED,
+                            guess=capi.GUESS_PREVIOUS)
+    assert st32["fix_fail"] == 0
+    for k in range(5):
+        ref = np.abs(U64[:, k]).sum()
+        l1 = np.abs(U32[:, k] - U64[:, k]).sum() / ref
+        print("field", k, "rel L1", l1)
+        assert l1 <= 1e-5, k
+
+
+def test_table_argument_checks(table):
+    (hdr, data), _ = table
+    m = M.kuhn_box(2, 2, 2, seed=SEED)
+    s = capi.Solver(m)
+    try:
+        with pytest.raises(capi.RhllcError):
+            s.set_opacity_table((hdr, data[:, :1]))  # ny < 2
+        bad = hdr.copy()
+        bad[2] = 0.0
+        with pytest.raises(capi.RhllcError):
+            s.set_opacity_table((bad, data))
+        with pytest.raises(capi.RhllcError):  # no opacities at all
+            s.set_radiation(np.ones(m.n_cells), np.zeros((m.n_cells, 3)))
+        s.set_opacity_table((hdr, data))
+        s.set_radiation(np.ones(m.n_cells), np.zeros((m.n_cells, 3)))
+        s.set_opacity_table(None)
+    finally:
+        s.close()
