@@ -153,6 +153,16 @@ int rhllc_set_radiation_emission(rhllc_solver *h, int32_t on);
  * emission included.  NULL data removes the table. */
 int rhllc_set_opacity_table(rhllc_solver *h, int32_t nx, int32_t ny, const double header[6], const double *data);
 
+/* The table-opacity source on its own, for a caller that runs the transport
+ * between the hydro update and the source as radRHD_mpi_main.cpp:112-122
+ * does: AddRadFlux (rhllc_3d_mpi.cpp:509-533) with the moments Ur, Fr3 (per
+ * cell, caller's numbering) and tau, then HllcConvToPhys
+ * (rhllc_3d_mpi.cpp:488-505), on the handle's current state.  Needs an opacity
+ * table; replaces the moments of rhllc_set_radiation and leaves rhllc_step's
+ * own source on or off as it was (run the step without it: Ur = NULL there).
+ * RHLLC_ERR_C2P when a PhysPressureFix failed. */
+int rhllc_add_rad_flux(rhllc_solver *h, double tau, const float *Ur, const float *Fr3);
+
 /* Average device time (ms) of each kernel of the step, measured with HIP
  * events around each launch on the handle's stream over nsteps eager steps:
  * out[0] face kernel, out[1] cell kernel, out[2] dt kernel, out[3] whole step. */

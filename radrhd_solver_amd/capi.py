@@ -96,6 +96,7 @@ SIGNATURES = {
     "rhllc_download_face_flux": (C.c_int, [_vp, _vp]),
     "rhllc_set_radiation": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "rhllc_set_radiation_emission": (C.c_int, [_vp, _i32]),
+    "rhllc_add_rad_flux": (C.c_int, [_vp, _dbl, _vp, _vp]),
     "rhllc_set_opacity_table": (C.c_int, [_vp, _i32, _i32, _vp, _vp]),
     "rhllc_time_kernels": (C.c_int, [_vp, _i32, _vp]),
     "rhllc_info": (C.c_int, [_vp, _vp, _vp, _vp]),
@@ -228,6 +229,12 @@ class Solver:
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float32) for a in (Ur, Fr, ka, ks)]
         self._rad = arrs
         check(_lib().rhllc_set_radiation(self.h, *[None if a is None else a.ctypes.data for a in arrs]))
+
+    def add_rad_flux(self, tau, Ur, Fr):
+        """AddRadFlux + HllcConvToPhys with the table's opacities on the current state."""
+        Ur = np.ascontiguousarray(Ur, dtype=np.float32)
+        Fr = np.ascontiguousarray(Fr, dtype=np.float32)
+        check(_lib().rhllc_add_rad_flux(self.h, tau, Ur.ctypes.data, Fr.ctypes.data))
 
     def set_radiation_emission(self, on):
         check(_lib().rhllc_set_radiation_emission(self.h, 1 if on else 0))

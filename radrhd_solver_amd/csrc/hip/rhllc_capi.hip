@@ -548,6 +548,34 @@ extern "C" int rhllc_set_radiation(rhllc_solver *h, const float *Ur, const float
   return RHLLC_OK;
 }
 
+extern "C" int rhllc_add_rad_flux(rhllc_solver *h, double tau, const float *Ur, const float *Fr3) {
+  if (!h) return fail(RHLLC_ERR_ARG, "null handle");
+  if (!Ur || !Fr3) return fail(RHLLC_ERR_ARG, "null radiation array");
+  if (!(tau >= 0.0) || !std::isfinite(tau)) return fail(RHLLC_ERR_ARG, "tau must be finite and >= 0");
+  if (!h->tab) return fail(RHLLC_ERR_STATE, "no opacity table set (rhllc_set_opacity_table)");
+  if (!h->ready()) return fail(RHLLC_ERR_STATE, "no state uploaded");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const int32_t N = h->n_cells;
+  std::vector<float4> fu(N);
+  for (int32_t i = 0; i < N; ++i) {
+    const int32_t c = h->cell_new_to_old[i];
+    fu[i] = make_float4(Fr3[3 * (size_t)c], Fr3[3 * (size_t)c + 1], Fr3[3 * (size_t)c + 2], Ur[c]);
+  }
+  if (!h->rad_FU) HIP_TRY(hipMalloc((void **)&h->rad_FU, sizeof(float4) * N));
+  HIP_TRY(hipMemcpy(h->rad_FU, fu.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
+  h->d.rad_FU = h->rad_FU;
+  HIP_TRY(hipMemcpy(h->host_sc, h->d.sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+  const unsigned long long fix_fail_before = h->host_sc->fix_fail;
+  HIP_TRY(launch_add_rad(h->d, h->mixed, tau, h->stream));
+  HIP_TRY(hipMemcpyAsync(h->host_sc, h->d.sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->host_sc->fix_fail > fix_fail_before)
+    return fail(RHLLC_ERR_C2P, std::to_string(h->host_sc->fix_fail - fix_fail_before) +
+                                   " cells failed PhysPressureFix after the radiation source");
+  return RHLLC_OK;
+}
+
 extern "C" int rhllc_set_radiation_emission(rhllc_solver *h, int32_t on) {
   if (!h) return fail(RHLLC_ERR_ARG, "null handle");
   HIP_TRY(hipSetDevice(h->prm.device));

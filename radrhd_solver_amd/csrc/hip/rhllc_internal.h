@@ -88,6 +88,8 @@ hipError_t launch_clear_scalars(const Dev &d, hipStream_t s);
 // Mixed path: W (FP64) recovered from U by c2p (start from the FP32 pressure).
 hipError_t launch_seed_mixed(const Dev &d, hipStream_t s);
 hipError_t launch_recover_W(const Dev &d, hipStream_t s);
+// The radiation source alone (d.rad_FU set): U += tau G(W), then c2p.
+hipError_t launch_add_rad(const Dev &d, bool mixed, double tau, hipStream_t s);
 
 hipError_t launch_copy(const void *src, void *dst, int64_t n16, hipStream_t s);
 

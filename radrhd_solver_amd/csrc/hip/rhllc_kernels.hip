@@ -236,6 +236,50 @@ __device__ __forceinline__ void update_U(const Dev &d, int c, double tau, State<
   dtau = s.p - s.d;
 }
 
+// The radiation source of one cell and the c2p after it, from the cell's
+// state W after the hydro update; U and W are updated in place.
+//   radRHD_main.cpp:117-132: U += (ds*tau)*G(W), ds = 1, then c2p (code ignored).
+//   With the opacity table: AddRadFlux (rhllc_3d_mpi.cpp:509-533), GetRadSourceOpt
+//   with the cell's full_phys_data_t::Init of W, then HllcConvToPhys
+//   (rhllc_3d_mpi.cpp:488-505: c2p, PhysPressureFix on failure).
+template <bool MIXED, int EOS>
+__device__ __forceinline__ void apply_rad(const Dev &d, int c, double tau, State<double> &U, State<double> &W,
+                                          int &iters, bool &fixed, bool &fix_failed) {
+  using A = typename std::conditional<MIXED, Fast, Ieee>::type;
+  const float4 fu = d.rad_FU[c];
+  const double Fr[3] = {(double)fu.x, (double)fu.y, (double)fu.z};
+  double G[4];
+  if (d.tab) {
+    const OpacityTable t{d.tab,        d.tab_nx,    d.tab_ny,    d.tab_min_x, d.tab_max_x,
+                         d.tab_step_x, d.tab_min_y, d.tab_max_y, d.tab_step_y};
+    rad_source_opt<A>(W, cell_rad<A>(t, W), (double)fu.w, Fr, G);
+  } else {
+    const float2 kk = d.rad_k[c];
+    rad_source<A>(W, (double)fu.w, Fr, (double)kk.x, (double)kk.y, G, d.rad_emission != 0);
+  }
+  const double ds = 1;
+  U.p += ds * tau * G[0];
+  U.v[0] += ds * tau * G[1];
+  U.v[1] += ds * tau * G[2];
+  U.v[2] += ds * tau * G[3];
+  int it = 0;
+  int code = c2p<A, EOS>(d.Gg, U, W, MIXED ? W.p : -1.0, it);
+  iters += it;
+  if (d.tab && code) {
+    // HllcConvToPhys: the reference start, then PhysPressureFix of the state
+    // that c2p left (the pre-source W).
+    if (MIXED) {
+      it = 0;
+      code = c2p<A, EOS>(d.Gg, U, W, -1.0, it);
+      iters += it;
+    }
+    if (code) {
+      fixed = true;
+      fix_failed = fix_failed || pressure_fix<EOS>(d.Gg, U, W) != 0;
+    }
+  }
+}
+
 // Everything after the update for one cell: c2p from `guess` with the
 // fallback chain, PhysPressureFix, the optional radiation source with its
 // second c2p, the stores and the counters.  Called by all lanes of a wave
@@ -275,43 +319,7 @@ __device__ __forceinline__ void finish_cell(const Dev &d, int c, bool live, Stat
       fixed = true;
       fix_failed = pressure_fix<EOS>(d.Gg, U, W) != 0;
     }
-    if (RAD) {
-      // radRHD_main.cpp:117-132: U += (ds*tau)*G(W), ds = 1, then c2p (code ignored).
-      // With the opacity table: AddRadFlux (rhllc_3d_mpi.cpp:509-533) with
-      // GetRadSourceOpt and the cell's full_phys_data_t of the updated W.
-      const float4 fu = d.rad_FU[c];
-      const double Fr[3] = {(double)fu.x, (double)fu.y, (double)fu.z};
-      double G[4];
-      if (d.tab) {
-        const OpacityTable t{d.tab,       d.tab_nx,    d.tab_ny,     d.tab_min_x, d.tab_max_x,
-                             d.tab_step_x, d.tab_min_y, d.tab_max_y, d.tab_step_y};
-        rad_source_opt<A>(W, cell_rad<A>(t, W), (double)fu.w, Fr, G);
-      } else {
-        const float2 kk = d.rad_k[c];
-        rad_source<A>(W, (double)fu.w, Fr, (double)kk.x, (double)kk.y, G, d.rad_emission != 0);
-      }
-      const double ds = 1;
-      U.p += ds * tau * G[0];
-      U.v[0] += ds * tau * G[1];
-      U.v[1] += ds * tau * G[2];
-      U.v[2] += ds * tau * G[3];
-      int it3 = 0;
-      int code2 = c2p<A, EOS>(d.Gg, U, W, MIXED ? W.p : -1.0, it3);
-      iters += it3;
-      if (d.tab && code2) {
-        // HllcConvToPhys (rhllc_3d_mpi.cpp:488-505): reference start, then
-        // PhysPressureFix of the state that c2p left (the pre-source W).
-        if (MIXED) {
-          it3 = 0;
-          code2 = c2p<A, EOS>(d.Gg, U, W, -1.0, it3);
-          iters += it3;
-        }
-        if (code2) {
-          fixed = true;
-          fix_failed = fix_failed || pressure_fix<EOS>(d.Gg, U, W) != 0;
-        }
-      }
-    }
+    if (RAD) apply_rad<MIXED, EOS>(d, c, tau, U, W, iters, fixed, fix_failed);
     if (STATS) resid = c2p_residual<EOS>(d.Gg, U, W.p);
     d.U4[c] = make_double4(U.d, U.v[0], U.v[1], U.v[2]);
     d.UE[c] = U.p;
@@ -481,6 +489,44 @@ __global__ __launch_bounds__(kBlock) void k_recover_W(Dev d) {
   d.Wp[c] = W.p;
 }
 
+// The radiation source on its own, after a step without it: AddRadFlux then
+// HllcConvToPhys (rhllc_3d_mpi.cpp:509-533, 488-505) for a caller that solves
+// the transport between the hydro update and the source, as the reference's
+// coupled loop does (radRHD_mpi_main.cpp:112-122).  The FP64 path reads the
+// stored W; the mixed path first recovers the FP64 W from U (c2p from the
+// stored FP32 pressure, as the fused step has it in registers).
+template <bool MIXED, int EOS>
+__global__ __launch_bounds__(kBlock) void k_add_rad(Dev d, double tau) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  const bool live = c < d.n_cells;
+  bool fixed = false, fix_failed = false;
+  if (live) {
+    using A = typename std::conditional<MIXED, Fast, Ieee>::type;
+    State<double> U, W;
+    load_U(d, c, U);
+    int iters = 0;
+    if (!MIXED) {
+      load_W(d, c, W);
+    } else {
+      load_W32(d, c, W);
+      c2p<A, EOS>(d.Gg, U, W, W.p, iters);  // keeps the FP32 W if it fails
+    }
+    apply_rad<MIXED, EOS>(d, c, tau, U, W, iters, fixed, fix_failed);
+    d.U4[c] = make_double4(U.d, U.v[0], U.v[1], U.v[2]);
+    d.UE[c] = U.p;
+    if (!MIXED) {
+      d.W4[c] = make_double4(W.d, W.v[0], W.v[1], W.v[2]);
+      d.Wp[c] = W.p;
+    } else {
+      store_W32<EOS>(d, c, W);
+    }
+  }
+  if (__ballot(fixed) != 0) {
+    count_if(&d.sc->fix_calls, fixed);
+    count_if(&d.sc->fix_fail, fix_failed);
+  }
+}
+
 // Clears the counters and speed slots with plain stores (graph-safe).
 __global__ void k_clear(Dev d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -607,6 +653,21 @@ hipError_t launch_recover_W(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_recover_W<kTaub>, dim3(blocks_for(d.n_cells)), dim3(kBlock), 0, s, d);
   else
     hipLaunchKernelGGL(k_recover_W<kIdeal>, dim3(blocks_for(d.n_cells)), dim3(kBlock), 0, s, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_rad(const Dev &d, bool mixed, double tau, hipStream_t s) {
+  if (d.n_cells == 0) return hipSuccess;
+  const dim3 g(blocks_for(d.n_cells)), b(kBlock);
+  const bool taub = d.eos == kTaub;
+  if (mixed && taub)
+    hipLaunchKernelGGL((k_add_rad<true, kTaub>), g, b, 0, s, d, tau);
+  else if (mixed)
+    hipLaunchKernelGGL((k_add_rad<true, kIdeal>), g, b, 0, s, d, tau);
+  else if (taub)
+    hipLaunchKernelGGL((k_add_rad<false, kTaub>), g, b, 0, s, d, tau);
+  else
+    hipLaunchKernelGGL((k_add_rad<false, kIdeal>), g, b, 0, s, d, tau);
   return hipGetLastError();
 }
 
