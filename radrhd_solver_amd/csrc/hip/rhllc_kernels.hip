@@ -62,21 +62,29 @@ __device__ __forceinline__ void store_W32(const Dev &d, int c, const State<doubl
   d.Wsd[c] = sd;
 }
 
+}  // namespace rhllc_dev
+// Wavefront reduction of the ROCm device library (ockl), declared by the HIP
+// headers only under HIP_ENABLE_EXTRA_WARP_SYNC_TYPES.
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(unsigned long long);
+namespace rhllc_dev {
+
 // Per-block max of non-negative speeds into one of kSpeedSlots slots.
+// Non-negative doubles order like their bit patterns, so the maximum is an
+// exact unsigned 64-bit maximum: the wave's is the DPP reduction of the ROCm
+// device library (every lane calls this), the block's goes through LDS.
 __device__ __forceinline__ void fold_speed(const Dev &d, double speed) {
-  __shared__ double part[kBlock / 64];
-  speed = wave_max(speed);
+  __shared__ unsigned long long part[kBlock / 64];
+  const unsigned long long w = __ockl_wfred_max_u64((unsigned long long)__double_as_longlong(speed));
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) part[wid] = speed;
+  if (lane == 0) part[wid] = w;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double m = part[0];
+    unsigned long long m = part[0];
 #pragma unroll
     for (int k = 1; k < kBlock / 64; ++k) m = m < part[k] ? part[k] : m;
-    // Non-negative doubles order like their bit patterns.
     unsigned long long *slot =
         reinterpret_cast<unsigned long long *>(d.speed_slots + (size_t)(blockIdx.x % kSpeedSlots) * kSlotStride);
-    atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+    atomicMax(slot, m);
   }
 }
 
