@@ -75,8 +75,8 @@ def test_eval_rad_source_opt(table):
 
 
 def _run_gpu(m, W0, Ur, Fr, steps, table=None, emission=False, precision=capi.PREC_FP64,
-             guess=capi.GUESS_REFERENCE, ka=None, ks=None):
-    s = capi.Solver(m, cfl=0.4, h_min=m.h_min(), precision=precision, guess=guess)
+             guess=capi.GUESS_REFERENCE, ka=None, ks=None, eos=capi.EOS_IDEAL):
+    s = capi.Solver(m, cfl=0.4, h_min=m.h_min(), precision=precision, guess=guess, eos=eos)
     s.upload_W(W0)
     if table is not None:
         s.set_opacity_table(table)
@@ -103,16 +103,17 @@ def test_step_emission_bitwise():
     assert np.array_equal(st.U, U) and np.array_equal(st.W, W)
 
 
-def test_step_table_vs_oracle(table):
-    """AddRadFlux + HllcConvToPhys with the table's opacities: 20 coupled steps."""
+@pytest.mark.parametrize("eos", [capi.EOS_IDEAL, capi.EOS_TAUB])
+def test_step_table_vs_oracle(table, eos):
+    """AddRadFlux + HllcConvToPhys with the table's opacities: 20 coupled steps, both EOS."""
     (hdr, data), t = table
     m = M.kuhn_box(12, 12, 12, seed=SEED)
     W0, Ur, Fr = physical_blast(m, seed=5, ur_scale=30.0)
-    st = O.State(m, W0)
+    st = O.State(m, W0, eos=eos)
     st.run_rad(0.4, m.h_min(), 20, Ur, Fr, table=t)
-    W, U, stats = _run_gpu(m, W0, Ur, Fr, 20, table=(hdr, data))
+    W, U, stats = _run_gpu(m, W0, Ur, Fr, 20, table=(hdr, data), eos=eos)
     assert stats["fix_fail"] == 0
-    hyd = O.State(m, W0)
+    hyd = O.State(m, W0, eos=eos)
     hyd.run(0.4, m.h_min(), 20)
     moved = np.abs(st.W[:, 4] - hyd.W[:, 4]).max() / np.abs(hyd.W[:, 4]).max()
     assert moved > 1e-3  # the source changes the pressure visibly
