@@ -150,7 +150,8 @@ int rhllc_set_radiation_emission(rhllc_solver *h, int32_t on);
  * AddRadFlux (rhllc_3d_mpi.cpp:509-533) with GetRadSourceOpt
  * (radRHD_utils.cpp:103-158) and each cell's full_phys_data_t::Init
  * (lib/grids/src/solvers_struct.cpp:264-281) of the updated state, thermal
- * emission included.  NULL data removes the table. */
+ * emission included.  NULL data removes the table; that is refused
+ * (RHLLC_ERR_STATE) while the source is on without kappa_a/kappa_s. */
 int rhllc_set_opacity_table(rhllc_solver *h, int32_t nx, int32_t ny, const double header[6], const double *data);
 
 /* The table-opacity source on its own, for a caller that runs the transport

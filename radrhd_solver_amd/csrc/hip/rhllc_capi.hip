@@ -50,6 +50,7 @@ struct rhllc_solver {
   int variant = kFp64Ref;
   bool has_state = false;
   bool rad = false;
+  bool rad_kappa = false;  // rhllc_set_radiation was given kappa_a / kappa_s
   hipStream_t stream = nullptr;
   Dev d{};
   std::vector<void *> allocs;
@@ -545,6 +546,7 @@ extern "C" int rhllc_set_radiation(rhllc_solver *h, const float *Ur, const float
   h->d.rad_FU = h->rad_FU;
   h->d.rad_k = h->rad_k;
   h->rad = true;
+  h->rad_kappa = kappa_a != nullptr;
   return RHLLC_OK;
 }
 
@@ -602,6 +604,9 @@ extern "C" int rhllc_set_opacity_table(rhllc_solver *h, int32_t nx, int32_t ny, 
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->drop_graph();
   if (!data) {
+    if (h->rad && !h->rad_kappa)
+      return fail(RHLLC_ERR_STATE, "the radiation source has no other opacities: remove it (rhllc_set_radiation "
+                                   "with Ur = NULL) or give kappa_a/kappa_s first");
     if (h->tab) HIP_TRY(hipFree(h->tab));
     h->tab = nullptr;
     h->d.tab = nullptr;

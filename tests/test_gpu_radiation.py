@@ -203,6 +203,9 @@ def test_table_argument_checks(table):
             s.set_radiation(np.ones(m.n_cells), np.zeros((m.n_cells, 3)))
         s.set_opacity_table((hdr, data))
         s.set_radiation(np.ones(m.n_cells), np.zeros((m.n_cells, 3)))
+        with pytest.raises(capi.RhllcError):  # the source would be left without opacities
+            s.set_opacity_table(None)
+        s.set_radiation(None, None)
         s.set_opacity_table(None)
     finally:
         s.close()
