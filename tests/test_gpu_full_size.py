@@ -1,0 +1,146 @@
+"""The benchmark configurations at their full sizes, checked through properties
+that do not depend on the size (the oracle would take minutes there; it is
+matched step for step at smaller sizes in test_gpu_step.py):
+
+* config 1: the blast wave on 100^3 hexes x 6 = 6M tets, 100 steps;
+* config 2: the W ~ 10 shear layer on 128^3 hexes x 6 = 12.6M tets with
+  periodic faces, gamma = 4/3, 200 steps;
+* config 3: the c2p surrogate inside config 2.
+
+Properties: mass and energy conservation (on the periodic box for every field;
+on the blast's box while its outer cells are still untouched).  The FP64 step
+sums each face flux into its two cells with opposite signs, so the totals move
+only by rounding (bar 1e-13 relative).  The mixed step also subtracts each
+cell's own flux through the closure defect of its FP32 face vectors, a per-cell
+term that keeps a uniform state at rest but does not telescope, so its totals
+move at the FP32 geometry's rounding times the flow's variation (bar 1e-9).
+Also: the mixed step against the FP64 step (per-field relative L1 <= 1e-5), no
+c2p failure and the c2p residual |f(p)|/(E + p) <= 1e-10 in every cell, and the
+FP64 step reproducing itself bitwise between eager steps and the captured graph.
+"""
+import numpy as np
+import pytest
+
+from radrhd_solver_amd import capi
+from radrhd_solver_amd import mesh as M
+from radrhd_solver_amd import surrogate
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261202
+
+
+def _run(m, W0, gamma, steps, precision, guess, model=None, graph=True):
+    s = capi.Solver(m, gamma=gamma, cfl=0.4, precision=precision, guess=guess, collect_stats=True)
+    try:
+        if model is not None:
+            s.set_mlp(model)
+        s.upload_W(W0)
+        s.sync()
+        U0 = s.download()[1]
+        if graph:
+            s.run(steps)
+        else:
+            tau = 0.4 * m.h_min()
+            for _ in range(steps):
+                tau = 0.4 * m.h_min() / s.step(tau)
+        s.sync()
+        W, U = s.download()
+        return U0, W, U, s.stats()
+    finally:
+        s.close()
+
+
+def _rel_l1(a, b, V):
+    return [float((np.abs(a[:, k] - b[:, k]) * V).sum() / (np.abs(b[:, k]) * V).sum()) for k in range(5)]
+
+
+def _drift(U0, U, V):
+    """|change of each field's total| over the field's total magnitude (start or end, the larger)."""
+    t0 = np.array([(U0[:, k] * V).sum() for k in range(5)])
+    t1 = np.array([(U[:, k] * V).sum() for k in range(5)])
+    a = np.maximum(np.abs(U0).T @ V, np.abs(U).T @ V)
+    return np.abs(t1 - t0) / a
+
+
+def _clean(st, label):
+    it = st["newton_iters"] / max(1, st["steps"])
+    print(label, "c2p_fail", st["c2p_fail"], "fix_calls", st["fix_calls"], "max residual", st["max_residual"],
+          "Newton iterations per step", round(it, 1), flush=True)
+    assert sum(st["c2p_fail"]) == 0 and st["fix_calls"] == 0
+    assert st["max_residual"] <= 1e-10
+
+
+def test_config1_blast_6M():
+    m = M.kuhn_box(100, 100, 100, seed=SEED)
+    assert m.n_cells == 6_000_000
+    W0 = M.init_state(m, M.INIT_BLAST)
+    V = m.cells["V"]
+    U0, W64, U64, st64 = _run(m, W0, 5.0 / 3.0, 100, capi.PREC_FP64, capi.GUESS_REFERENCE)
+    _clean(st64, "fp64")
+    _, _, U32, st32 = _run(m, W0, 5.0 / 3.0, 100, capi.PREC_MIXED, capi.GUESS_PREVIOUS)
+    _clean(st32, "mixed")
+    # the blast has not reached the outer cells (they move only by the FP64 closure
+    # defect of their faces), so no mass or energy has crossed the boundary
+    x = m.cells["center"]
+    outer = np.abs(x - 0.5).max(axis=1) > 0.45
+    assert np.abs(W64[outer, 1:4]).max() <= 1e-10
+    assert np.abs(W64[outer, 4] / W0[outer, 4] - 1).max() <= 1e-10
+    for label, U, bar in (("fp64", U64, 1e-13), ("mixed", U32, 1e-9)):
+        drift = _drift(U0, U, V)
+        print(label, "relative drift of (D, mx, my, mz, E) totals", drift, flush=True)
+        assert drift[0] <= bar and drift[4] <= bar
+    l1 = _rel_l1(U32, U64, V)
+    print("mixed vs fp64 rel L1", l1, flush=True)
+    assert max(l1) <= 1e-5
+
+
+def test_config2_shear_12M():
+    m = M.kuhn_box(128, 128, 128, origin=(-0.5, -0.5, -0.5), periodic=7, seed=SEED)
+    assert m.n_cells == 12_582_912
+    W0 = M.init_state(m, M.INIT_SHEAR)
+    V = m.cells["V"]
+    U0, _, U64, st64 = _run(m, W0, 4.0 / 3.0, 200, capi.PREC_FP64, capi.GUESS_REFERENCE)
+    _clean(st64, "fp64")
+    _, _, U32, st32 = _run(m, W0, 4.0 / 3.0, 200, capi.PREC_MIXED, capi.GUESS_PREVIOUS)
+    _clean(st32, "mixed")
+    for label, U, bar in (("fp64", U64, 1e-13), ("mixed", U32, 1e-9)):
+        drift = _drift(U0, U, V)
+        print(label, "relative drift of (D, mx, my, mz, E) totals", drift, flush=True)
+        assert drift.max() <= bar  # every face is interior: each flux enters two cells with opposite signs
+    l1 = _rel_l1(U32, U64, V)
+    print("mixed vs fp64 rel L1", l1, flush=True)
+    assert max(l1) <= 1e-5
+
+
+def test_config2_fp64_graph_equals_eager_12M():
+    m = M.kuhn_box(128, 128, 128, origin=(-0.5, -0.5, -0.5), periodic=7, seed=SEED)
+    W0 = M.init_state(m, M.INIT_SHEAR)
+    _, Wg, Ug, _ = _run(m, W0, 4.0 / 3.0, 20, capi.PREC_FP64, capi.GUESS_REFERENCE, graph=True)
+    _, We, Ue, _ = _run(m, W0, 4.0 / 3.0, 20, capi.PREC_FP64, capi.GUESS_REFERENCE, graph=False)
+    assert np.array_equal(Ug, Ue) and np.array_equal(Wg, We)
+
+
+def test_config3_surrogate_in_shear_12M():
+    """The surrogate trained for gamma = 4/3 by the library's kernels starts Newton in the
+    config-2 shear layer: same residual bar, no fallback beyond the previous pressure's,
+    L1 against FP64 <= 1e-5; the iteration histograms of both starts are printed."""
+    model, info = surrogate.train(4.0 / 3.0)
+    print("surrogate", {k: info[k] for k in ("steps", "batch", "loss_first", "loss_last", "seconds")}, flush=True)
+    m = M.kuhn_box(128, 128, 128, origin=(-0.5, -0.5, -0.5), periodic=7, seed=SEED)
+    W0 = M.init_state(m, M.INIT_SHEAR)
+    V = m.cells["V"]
+    _, _, U64, _ = _run(m, W0, 4.0 / 3.0, 200, capi.PREC_FP64, capi.GUESS_REFERENCE)
+    res = {}
+    for guess in (capi.GUESS_PREVIOUS, capi.GUESS_MLP):
+        _, _, U, st = _run(m, W0, 4.0 / 3.0, 200, capi.PREC_MIXED, guess, model=model if guess == capi.GUESS_MLP else None)
+        _clean(st, f"guess {guess}")
+        hist = np.array(st["newton_hist"])
+        print(f"guess {guess}: Newton iteration histogram (bins 0..)", hist[:12].tolist(), "fallback cells",
+              st["fallback_cells"], "surrogate starts", st["mlp_starts"], flush=True)
+        l1 = _rel_l1(U, U64, V)
+        print(f"guess {guess}: rel L1 vs fp64", l1, flush=True)
+        assert max(l1) <= 1e-5
+        res[guess] = st
+    assert res[capi.GUESS_MLP]["mlp_starts"] > 0
+    assert res[capi.GUESS_MLP]["fallback_cells"] <= res[capi.GUESS_PREVIOUS]["fallback_cells"] + 0.001 * m.n_cells * 200
