@@ -60,7 +60,7 @@ def per_kernel_bytes(N, F, mixed, rad):
     """Algorithmic bytes per launch: each array element counted once per kernel
     that touches it, gathers once, at the storage precision of the variant."""
     if mixed:
-        face = 32 * N + 44 * F  # cell record read (v, p | s, D, cs2, ext); ids 8 + S*n 12 + packed flux write 24
+        face = 32 * N + 44 * F  # cell record read (v, p | s - D, D, cs2, ext); ids 8 + S*n 12 + packed flux write 24
         cell = 148 * N + 24 * F  # ids+signs 16 + (V, closure defect) 16 + U r/w 80 + p32 read 4 + record write 32; packed flux read
     else:
         face = 80 * N + 80 * F  # U+W read; ids 8 + (n,S) 32 + flux write 40
@@ -144,8 +144,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(
-            f"bench.py: --gpus {args.gpus} is not supported: this build measures one MI355X; "
-            "multi-GPU sharding is a design note only (DESIGN.md).",
+            f"bench.py: --gpus {args.gpus} is not supported: the step runs on one MI355X; "
+            "sharding across GPUs is designed but not built.",
             file=sys.stderr,
         )
         sys.exit(2)

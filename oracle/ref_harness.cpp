@@ -161,7 +161,7 @@ int64_t ref_convert_bin_to_geo(int64_t n_cells, const int32_t *nb, const double 
   return (int64_t)faces.size();
 }
 
-// The GEOMETRY_TYPE Cone states of this build: the default initial W of
+// The GEOMETRY_TYPE Cone states of the compiled reference: the default initial W of
 // rhllc_init.cpp:130-132 and the e_bound_out_source ghost of
 // rhllc_utils.cpp:80-84 (W, and U from rhllc::GetConvValue), from the
 // reference's own constants and GetPressure (lib/physics/include/gas_state.h:38-43).

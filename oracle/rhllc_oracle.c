@@ -2,7 +2,7 @@
  * rhllc_oracle.c -- CPU restatement of the reference's RHLLC step (FP64).
  *
  * TEST INFRASTRUCTURE ONLY: the checker that tests/, the smoke check and the
- * CPU-baseline leg of bench.py use to judge the HIP path.  The product never
+ * CPU-baseline leg of bench.py use to check the HIP path.  The product never
  * links, loads or calls this file.
  *
  * Build: gcc -O2 -fno-fast-math -ffp-contract=off -fopenmp (oracle/Makefile).

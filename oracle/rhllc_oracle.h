@@ -2,7 +2,7 @@
  * rhllc_oracle.h -- CPU restatement of the reference's RHLLC step.
  *
  * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke
- * check and the CPU-baseline leg of bench.py call it to judge the HIP path.
+ * check and the CPU-baseline leg of bench.py call it to check the HIP path.
  * The product (radrhd_solver_amd, librhllc_hip.so) never links or calls it.
  *
  * Record layouts are the reference's AoS records so that the CPU timing sees
