@@ -63,12 +63,13 @@ def build_hip() -> str:
 SHIM_DIR = os.path.join(PKG_DIR, "shim")
 SOLVER_EXE = os.path.join(PKG_DIR, "bin", "rhllc_solver")
 MAKE_GEO_EXE = os.path.join(PKG_DIR, "bin", "rhllc_make_geo")
+SOLVER_MPI_EXE = os.path.join(PKG_DIR, "bin", "rhllc_solver_mpi")
 DROPIN_LIB = os.path.join(PKG_DIR, "librhllc_dropin.so")
 
 
 def build_shim() -> str:
-    """The drop-in C++ interface (librhllc_dropin.so) and the rhllc_solver and
-    rhllc_make_geo executables."""
+    """The drop-in C++ interface (librhllc_dropin.so) and the rhllc_solver,
+    rhllc_solver_mpi (one rank) and rhllc_make_geo executables."""
     os.makedirs(os.path.dirname(SOLVER_EXE), exist_ok=True)
     inc = ["-I", os.path.join(os.path.dirname(PKG_DIR), "include")]
     link = ["-L", PKG_DIR, "-lrhllc_hip", "-lradrhd_mesh", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."]
@@ -76,6 +77,8 @@ def build_shim() -> str:
           "-o", DROPIN_LIB, os.path.join(SHIM_DIR, "rhllc_dropin.cpp")] + inc + link)
     exe_link = ["-L", PKG_DIR, "-lrhllc_dropin", "-lrhllc_hip", "-lradrhd_mesh", "-Wl,-rpath,$ORIGIN/.."]
     _run(["g++", "-std=c++17", "-O2", "-Wall", "-o", SOLVER_EXE, os.path.join(SHIM_DIR, "rhllc_solver_main.cpp")]
+         + inc + exe_link)
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-o", SOLVER_MPI_EXE, os.path.join(SHIM_DIR, "rhllc_solver_mpi_main.cpp")]
          + inc + exe_link)
     _run(["g++", "-std=c++17", "-O2", "-Wall", "-o", MAKE_GEO_EXE, os.path.join(SHIM_DIR, "rhllc_make_geo_main.cpp")]
          + inc + exe_link)

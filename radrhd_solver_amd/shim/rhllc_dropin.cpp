@@ -270,6 +270,70 @@ int RunRhllcModule() {
   return e_completion_success;
 }
 
+// rhllc_main_mpi.cpp:18-84 on one GPU, i.e. one rank: the partition file
+// (glb_files.base_address + "metis", ReadSimple text: count, then one rank id
+// per cell) must put every cell on rank 0.  rhllc_mpi::Hllc3dStab at one rank
+// is rhllc::Hllc3dStab (rhllc_3d_mpi.cpp:371-486: the same BoundConditions,
+// GetFluxStab, slot-order flux sum, c2p and PhysPressureFix), and the
+// solution is written by WriteSolutionMPI (flux.bin).
+int RunRhllcMpiModule() {
+  grid_t grid;
+  uint32_t err = 0;
+  err |= files_sys::bin::ReadGridGeo(glb_files.name_file_geometry_faces, grid.faces);
+  err |= files_sys::bin::ReadGridGeo(glb_files.name_file_geometry_cells, grid.cells);
+  if (err) {
+    std::fprintf(stderr, "Error reading \n");
+    return e_completion_fail;
+  }
+  grid.size = (IdType)grid.cells.size();
+  {
+    // Read before any device work; a partition over several ranks is an error return.
+    std::ifstream in(glb_files.base_address + "metis");
+    long n = -1;
+    if (!in || !(in >> n)) {
+      std::fprintf(stderr, "Error reading metis \n");
+      return e_completion_fail;
+    }
+    if (n != (long)grid.size) {
+      std::fprintf(stderr, "metis: %ld entries for %ld cells\n", n, (long)grid.size);
+      return e_completion_fail;
+    }
+    for (long i = 0; i < n; ++i) {
+      long r = -1;
+      if (!(in >> r) || r != 0) {
+        std::fprintf(stderr, "metis: cell %ld is not on rank 0; this step runs one rank\n", i);
+        return e_completion_fail;
+      }
+    }
+  }
+  if (rhllc::Init(glb_files.hllc_init_value, grid.cells)) die("rhllc::Init(glb_files.hllc_init_value, grid.cells)");
+  Type t = 0.0;
+  Type cur_timer = 0;
+  int res_count = _solve_mode.start_point;
+  _hllc_cfg.tau = GetTimeStep(_hllc_cfg, grid.cells);
+  files_sys::bin::WriteSolutionMPI(glb_files.solve_address + std::to_string(res_count++), grid);
+  const bool was_resident = g_dev.resident;
+  SetDeviceResident(true);
+  while (t < _hllc_cfg.T) {
+    Hllc3dStab(_hllc_cfg.tau, grid);
+    t += _hllc_cfg.tau;
+    cur_timer += _hllc_cfg.tau;
+    if (cur_timer >= _hllc_cfg.save_timer) {
+      SyncGridFromDevice(grid);
+      if (files_sys::bin::WriteSolutionMPI(glb_files.solve_address + std::to_string(res_count++), grid) !=
+          e_completion_success)
+        die("WriteSolutionMPI");
+      cur_timer = 0;
+    }
+    _hllc_cfg.tau = GetTimeStep(_hllc_cfg, grid.cells);
+  }
+  SyncGridFromDevice(grid);
+  files_sys::bin::WriteSolutionMPI(glb_files.solve_address + std::to_string(res_count++), grid);
+  SetDeviceResident(was_resident);
+  ReleaseDevice();
+  return e_completion_success;
+}
+
 }  // namespace rhllc
 
 namespace files_sys {
@@ -385,6 +449,18 @@ int WriteSolution(const std::string &main_dir, const grid_t &grid) {
             write("pressure.bin", pv + offsetof(flux_t, p), sizeof(Type)) &&
             write("velocity.bin", pv + offsetof(flux_t, v), sizeof(Vector3));
   return ok ? e_completion_success : e_completion_fail;
+}
+
+// lib/files_sys/src/writer_bin.cpp:120-170 at one rank: WriteSimpleMPI of
+// flux.bin (writer_bin.h:240-263) writes the int count, then each cell's
+// phys_val (MPI_phys_val_t: the 40-byte flux_t of an elem_t, mpi_ext.cpp:90).
+int WriteSolutionMPI(const std::string &main_dir, const grid_t &grid) {
+  std::FILE *f = std::fopen((main_dir + "flux.bin").c_str(), "wb");
+  if (!f) return e_completion_fail;
+  const int n = (int)grid.cells.size();
+  bool ok = std::fwrite(&n, sizeof(int), 1, f) == 1;
+  for (const auto &el : grid.cells) ok = ok && std::fwrite(&el.phys_val, sizeof(flux_t), 1, f) == 1;
+  return (std::fclose(f) == 0) && ok ? e_completion_success : e_completion_fail;
 }
 
 }  // namespace bin

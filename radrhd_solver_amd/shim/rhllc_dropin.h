@@ -8,10 +8,12 @@
 //   rhllc::HllcPhysToConv(cells)        solvers/rhllc/include/rhllc_utils.h:43
 //   rhllc::Init(file, cells)            solvers/rhllc/include/rhllc_init.h:32
 //   rhllc::RunRhllcModule()             solvers/rhllc/include/rhllc_main.h:21
+//   rhllc::RunRhllcMpiModule()          solvers/rhllc/include/rhllc_main.h:22 (one rank)
 //   files_sys::json::ReadStartSettings  lib/files_sys/include/reader_json.h:63-89
 //   files_sys::bin::ReadGridGeo / ReadHllcInit / WriteSolution
 //                                       lib/files_sys/include/reader_bin.h:113-135,232-252,
 //                                       lib/files_sys/src/writer_bin.cpp:56-114
+//   files_sys::bin::WriteSolutionMPI    lib/files_sys/src/writer_bin.cpp:120-170 (one rank)
 //   BuildDataFromVTK(glb_files)         lib/geometry/include/make_internal_format.h
 //   BinToGeo(base_address)              utils/template_functions/bin_files_to_geo.h:11-38
 // Where the reference aborts (DIE_IF), these abort too, after a message on
@@ -42,6 +44,7 @@ Type GetTimeStep(const hllc_value_t &hllc_set, const std::vector<elem_t> &cells)
 void HllcPhysToConv(std::vector<elem_t> &cells);
 int Init(std::string &file_init_value, std::vector<elem_t> &cells);
 int RunRhllcModule();
+int RunRhllcMpiModule();  // at one rank: the partition must put every cell on rank 0
 
 // Extensions of the drop-in (no reference counterpart).
 void SetDeviceResident(bool resident);
@@ -63,6 +66,7 @@ int ReadGridGeo(const std::string &name_file, std::vector<face_t> &faces);
 int ReadGridGeo(const std::string &name_file, std::vector<elem_t> &cells);
 int ReadHllcInit(const std::string &file_init_value, std::vector<elem_t> &cells);
 int WriteSolution(const std::string &main_dir, const grid_t &grid);
+int WriteSolutionMPI(const std::string &main_dir, const grid_t &grid);  // flux.bin, one rank
 }  // namespace bin
 }  // namespace files_sys
 
