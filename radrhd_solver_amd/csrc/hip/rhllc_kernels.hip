@@ -779,23 +779,20 @@ __global__ void k_eval_rad_opt(OpacityTable t, int64_t n, const double *W, const
   o[0] = c.T, o[1] = c.vel, o[2] = c.lorenz, o[3] = c.alpha, o[4] = c.betta;
 }
 
-// Plain copy, 16 B per thread per iteration (calibration of byte counters).
-// Four loads in flight per thread before their stores.
+// Plain copy (calibration of byte counters and the streaming-rate reference).
+// One float4 per thread over a grid that covers the array: on gfx950 this
+// streams faster (6.15 TB/s at 2 GiB) than grid-stride loops of a fixed grid,
+// with one or four loads in flight per thread (4.5-4.8 TB/s).
 __global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, int64_t n) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (; i + 3 * stride < n; i += 4 * stride) {
-    const float4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], e = src[i + 3 * stride];
-    dst[i] = a;
-    dst[i + stride] = b;
-    dst[i + 2 * stride] = c;
-    dst[i + 3 * stride] = e;
-  }
-  for (; i < n; i += stride) dst[i] = src[i];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
 }
 
 hipError_t launch_copy(const void *src, void *dst, int64_t n16, hipStream_t s) {
-  hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, s, (const float4 *)src, (float4 *)dst, n16);
+  if (n16 <= 0) return hipSuccess;
+  const int64_t blocks = (n16 + 255) / 256;
+  if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned int)blocks), dim3(256), 0, s, (const float4 *)src, (float4 *)dst, n16);
   return hipGetLastError();
 }
 
