@@ -79,6 +79,22 @@ def test_create_rejects_bad_parameters():
     assert _create(m, precision=0, guess=2)[0] == 1
 
 
+def test_create_rejects_empty_and_oversized_meshes():
+    # geo_face_t::id_l / id_r are 25- and 26-bit fields: 2^25 cells is the
+    # largest mesh the reference's face format can address.  The sizes are
+    # checked before either array is read.
+    m = M.kuhn_box(2, 2, 2, seed=1)
+    p = capi.Params()
+    p.gamma, p.cfl, p.h_min = 5 / 3, 0.4, 1e-3
+    lib = _lib.load_hip_lib()
+    h = C.c_void_p()
+    for n_cells, n_faces, what in ((0, m.n_faces, "n_cells"), (1 << 25, m.n_faces, "n_cells"),
+                                   (m.n_cells, 0, "n_faces"), (m.n_cells, 1 << 31, "n_faces")):
+        rc = lib.rhllc_create(C.byref(p), n_cells, m.cells.ctypes.data, n_faces, m.faces.ctypes.data, C.byref(h))
+        assert rc == 1 and what in lib.rhllc_last_error().decode()
+        assert not h.value
+
+
 def test_null_arguments_are_errors_not_crashes():
     lib = _lib.load_hip_lib()
     assert lib.rhllc_create(None, 1, None, 1, None, None) == 1

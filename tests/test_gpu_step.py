@@ -110,6 +110,33 @@ def test_step_shock_tube_200_steps_bitwise():
         assert np.abs(st.U[:, k] - Ug[:, k]).sum() <= 1e-12 * ref
 
 
+def test_step_smallest_meshes_bitwise():
+    """One and two hexes (6 and 12 tets, 18 and 34 faces): every cell touches the
+    boundary and one partial workgroup holds the whole mesh.  FP64: random states
+    with jumps of up to seven decades, 30 steps, eager and graph, bitwise.  Mixed:
+    states within a decade of each other, within 1e-5 L1 of the oracle."""
+    rng = np.random.default_rng(SEED)
+    for shape in ((1, 1, 1), (2, 1, 1)):
+        m = M.kuhn_box(*shape, seed=SEED)
+        W0 = random_prims(rng, m.n_cells, vmax=0.9)
+        for graph in (False, True):
+            st, (t_o, _, speeds), (Wg, Ug, stats) = _run_both(m, W0, 0.4, 30, graph=graph)
+            assert np.array_equal(st.U, Ug) and np.array_equal(st.W, Wg), (shape, graph)
+            assert stats["t"] == t_o and stats["max_speed"] == speeds[-1]
+        W0 = random_prims(rng, m.n_cells, vmax=0.9, rho=(-0.5, 0.5), p=(-0.5, 0.5))
+        st = O.State(m, W0, 5.0 / 3.0)
+        st.run(0.4, m.h_min(), 30)
+        s = capi.Solver(m, gamma=5.0 / 3.0, cfl=0.4, h_min=m.h_min(), precision=capi.PREC_MIXED,
+                        guess=capi.GUESS_PREVIOUS)
+        s.upload_W(W0)
+        s.run(30)
+        s.sync()
+        Wm, Um = s.download()
+        s.close()
+        for k in range(5):
+            assert np.abs(Um[:, k] - st.U[:, k]).sum() <= 1e-5 * np.abs(st.U[:, k]).sum(), (shape, k)
+
+
 def test_step_periodic_shear_bitwise_eager_and_graph():
     """Periodic shear layer (W ~ 10, gamma 4/3) on 12^3 hexes, eager and graph replay agree bitwise."""
     m = M.kuhn_box(12, 12, 12, origin=(-0.5, -0.5, -0.5), periodic=7, seed=SEED)
