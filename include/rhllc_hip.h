@@ -65,7 +65,7 @@ typedef struct rhllc_params {
   double h_min;       /* hllc_cfg.json "h_min" */
   int32_t precision;  /* RHLLC_PREC_* */
   int32_t c2p_guess;  /* RHLLC_GUESS_*; must be REFERENCE for RHLLC_PREC_FP64 parity */
-  int32_t reorder;    /* 1: relabel cells/faces in Morton order on the device (results are
+  int32_t reorder;    /* 1: relabel cells/faces in Hilbert order on the device (results are
                          returned in the caller's numbering either way) */
   int32_t device;     /* HIP device ordinal */
   int32_t collect_stats; /* 1: Newton histogram + iteration totals each step */

@@ -100,13 +100,14 @@ int rhllc_init_radiation(int64_t n_cells, uint64_t seed, double ur_lo, double ur
 
 /*
  * Relabel-only locality order: a permutation new->old of the cells sorted by
- * the Morton code of their centres, and new->old of the faces sorted by the
- * new index of their id_l cell (ties keep the old face order).  The mesh
+ * the Hilbert index of their centres (21 bits per axis, one common scale),
+ * and new->old of the faces sorted by the earlier new index of their two
+ * cells, id_l for a boundary face (ties keep the old face order).  The mesh
  * itself (orientation, slots, id_l/id_r roles) is not changed.
  */
-int rhllc_morton_order(int64_t n_cells, const void *geo_cells, int64_t n_faces,
-                       const void *geo_faces, int32_t *cell_new_to_old,
-                       int32_t *face_new_to_old);
+int rhllc_curve_order(int64_t n_cells, const void *geo_cells, int64_t n_faces,
+                      const void *geo_faces, int32_t *cell_new_to_old,
+                      int32_t *face_new_to_old);
 
 /*
  * Legacy-format VTK files of tetrahedral unstructured grids (no VTK library).

@@ -37,7 +37,7 @@ def load_mesh_lib() -> C.CDLL:
         lib.rhllc_mesh_h_min.restype = dbl
         lib.rhllc_init_state.argtypes = [i32, i64, vp, vp, u64, vp]
         lib.rhllc_init_radiation.argtypes = [i64, u64, dbl, dbl, dbl, dbl, vp, vp, vp, vp]
-        lib.rhllc_morton_order.argtypes = [i64, vp, i64, vp, vp, vp]
+        lib.rhllc_curve_order.argtypes = [i64, vp, i64, vp, vp, vp]
         lib.rhllc_file_count.argtypes = [C.c_char_p]
         lib.rhllc_file_count.restype = i64
         lib.rhllc_file_read_records.argtypes = [C.c_char_p, C.c_size_t, i64, vp]

@@ -1,7 +1,7 @@
 // C ABI of librhllc_hip.so (declarations and the reference interfaces each
 // entry point replaces: include/rhllc_hip.h).
 //
-// A handle owns the mesh and state in device memory (SoA, Morton-relabelled
+// A handle owns the mesh and state in device memory (SoA, Hilbert-relabelled
 // when asked), one HIP stream, and optionally a captured hipGraph of the step.
 #include <hip/hip_runtime.h>
 
@@ -192,7 +192,7 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
   h->cell_new_to_old.resize(N);
   h->face_new_to_old.resize(F);
   if (p.reorder) {
-    rhllc_mesh::morton_order_impl(N, geo_cells, F, geo_faces, h->cell_new_to_old.data(), h->face_new_to_old.data());
+    rhllc_mesh::curve_order_impl(N, geo_cells, F, geo_faces, h->cell_new_to_old.data(), h->face_new_to_old.data());
   } else {
     for (int32_t i = 0; i < N; ++i) h->cell_new_to_old[i] = i;
     for (int32_t i = 0; i < F; ++i) h->face_new_to_old[i] = i;

@@ -28,7 +28,7 @@ struct Scalars {
 };
 
 // Everything a kernel needs, passed by value.  Cells and faces are in the
-// device (Morton) numbering.  SoA: each per-cell / per-face quantity is a
+// device (Hilbert) numbering.  SoA: each per-cell / per-face quantity is a
 // 16/32-byte vector array plus a scalar array so that gathers are single
 // wide loads.
 struct Dev {

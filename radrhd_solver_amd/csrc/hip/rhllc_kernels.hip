@@ -19,7 +19,7 @@ namespace rhllc_dev {
 
 // Workgroups are dispatched round-robin over the 8 XCDs.  Give each XCD a
 // contiguous range of logical blocks so neighbouring faces/cells (close in the
-// Morton numbering) are gathered through the same L2.
+// Hilbert numbering) are gathered through the same L2.
 __device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nblk) {
   const uint32_t x = bid & 7u, k = bid >> 3;
   const uint32_t q = nblk >> 3, r = nblk & 7u;

@@ -33,10 +33,11 @@ inline bool fail(const std::string &msg) {
   return false;
 }
 
-// Cells sorted by the Morton code of their centres; faces by the new index of
-// their id_l cell, stable in the old face order (counting sort).
-void morton_order_impl(int64_t n_cells, const void *geo_cells, int64_t n_faces, const void *geo_faces,
-                       int32_t *cell_new_to_old, int32_t *face_new_to_old);
+// Cells sorted by the Hilbert index of their centres; faces by the earlier new
+// index of their two cells (id_l for a boundary face), stable in the old face
+// order (counting sort).
+void curve_order_impl(int64_t n_cells, const void *geo_cells, int64_t n_faces, const void *geo_faces,
+                      int32_t *cell_new_to_old, int32_t *face_new_to_old);
 
 inline int num_threads() {
 #ifdef _OPENMP

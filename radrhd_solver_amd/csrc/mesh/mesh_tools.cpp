@@ -1,4 +1,4 @@
-// Host mesh tools: h_min, initial states, radiation fields, Morton relabelling
+// Host mesh tools: h_min, initial states, radiation fields, Hilbert relabelling
 // and the "int32 n + n records" binary files of the reference.
 #include "mesh_common.h"
 
@@ -111,10 +111,10 @@ extern "C" int rhllc_init_radiation(int64_t n_cells, uint64_t seed, double ur_lo
   return 0;
 }
 
-extern "C" int rhllc_morton_order(int64_t n_cells, const void *geo_cells, int64_t n_faces, const void *geo_faces,
-                                  int32_t *cell_new_to_old, int32_t *face_new_to_old) {
+extern "C" int rhllc_curve_order(int64_t n_cells, const void *geo_cells, int64_t n_faces, const void *geo_faces,
+                                 int32_t *cell_new_to_old, int32_t *face_new_to_old) {
   if (!geo_cells || !geo_faces || !cell_new_to_old || !face_new_to_old) return fail("null array"), 1;
-  morton_order_impl(n_cells, geo_cells, n_faces, geo_faces, cell_new_to_old, face_new_to_old);
+  curve_order_impl(n_cells, geo_cells, n_faces, geo_faces, cell_new_to_old, face_new_to_old);
   return 0;
 }
 

@@ -146,12 +146,12 @@ def init_radiation(n_cells: int, seed: int = 20261202, ur=(1e-3, 1.0), kappa=(0.
     return Ur, Fr, ka, ks
 
 
-def morton_order(mesh: Mesh):
-    """(cell_new_to_old, face_new_to_old) relabelling permutations."""
+def curve_order(mesh: Mesh):
+    """(cell_new_to_old, face_new_to_old) relabelling permutations (Hilbert order)."""
     lib = load_mesh_lib()
     cp = np.zeros(mesh.n_cells, np.int32)
     fp = np.zeros(mesh.n_faces, np.int32)
-    rc = lib.rhllc_morton_order(
+    rc = lib.rhllc_curve_order(
         mesh.n_cells, mesh.cells.ctypes.data, mesh.n_faces, mesh.faces.ctypes.data, cp.ctypes.data, fp.ctypes.data
     )
     if rc != 0:

@@ -124,15 +124,31 @@ def test_initial_states():
     assert ka.min() >= 0.1 and ks.max() <= 10.0
 
 
-def test_morton_order_is_a_permutation_with_faces_grouped():
+def test_curve_order_is_a_permutation_with_faces_grouped():
     m = M.kuhn_box(6, 5, 4, seed=SEED)
-    cp, fp = M.morton_order(m)
+    cp, fp = M.curve_order(m)
     assert np.array_equal(np.sort(cp), np.arange(m.n_cells))
     assert np.array_equal(np.sort(fp), np.arange(m.n_faces))
     new_of_old = np.empty_like(cp)
     new_of_old[cp] = np.arange(m.n_cells)
-    idl = M.face_id_l(m.faces)
-    assert (np.diff(new_of_old[idl[fp]]) >= 0).all()
+    idl, idr = M.face_id_l(m.faces), M.face_id_r(m.faces)
+    owner = new_of_old[idl]
+    inner = idr >= 0
+    owner[inner] = np.minimum(owner[inner], new_of_old[idr[inner]])
+    assert (np.diff(owner[fp]) >= 0).all()
+    # ties keep the old face order
+    same = np.diff(owner[fp]) == 0
+    assert (np.diff(fp)[same] > 0).all()
+
+
+def test_curve_order_walks_between_neighbours():
+    # On a 16^3 box (24,576 tets of 1/16 edge) consecutive cells of a Hilbert
+    # order are neighbours or near it; a Morton order jumps by up to the box.
+    m = M.kuhn_box(16, 16, 16, seed=SEED)
+    cp, _ = M.curve_order(m)
+    x = m.cells["center"][cp]
+    step = np.linalg.norm(np.diff(x, axis=0), axis=1) * 16
+    assert step.max() < 2.0 and step.mean() < 0.6
 
 
 def test_generator_rejects_bad_boxes():
