@@ -73,7 +73,19 @@ typedef struct rhllc_params {
   double out_source_W[5], out_source_U[5];
   int32_t eos;            /* RHLLC_EOS_*; 0 (zero-initialised params) is the ideal gas.
                              RHLLC_GUESS_MLP needs RHLLC_EOS_IDEAL */
+  int32_t schedule;       /* RHLLC_SCHED_*; 0 (zero-initialised params) is the tiled step */
 } rhllc_params;
+
+/* Kernel schedule of the step; the results are the same bits either way on
+ * the FP64 path.  The mixed path differs between the two in the face flux
+ * only: the tiled step keeps it in FP64, the two-kernel step stores it packed
+ * (FP32 with 8-bit extensions). */
+#define RHLLC_SCHED_TILED 0      /* one kernel: each workgroup computes the faces of a tile of 256
+                                    cells (Hilbert order) into LDS, then updates the tile's cells
+                                    from there; faces shared with another tile are computed by
+                                    both; the state the faces read is double-buffered */
+#define RHLLC_SCHED_TWO_KERNEL 1 /* a face kernel writes every face flux to device memory, a cell
+                                    kernel gathers them (the reference's face loop / cell loop) */
 
 #define RHLLC_HIST_BINS 41 /* Newton iterations 0..40 */
 
@@ -118,6 +130,12 @@ int rhllc_step(rhllc_solver *h, double tau, double *max_speed);
  * the step.  Asynchronous on the handle's stream; rhllc_sync waits. */
 int rhllc_run(rhllc_solver *h, int32_t nsteps, int32_t use_graph);
 int rhllc_sync(rhllc_solver *h);
+
+/* Switch rhllc_params.collect_stats (Newton histogram, iteration total and
+ * residual maximum of every step) on or off for the following steps; the
+ * statistics kernels differ, so a captured graph is rebuilt.  No counterpart
+ * in the reference. */
+int rhllc_set_collect_stats(rhllc_solver *h, int32_t on);
 
 /* Set the device's next tau and simulated time (e.g. from hllc_cfg.json). */
 int rhllc_set_time(rhllc_solver *h, double tau, double t);
@@ -164,13 +182,44 @@ int rhllc_set_opacity_table(rhllc_solver *h, int32_t nx, int32_t ny, const doubl
  * RHLLC_ERR_C2P when a PhysPressureFix failed. */
 int rhllc_add_rad_flux(rhllc_solver *h, double tau, const float *Ur, const float *Fr3);
 
+/* The radiation pressure tensor Tr of each cell (9 FP32 values, row-major
+ * Tr(j, i) at [9 c + 3 j + i], caller's numbering), the illum module's
+ * grid.impuls.  With it the source uses D(j, i) = Tr(j, i) instead of the M1
+ * Eddington tensor, as the reference is compiled (radRHD_config.h:8,
+ * ISOTROPIC_INTENSITY 0; radRHD_utils.cpp:57-63, 80-86, 117-123, 137-143),
+ * including the product Tr(j, i) * Ur that it forms there.  NULL returns to
+ * the M1 closure.  The fused source of rhllc_step / rhllc_run uses it. */
+int rhllc_set_radiation_tensor(rhllc_solver *h, const float *Tr9);
+
+/* rhllc_add_rad_flux with the moments' pressure tensor Tr (9 FP32 per cell,
+ * as rhllc_set_radiation_tensor): AddRadFlux of the reference as compiled,
+ * ISOTROPIC_INTENSITY 0 (radRHD_utils.cpp:137-143). */
+int rhllc_add_rad_flux_tensor(rhllc_solver *h, double tau, const float *Ur, const float *Fr3, const float *Tr9);
+
 /* Average device time (ms) of each kernel of the step, measured with HIP
  * events around each launch on the handle's stream over nsteps eager steps:
- * out[0] face kernel, out[1] cell kernel, out[2] dt kernel, out[3] whole step. */
+ * out[0] face kernel (tiled step: the tile kernel, faces and cells),
+ * out[1] cell kernel and surrogate-list kernel (tiled step: the list kernel
+ * only), out[2] dt kernel, out[3] whole step. */
 int rhllc_time_kernels(rhllc_solver *h, int32_t nsteps, double out[4]);
 
-/* Sizes the handle moved per step by the algorithmic byte model (both
- * kernels), for roofline reporting. */
+/* The tiled step's layout of a mesh, computed on the host (no device is
+ * touched), for tests and tools; no counterpart in the reference.  Returns
+ * the tile count, the total of extra faces and the largest face count of a
+ * tile; each non-NULL array receives: cell_new_to_old[n_cells] and
+ * face_new_to_old[n_faces] (device numbering -> caller's), tile_meta
+ * [4 n_tiles] (first own face, own count, first extra, extra count),
+ * tile_extra[n_extra] (device face indices) and cell_slots[2 n_cells]
+ * (four 16-bit tile-local slots per cell in slot order, bit 15 when the cell
+ * is the face's id_r).  Call once with NULL arrays for the sizes. */
+int rhllc_tile_layout(int64_t n_cells, const void *geo_cells, int64_t n_faces, const void *geo_faces, int32_t reorder,
+                      int32_t *n_tiles, int64_t *n_extra, int32_t *max_slots, int32_t *cell_new_to_old,
+                      int32_t *face_new_to_old, int32_t *tile_meta, int32_t *tile_extra, uint32_t *cell_slots);
+
+/* Sizes, and the algorithmic bytes of one step for roofline reporting:
+ * the two-kernel structure's compulsory traffic at the variant's storage
+ * precision (FP64 233 N + 120 F, mixed 161 N + 64 F, plus 24 N for the
+ * radiation fields), whatever the schedule moves. */
 int rhllc_info(rhllc_solver *h, int64_t *n_cells, int64_t *n_faces, int64_t *device_bytes);
 
 /* Pointwise device evaluations over host arrays (n states, flux_t layout),
@@ -185,11 +234,14 @@ int rhllc_eval_c2p(int32_t precision, int32_t eos, double gamma, int64_t n, cons
 int rhllc_eval_p2c(int32_t eos, double gamma, int64_t n, double *W, double *U);
 int rhllc_eval_pressure_fix(int32_t eos, double gamma, int64_t n, double *U, double *W, int32_t *codes);
 int rhllc_eval_rad_source(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
-                          const double *ks, int32_t emission, double *G);
+                          const double *ks, int32_t emission, const float *Tr9, double *G);
 /* GetRadSourceOpt with the table's opacities: G (4 per state) and the cell
- * data (T, vel, lorenz, alpha, betta: 5 per state). */
+ * data (T, vel, lorenz, alpha, betta: 5 per state).  For both: Tr9 (9 FP32
+ * per state, row-major) selects the pressure-tensor branch, NULL the M1
+ * closure. */
 int rhllc_eval_rad_source_opt(int32_t nx, int32_t ny, const double header[6], const double *table, int64_t n,
-                              const double *W, const double *Ur, const double *Fr3, double *G, double *cell);
+                              const double *W, const double *Ur, const double *Fr3, const float *Tr9, double *G,
+                              double *cell);
 
 /* Calibration copy: reps launches of a plain vectorised copy kernel over
  * nbytes (read nbytes + write nbytes per launch); *gbps receives the mean

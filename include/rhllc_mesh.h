@@ -77,9 +77,11 @@ int rhllc_box_build(const rhllc_box_desc *d, void *geo_cells, void *geo_faces);
 /*
  * h_min as computed by utils/src/get_average_volume.cpp:31-46:
  * min over cells of (3/4) * V / (sum of the 4 face areas), the areas summed in
- * slot order.  Needs the face records to look up the areas.
+ * slot order.  Needs the face records to look up the areas.  The records are
+ * validated first (ids in range, slots agreeing with their faces); a
+ * malformed mesh returns -1 with the reason in rhllc_mesh_last_error().
  */
-double rhllc_mesh_h_min(int64_t n_cells, const void *geo_cells, const void *geo_faces);
+double rhllc_mesh_h_min(int64_t n_cells, const void *geo_cells, int64_t n_faces, const void *geo_faces);
 
 /* Initial states of the synthetic workloads (flux_t W records, n_cells*40 B). */
 #define RHLLC_INIT_SHOCK_TUBE 0  /* x<0.5: rho=10,p=13.33; else rho=1,p=1e-6; v=0 */

@@ -22,6 +22,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "liboracle_rhllc.so")
+ORACLE_FAST_LIB = os.path.join(HERE, "liboracle_rhllc_fast.so")  # -O3 timing build (cpu_baseline only)
+FAST_FLAGS = "gcc -std=c11 -O3 -fopenmp -fPIC"
 REF_LIB = os.path.join(HERE, "_ref", "libref_rhllc.so")
 REF_LIB_TAUB = os.path.join(HERE, "_ref", "libref_rhllc_taub.so")
 
@@ -65,7 +67,7 @@ _ref = {}
 
 def build(ref: bool = True) -> None:
     """Compile the oracle (and the reference build when the tree is present)."""
-    subprocess.run(["make", "-s", "-C", HERE, "liboracle_rhllc.so"], check=True)
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle_rhllc.so", "liboracle_rhllc_fast.so"], check=True)
     if ref and os.path.isdir("/root/reference/solvers/rhllc"):
         subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
 
@@ -74,12 +76,26 @@ def _p(a):
     return None if a is None else a.ctypes.data
 
 
-def lib():
-    global _orc
+_orc_fast = None
+
+
+def lib(fast: bool = False):
+    """The strict build (parity), or with fast=True the -O3 timing build."""
+    global _orc, _orc_fast
+    if fast:
+        if _orc_fast is None:
+            _orc_fast = _declare(ORACLE_FAST_LIB)
+        return _orc_fast
     if _orc is None:
-        if not os.path.exists(ORACLE_LIB):
-            raise RuntimeError("oracle/liboracle_rhllc.so is not built (make -C oracle)")
-        L = C.CDLL(ORACLE_LIB)
+        _orc = _declare(ORACLE_LIB)
+    return _orc
+
+
+def _declare(path):
+    if True:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built (make -C oracle)")
+        L = C.CDLL(path)
         vp, i64, dbl, i32 = C.c_void_p, C.c_int64, C.c_double, C.c_int
         L.orc_p2c_n.argtypes = [vp, i64, vp, vp]
         L.orc_c2p_n.argtypes = [vp, i64, vp, vp, vp, vp]
@@ -93,20 +109,19 @@ def lib():
         L.orc_run.restype = dbl
         L.orc_edd_tensor.argtypes = [i32, i32, dbl, vp]
         L.orc_edd_tensor.restype = dbl
-        L.orc_rad_source.argtypes = [vp, dbl, vp, dbl, dbl, i32, vp]
-        L.orc_run_rad.argtypes = [vp, i64, vp, i64, vp, dbl, dbl, i32, vp, vp, vp, vp, i32, vp, vp, vp, vp]
+        L.orc_rad_source.argtypes = [vp, dbl, vp, dbl, dbl, i32, vp, vp]
+        L.orc_run_rad.argtypes = [vp, i64, vp, i64, vp, dbl, dbl, i32, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp]
         L.orc_run_rad.restype = dbl
-        L.orc_step_rad.argtypes = [vp, i64, vp, i64, vp, dbl, vp, vp, vp, vp, i32, vp, vp]
+        L.orc_step_rad.argtypes = [vp, i64, vp, i64, vp, dbl, vp, vp, vp, vp, i32, vp, vp, vp]
         L.orc_step_rad.restype = dbl
         L.orc_table_read.argtypes = [C.c_char_p, vp, vp, vp, vp]
         L.orc_table_read.restype = i32
         L.orc_table_lookup.argtypes = [vp, dbl, dbl]
         L.orc_table_lookup.restype = dbl
         L.orc_cell_rad.argtypes = [vp, vp, vp]
-        L.orc_rad_source_opt.argtypes = [vp, vp, dbl, vp, vp]
+        L.orc_rad_source_opt.argtypes = [vp, vp, dbl, vp, vp, vp]
         L.orc_num_threads.restype = i32
-        _orc = L
-    return _orc
+        return L
 
 
 def _ref_path(eos):
@@ -202,12 +217,13 @@ def flux(UL, UR, WL, WR, nS4, gamma=5.0 / 3.0, eos=EOS_IDEAL):
 class State:
     """AoS elem_t / face_t records of the oracle for one mesh."""
 
-    def __init__(self, mesh, W0, gamma=5.0 / 3.0, eos=EOS_IDEAL):
+    def __init__(self, mesh, W0, gamma=5.0 / 3.0, eos=EOS_IDEAL, fast=False):
+        self.L = lib(fast)
         self.n_cells, self.n_faces = mesh.n_cells, mesh.n_faces
         self.cells = np.zeros(self.n_cells * ELEM_BYTES, np.uint8)
         self.faces = np.zeros(self.n_faces * FACE_BYTES, np.uint8)
         W0 = np.ascontiguousarray(W0, dtype=np.float64)
-        lib().orc_load(
+        self.L.orc_load(
             self.n_cells, mesh.cells.ctypes.data, W0.ctypes.data, self.n_faces, mesh.faces.ctypes.data,
             self.cells.ctypes.data, self.faces.ctypes.data,
         )
@@ -233,50 +249,56 @@ class State:
     def run(self, cfl, h_min, steps):
         taus = np.zeros(steps)
         speeds = np.zeros(steps)
-        t = lib().orc_run(
+        t = self.L.orc_run(
             C.byref(self.ph), self.n_cells, self.cells.ctypes.data, self.n_faces, self.faces.ctypes.data,
             cfl, h_min, steps, taus.ctypes.data, speeds.ctypes.data, C.byref(self.stats),
         )
         return t, taus, speeds
 
-    def run_rad(self, cfl, h_min, steps, Ur, Fr, ka=None, ks=None, emission=0, table=None):
-        """Coupled steps; with `table` (a Table) the opacities come from each cell's state."""
+    def run_rad(self, cfl, h_min, steps, Ur, Fr, ka=None, ks=None, emission=0, table=None, Tr=None):
+        """Coupled steps; with `table` (a Table) the opacities come from each cell's state; with
+        Tr (n x 3 x 3, FP32) the pressure-tensor branch instead of the M1 closure."""
         taus = np.zeros(steps)
         speeds = np.zeros(steps)
         if ka is None:
             ka = ks = np.zeros(self.n_cells, np.float32)
         Ur, Fr, ka, ks = (np.ascontiguousarray(a, dtype=np.float32) for a in (Ur, Fr, ka, ks))
+        T = None if Tr is None else np.ascontiguousarray(Tr, dtype=np.float32).reshape(self.n_cells, 9)
         tab = None if table is None else C.byref(table.c())
-        t = lib().orc_run_rad(
+        t = self.L.orc_run_rad(
             C.byref(self.ph), self.n_cells, self.cells.ctypes.data, self.n_faces, self.faces.ctypes.data,
             cfl, h_min, steps, Ur.ctypes.data, Fr.ctypes.data, ka.ctypes.data, ks.ctypes.data, emission,
-            tab, taus.ctypes.data, speeds.ctypes.data, C.byref(self.stats),
+            tab, _p(T), taus.ctypes.data, speeds.ctypes.data, C.byref(self.stats),
         )
         return t, taus, speeds
 
-    def step_rad(self, tau, Ur, Fr, ka=None, ks=None, emission=0, table=None):
+    def step_rad(self, tau, Ur, Fr, ka=None, ks=None, emission=0, table=None, Tr=None):
         """One coupled step (hydro, source, c2p); returns the step's max signal speed."""
         if ka is None:
             ka = ks = np.zeros(self.n_cells, np.float32)
         Ur, Fr, ka, ks = (np.ascontiguousarray(a, dtype=np.float32) for a in (Ur, Fr, ka, ks))
+        T = None if Tr is None else np.ascontiguousarray(Tr, dtype=np.float32).reshape(self.n_cells, 9)
         tab = None if table is None else C.byref(table.c())
-        return lib().orc_step_rad(
+        return self.L.orc_step_rad(
             C.byref(self.ph), self.n_cells, self.cells.ctypes.data, self.n_faces, self.faces.ctypes.data, tau,
-            Ur.ctypes.data, Fr.ctypes.data, ka.ctypes.data, ks.ctypes.data, emission, tab, C.byref(self.stats),
+            Ur.ctypes.data, Fr.ctypes.data, ka.ctypes.data, ks.ctypes.data, emission, tab, _p(T),
+            C.byref(self.stats),
         )
 
     def step(self, tau):
-        return lib().orc_step(
+        return self.L.orc_step(
             C.byref(self.ph), self.n_cells, self.cells.ctypes.data, self.n_faces, self.faces.ctypes.data, tau,
             C.byref(self.stats),
         )
 
 
-def rad_source(W, Ur, Fr, ka, ks, emission=0):
+def rad_source(W, Ur, Fr, ka, ks, emission=0, Tr=None):
     W = np.ascontiguousarray(W, dtype=np.float64).reshape(5)
     Fr = np.ascontiguousarray(Fr, dtype=np.float64).reshape(3)
+    T = None if Tr is None else np.ascontiguousarray(Tr, dtype=np.float32).reshape(9)
     G = np.zeros(4)
-    lib().orc_rad_source(W.ctypes.data, float(Ur), Fr.ctypes.data, float(ka), float(ks), emission, G.ctypes.data)
+    lib().orc_rad_source(W.ctypes.data, float(Ur), Fr.ctypes.data, float(ka), float(ks), emission, _p(T),
+                         G.ctypes.data)
     return G
 
 
@@ -327,12 +349,13 @@ def cell_rad(t, W):
     return out
 
 
-def rad_source_opt(W, cell, Ur, Fr):
+def rad_source_opt(W, cell, Ur, Fr, Tr=None):
     W = np.ascontiguousarray(W, dtype=np.float64).reshape(5)
     cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(5)
     Fr = np.ascontiguousarray(Fr, dtype=np.float64).reshape(3)
+    T = None if Tr is None else np.ascontiguousarray(Tr, dtype=np.float32).reshape(9)
     G = np.zeros(4)
-    lib().orc_rad_source_opt(W.ctypes.data, cell.ctypes.data, float(Ur), Fr.ctypes.data, G.ctypes.data)
+    lib().orc_rad_source_opt(W.ctypes.data, cell.ctypes.data, float(Ur), Fr.ctypes.data, _p(T), G.ctypes.data)
     return G
 
 

@@ -571,9 +571,17 @@ static inline double b_planck(double T) {
   return (T2 * T2) * kStefanBoltzmann / kPI;
 }
 
-/* radRHD_utils.cpp:44-101 with D = EddTensor(j, i, Ur, Fr) */
+/* D(j, i) of the source: EddTensor(j, i, Ur, Fr) (ISOTROPIC_INTENSITY == 1), or
+ * Tr(j, i) of the moments' pressure tensor (row-major FP32, Tr9[3 j + i]) as
+ * the reference is compiled, ISOTROPIC_INTENSITY 0 (radRHD_config.h:8,
+ * radRHD_utils.cpp:80-86 and 137-143). */
+static inline double src_D(int j, int i, double Ur, const double Fr[3], const float *Tr9) {
+  return Tr9 ? (double)Tr9[3 * j + i] : orc_edd_tensor(j, i, Ur, Fr);
+}
+
+/* radRHD_utils.cpp:44-101 */
 void orc_rad_source(const orc_flux_t *W, double Ur, const double Fr[3], double abs_op, double scat_op, int emission,
-                    double G[4]) {
+                    const float *Tr9, double G[4]) {
   const double tot_op = abs_op + scat_op;
   const double rho = W->d, prs = W->p;
   const double *v = W->v;
@@ -589,7 +597,7 @@ void orc_rad_source(const orc_flux_t *W, double Ur, const double Fr[3], double a
   for (int i = 0; i < 3; i++) {
     uD[i] = 0.;
     for (int j = 0; j < 3; j++) {
-      const double D = orc_edd_tensor(j, i, Ur, Fr);
+      const double D = src_D(j, i, Ur, Fr, Tr9);
       uD[i] += u[j] * D;
       uuD += u[i] * u[j] * D;
     }
@@ -704,7 +712,8 @@ void orc_cell_rad(const orc_table_t *t, const orc_flux_t *W, double cell[5]) {
 }
 
 /* radRHD_utils.cpp:103-158 */
-void orc_rad_source_opt(const orc_flux_t *W, const double cell[5], double Ur, const double Fr[3], double G[4]) {
+void orc_rad_source_opt(const orc_flux_t *W, const double cell[5], double Ur, const double Fr[3], const float *Tr9,
+                        double G[4]) {
   const double abs_op = cell[3], scat_op = cell[4];
   const double tot_op = abs_op + scat_op;
   const double rho = W->d;
@@ -719,7 +728,7 @@ void orc_rad_source_opt(const orc_flux_t *W, const double cell[5], double Ur, co
   for (int i = 0; i < 3; i++) {
     uD[i] = 0.;
     for (int j = 0; j < 3; j++) {
-      const double D = orc_edd_tensor(j, i, Ur, Fr);
+      const double D = src_D(j, i, Ur, Fr, Tr9);
       uD[i] += u[j] * D;
       uuD += u[i] * u[j] * D;
     }
@@ -737,19 +746,20 @@ void orc_rad_source_opt(const orc_flux_t *W, const double cell[5], double Ur, co
  * With a table: radRHD_mpi_main.cpp:112-122, AddRadFlux then HllcConvToPhys. */
 double orc_step_rad(const orc_physics_t *ph, int64_t n_cells, orc_elem_t *cells, int64_t n_faces, orc_face_t *faces,
                     double tau, const float *Ur, const float *Fr3, const float *ka, const float *ks, int emission,
-                    const orc_table_t *tab, orc_stats_t *st) {
+                    const orc_table_t *tab, const float *Tr9, orc_stats_t *st) {
   const double speed = orc_step(ph, n_cells, cells, n_faces, faces, tau, st);
   int64_t fix_fail = 0;
 #pragma omp parallel for schedule(static) reduction(+ : fix_fail)
   for (int64_t c = 0; c < n_cells; ++c) {
     double G[4];
     const double Fr[3] = {Fr3[3 * c], Fr3[3 * c + 1], Fr3[3 * c + 2]};
+    const float *T9 = Tr9 ? Tr9 + 9 * c : NULL;
     if (tab) {
       double cell[5];
       orc_cell_rad(tab, &cells[c].phys_val, cell);
-      orc_rad_source_opt(&cells[c].phys_val, cell, (double)Ur[c], Fr, G);
+      orc_rad_source_opt(&cells[c].phys_val, cell, (double)Ur[c], Fr, T9, G);
     } else {
-      orc_rad_source(&cells[c].phys_val, (double)Ur[c], Fr, (double)ka[c], (double)ks[c], emission, G);
+      orc_rad_source(&cells[c].phys_val, (double)Ur[c], Fr, (double)ka[c], (double)ks[c], emission, T9, G);
     }
     const double ds = 1;
     orc_flux_t *U = &cells[c].conv_val;
@@ -767,14 +777,14 @@ double orc_step_rad(const orc_physics_t *ph, int64_t n_cells, orc_elem_t *cells,
 
 double orc_run_rad(const orc_physics_t *ph, int64_t n_cells, orc_elem_t *cells, int64_t n_faces, orc_face_t *faces,
                    double cfl, double h_min, int steps, const float *Ur, const float *Fr3, const float *ka,
-                   const float *ks, int emission, const orc_table_t *tab, double *taus, double *speeds,
-                   orc_stats_t *st) {
+                   const float *ks, int emission, const orc_table_t *tab, const float *Tr9, double *taus,
+                   double *speeds, orc_stats_t *st) {
   phys_to_conv(ph, n_cells, cells);
   double speed = 1;
   double tau = cfl * h_min / speed;
   double t = 0;
   for (int k = 0; k < steps; ++k) {
-    speed = orc_step_rad(ph, n_cells, cells, n_faces, faces, tau, Ur, Fr3, ka, ks, emission, tab, st);
+    speed = orc_step_rad(ph, n_cells, cells, n_faces, faces, tau, Ur, Fr3, ka, ks, emission, tab, Tr9, st);
     if (taus) taus[k] = tau;
     if (speeds) speeds[k] = speed;
     t += tau;

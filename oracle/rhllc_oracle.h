@@ -106,12 +106,15 @@ void orc_load(int64_t n_cells, const void *geo_cells, const double *W, int64_t n
               orc_elem_t *cells, orc_face_t *faces);
 
 /* Radiation 4-momentum source (solvers/RadRHD/src/radRHD_utils.cpp:8-101).
- * Eddington tensor by the M1 closure (the ISOTROPIC_INTENSITY == 1 branch).
- * emission != 0 adds the thermal term q = -ka*rho*B(T)/kRadiation with the
- * reference's CGS constants; G[4] receives (G0, G1, G2, G3). */
+ * D(j, i) is the Eddington tensor by the M1 closure (the ISOTROPIC_INTENSITY
+ * == 1 branch) when Tr9 is NULL, else the pressure tensor Tr(j, i) =
+ * Tr9[3 j + i] (ISOTROPIC_INTENSITY 0, the reference's compiled setting,
+ * radRHD_config.h:8).  emission != 0 adds the thermal term
+ * q = -ka*rho*B(T)/kRadiation with the reference's CGS constants; G[4]
+ * receives (G0, G1, G2, G3). */
 double orc_edd_tensor(int i, int j, double U, const double F[3]);
 void orc_rad_source(const orc_flux_t *W, double Ur, const double Fr[3], double ka, double ks, int emission,
-                    double G[4]);
+                    const float *Tr9, double G[4]);
 /* TableFunc (lib/grids/include/solvers_struct.h:24-44): ln Lambda over
  * (ln D, ln T), data[i*ny + j]; ReadTableFunc (lib/files_sys/src/reader_txt.cpp:59-118)
  * fills it from tab_func/rcf_H.dat. */
@@ -129,8 +132,9 @@ double orc_table_lookup(const orc_table_t *t, double x, double y);
 /* full_phys_data_t::Init(const flux_t *) (lib/grids/src/solvers_struct.cpp:264-281):
  * cell[5] = (T, vel, lorenz, alpha, betta). */
 void orc_cell_rad(const orc_table_t *t, const orc_flux_t *W, double cell[5]);
-/* GetRadSourceOpt (radRHD_utils.cpp:103-158) with the M1 Eddington tensor. */
-void orc_rad_source_opt(const orc_flux_t *W, const double cell[5], double Ur, const double Fr[3], double G[4]);
+/* GetRadSourceOpt (radRHD_utils.cpp:103-158); Tr9 as for orc_rad_source. */
+void orc_rad_source_opt(const orc_flux_t *W, const double cell[5], double Ur, const double Fr[3], const float *Tr9,
+                        double G[4]);
 /* Coupled step of radRHD_main.cpp:99-132: hydro step, then U += tau*G(W),
  * then c2p (return code ignored, as there).  Ur..ks are per cell.  With a
  * table (tab != NULL; ka and ks unused) the source step is AddRadFlux
@@ -138,11 +142,11 @@ void orc_rad_source_opt(const orc_flux_t *W, const double cell[5], double Ur, co
  * HllcConvToPhys (rhllc_3d_mpi.cpp:488-505: c2p, PhysPressureFix on failure). */
 double orc_step_rad(const orc_physics_t *ph, int64_t n_cells, orc_elem_t *cells, int64_t n_faces,
                     orc_face_t *faces, double tau, const float *Ur, const float *Fr3, const float *ka,
-                    const float *ks, int emission, const orc_table_t *tab, orc_stats_t *st);
+                    const float *ks, int emission, const orc_table_t *tab, const float *Tr9, orc_stats_t *st);
 double orc_run_rad(const orc_physics_t *ph, int64_t n_cells, orc_elem_t *cells, int64_t n_faces,
                    orc_face_t *faces, double cfl, double h_min, int steps, const float *Ur, const float *Fr3,
-                   const float *ka, const float *ks, int emission, const orc_table_t *tab, double *taus,
-                   double *speeds, orc_stats_t *st);
+                   const float *ka, const float *ks, int emission, const orc_table_t *tab, const float *Tr9,
+                   double *taus, double *speeds, orc_stats_t *st);
 
 int orc_num_threads(void);
 

@@ -33,7 +33,7 @@ def load_mesh_lib() -> C.CDLL:
         i64, vp, dbl, i32, u64 = C.c_int64, C.c_void_p, C.c_double, C.c_int32, C.c_uint64
         lib.rhllc_box_sizes.argtypes = [vp, vp, vp]
         lib.rhllc_box_build.argtypes = [vp, vp, vp]
-        lib.rhllc_mesh_h_min.argtypes = [i64, vp, vp]
+        lib.rhllc_mesh_h_min.argtypes = [i64, vp, i64, vp]
         lib.rhllc_mesh_h_min.restype = dbl
         lib.rhllc_init_state.argtypes = [i32, i64, vp, vp, u64, vp]
         lib.rhllc_init_radiation.argtypes = [i64, u64, dbl, dbl, dbl, dbl, vp, vp, vp, vp]

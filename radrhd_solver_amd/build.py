@@ -17,7 +17,8 @@ from ._lib import HIP_LIB, MESH_LIB, PKG_DIR
 CSRC = os.path.join(PKG_DIR, "csrc")
 MESH_SRC = [
     os.path.join(CSRC, "mesh", f)
-    for f in ("kuhn_box.cpp", "mesh_tools.cpp", "curve_order.cpp", "vtk_legacy.cpp", "tet_geometry.cpp", "table_txt.cpp")
+    for f in ("kuhn_box.cpp", "mesh_tools.cpp", "curve_order.cpp", "mesh_check.cpp", "vtk_legacy.cpp",
+              "tet_geometry.cpp", "table_txt.cpp")
 ]
 HIP_SRC = [os.path.join(CSRC, "hip", f) for f in ("rhllc_kernels.hip", "rhllc_capi.hip", "mlp_kernels.hip", "mlp_capi.hip")]
 HIP_FLAGS = [
@@ -51,11 +52,15 @@ def build_hip() -> str:
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
         _run([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
         objs.append(obj)
-    order_obj = os.path.join(objdir, "curve_order.o")
-    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-c", os.path.join(CSRC, "mesh", "curve_order.cpp"), "-o", order_obj])
-    objs.append(order_obj)
+    # Host preparation of a handle's layout (validation, relabelling, tiles):
+    # g++ with OpenMP (libgomp, as the mesh library uses).
+    for src in (os.path.join(CSRC, "mesh", "curve_order.cpp"), os.path.join(CSRC, "mesh", "mesh_check.cpp"),
+                os.path.join(CSRC, "hip", "rhllc_layout.cpp")):
+        obj = os.path.join(objdir, os.path.splitext(os.path.basename(src))[0] + ".o")
+        _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC", "-c", src, "-o", obj])
+        objs.append(obj)
     tmp = HIP_LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lgomp"])
     os.replace(tmp, HIP_LIB)
     return HIP_LIB
 

@@ -12,6 +12,7 @@ import numpy as np
 PREC_FP64, PREC_MIXED = 0, 1
 GUESS_REFERENCE, GUESS_PREVIOUS, GUESS_MLP = 0, 1, 2
 EOS_IDEAL, EOS_TAUB = 0, 1
+SCHED_TILED, SCHED_TWO_KERNEL = 0, 1
 HIST_BINS = 41
 
 
@@ -44,6 +45,7 @@ class Params(C.Structure):
         ("out_source_W", C.c_double * 5),
         ("out_source_U", C.c_double * 5),
         ("eos", C.c_int32),
+        ("schedule", C.c_int32),
     ]
 
 
@@ -91,21 +93,25 @@ SIGNATURES = {
     "rhllc_run": (C.c_int, [_vp, _i32, _i32]),
     "rhllc_sync": (C.c_int, [_vp]),
     "rhllc_set_time": (C.c_int, [_vp, _dbl, _dbl]),
+    "rhllc_set_collect_stats": (C.c_int, [_vp, _i32]),
     "rhllc_get_stats": (C.c_int, [_vp, _vp]),
     "rhllc_reset_stats": (C.c_int, [_vp]),
     "rhllc_download_face_flux": (C.c_int, [_vp, _vp]),
     "rhllc_set_radiation": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "rhllc_set_radiation_emission": (C.c_int, [_vp, _i32]),
     "rhllc_add_rad_flux": (C.c_int, [_vp, _dbl, _vp, _vp]),
+    "rhllc_add_rad_flux_tensor": (C.c_int, [_vp, _dbl, _vp, _vp, _vp]),
+    "rhllc_set_radiation_tensor": (C.c_int, [_vp, _vp]),
     "rhllc_set_opacity_table": (C.c_int, [_vp, _i32, _i32, _vp, _vp]),
     "rhllc_time_kernels": (C.c_int, [_vp, _i32, _vp]),
     "rhllc_info": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "rhllc_tile_layout": (C.c_int, [_i64, _vp, _i64, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_eval_flux": (C.c_int, [_i32, _i32, _dbl, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_eval_c2p": (C.c_int, [_i32, _i32, _dbl, _i64, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_eval_p2c": (C.c_int, [_i32, _dbl, _i64, _vp, _vp]),
     "rhllc_eval_pressure_fix": (C.c_int, [_i32, _dbl, _i64, _vp, _vp, _vp]),
-    "rhllc_eval_rad_source": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp]),
-    "rhllc_eval_rad_source_opt": (C.c_int, [_i32, _i32, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "rhllc_eval_rad_source": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp]),
+    "rhllc_eval_rad_source_opt": (C.c_int, [_i32, _i32, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_device_query": (C.c_int, [_i32, C.c_char_p, _i32, _vp, _vp]),
     "rhllc_copy_probe": (C.c_int, [_i32, _i64, _i32, _vp]),
     "rhllc_mlp_train": (C.c_int, [_vp, _vp, _vp]),
@@ -149,7 +155,7 @@ class Solver:
 
     def __init__(self, mesh, gamma=5.0 / 3.0, cfl=0.4, h_min=None, precision=PREC_FP64,
                  guess=GUESS_REFERENCE, reorder=True, device=0, collect_stats=False, out_source=None,
-                 eos=EOS_IDEAL):
+                 eos=EOS_IDEAL, schedule=SCHED_TILED):
         L = _lib()
         p = Params()
         p.gamma, p.cfl = gamma, cfl
@@ -157,6 +163,7 @@ class Solver:
         p.precision, p.c2p_guess, p.reorder, p.device = precision, guess, int(reorder), device
         p.collect_stats = int(collect_stats)
         p.eos = eos
+        p.schedule = schedule
         if out_source is not None:
             p.has_out_source = 1
             for k in range(5):
@@ -208,6 +215,9 @@ class Solver:
     def set_time(self, tau: float, t: float = 0.0):
         check(_lib().rhllc_set_time(self.h, tau, t))
 
+    def set_collect_stats(self, on: bool):
+        check(_lib().rhllc_set_collect_stats(self.h, int(on)))
+
     def stats(self) -> dict:
         s = Stats()
         check(_lib().rhllc_get_stats(self.h, C.byref(s)))
@@ -230,11 +240,24 @@ class Solver:
         self._rad = arrs
         check(_lib().rhllc_set_radiation(self.h, *[None if a is None else a.ctypes.data for a in arrs]))
 
-    def add_rad_flux(self, tau, Ur, Fr):
-        """AddRadFlux + HllcConvToPhys with the table's opacities on the current state."""
+    def add_rad_flux(self, tau, Ur, Fr, Tr=None):
+        """AddRadFlux + HllcConvToPhys with the table's opacities on the current state; with
+        Tr (n x 3 x 3) the pressure-tensor branch (ISOTROPIC_INTENSITY 0)."""
         Ur = np.ascontiguousarray(Ur, dtype=np.float32)
         Fr = np.ascontiguousarray(Fr, dtype=np.float32)
-        check(_lib().rhllc_add_rad_flux(self.h, tau, Ur.ctypes.data, Fr.ctypes.data))
+        if Tr is None:
+            check(_lib().rhllc_add_rad_flux(self.h, tau, Ur.ctypes.data, Fr.ctypes.data))
+        else:
+            Tr = np.ascontiguousarray(Tr, dtype=np.float32).reshape(self.n_cells, 9)
+            check(_lib().rhllc_add_rad_flux_tensor(self.h, tau, Ur.ctypes.data, Fr.ctypes.data, Tr.ctypes.data))
+
+    def set_radiation_tensor(self, Tr):
+        """Per-cell radiation pressure tensor (n x 3 x 3) for the fused source; None: M1 closure."""
+        if Tr is None:
+            check(_lib().rhllc_set_radiation_tensor(self.h, None))
+            return
+        self._Tr = np.ascontiguousarray(Tr, dtype=np.float32).reshape(self.n_cells, 9)
+        check(_lib().rhllc_set_radiation_tensor(self.h, self._Tr.ctypes.data))
 
     def set_radiation_emission(self, on):
         check(_lib().rhllc_set_radiation_emission(self.h, 1 if on else 0))
@@ -264,6 +287,27 @@ class Solver:
         b = C.c_int64()
         check(_lib().rhllc_info(self.h, None, None, C.byref(b)))
         return b.value
+
+
+def tile_layout(mesh, reorder=True):
+    """The tiled step's host layout of a mesh (no device): a dict of arrays."""
+    L = _lib()
+    nt, ne, ms = C.c_int32(), C.c_int64(), C.c_int32()
+    args = (mesh.n_cells, mesh.cells.ctypes.data, mesh.n_faces, mesh.faces.ctypes.data, int(reorder))
+    check(L.rhllc_tile_layout(*args, C.byref(nt), C.byref(ne), C.byref(ms), None, None, None, None, None))
+    out = {
+        "cell_new_to_old": np.zeros(mesh.n_cells, np.int32),
+        "face_new_to_old": np.zeros(mesh.n_faces, np.int32),
+        "tile_meta": np.zeros((nt.value, 4), np.int32),
+        "tile_extra": np.zeros(max(ne.value, 1), np.int32),
+        "cell_slots": np.zeros((mesh.n_cells, 2), np.uint32),
+    }
+    check(L.rhllc_tile_layout(*args, C.byref(nt), C.byref(ne), C.byref(ms),
+                              *[out[k].ctypes.data for k in ("cell_new_to_old", "face_new_to_old", "tile_meta",
+                                                              "tile_extra", "cell_slots")]))
+    out["tile_extra"] = out["tile_extra"][:ne.value]
+    out["max_slots"] = ms.value
+    return out
 
 
 # ----------------------------------------------------------- pointwise evals
@@ -305,18 +349,23 @@ def eval_pressure_fix(U, W, gamma=5.0 / 3.0, eos=EOS_IDEAL):
     return U, W, codes
 
 
-def eval_rad_source(W, Ur, Fr, ka, ks, emission=False):
+def _tensor(Tr, n):
+    return None if Tr is None else np.ascontiguousarray(Tr, dtype=np.float32).reshape(n, 9)
+
+
+def eval_rad_source(W, Ur, Fr, ka, ks, emission=False, Tr=None):
     W = np.ascontiguousarray(W, dtype=np.float64).reshape(-1, 5)
     n = W.shape[0]
     Ur, ka, ks = (np.ascontiguousarray(x, dtype=np.float64).reshape(n) for x in (Ur, ka, ks))
     Fr = np.ascontiguousarray(Fr, dtype=np.float64).reshape(n, 3)
+    T = _tensor(Tr, n)
     G = np.zeros((n, 4))
     check(_lib().rhllc_eval_rad_source(n, W.ctypes.data, Ur.ctypes.data, Fr.ctypes.data, ka.ctypes.data,
-                                       ks.ctypes.data, 1 if emission else 0, G.ctypes.data))
+                                       ks.ctypes.data, 1 if emission else 0, _ptr(T), G.ctypes.data))
     return G
 
 
-def eval_rad_source_opt(table, W, Ur, Fr):
+def eval_rad_source_opt(table, W, Ur, Fr, Tr=None):
     """GetRadSourceOpt with the table's opacities: (G[n, 4], cell[n, 5] = (T, vel, lorenz, alpha, betta))."""
     hdr, data = table
     hdr = np.ascontiguousarray(hdr, dtype=np.float64).reshape(6)
@@ -325,10 +374,11 @@ def eval_rad_source_opt(table, W, Ur, Fr):
     n = W.shape[0]
     Ur = np.ascontiguousarray(Ur, dtype=np.float64).reshape(n)
     Fr = np.ascontiguousarray(Fr, dtype=np.float64).reshape(n, 3)
+    T = _tensor(Tr, n)
     G = np.zeros((n, 4))
     cell = np.zeros((n, 5))
     check(_lib().rhllc_eval_rad_source_opt(data.shape[0], data.shape[1], hdr.ctypes.data, data.ctypes.data, n,
-                                           W.ctypes.data, Ur.ctypes.data, Fr.ctypes.data, G.ctypes.data,
+                                           W.ctypes.data, Ur.ctypes.data, Fr.ctypes.data, _ptr(T), G.ctypes.data,
                                            cell.ctypes.data))
     return G, cell
 

@@ -16,6 +16,7 @@
 #include "mlp_internal.h"
 #include "rhllc_internal.h"
 #include "flux_ext.h"
+#include "rhllc_layout.h"
 
 using namespace rhllc_dev;
 
@@ -47,20 +48,42 @@ struct rhllc_solver {
   rhllc_params prm{};
   int32_t n_cells = 0, n_faces = 0;
   bool mixed = false;
+  bool tiled = true;
   int variant = kFp64Ref;
   bool has_state = false;
   bool rad = false;
   bool rad_kappa = false;  // rhllc_set_radiation was given kappa_a / kappa_s
   hipStream_t stream = nullptr;
   Dev d{};
+  // Buffers of the state.  [cur] holds the state at the start of the next
+  // step.  The tiled step reads [cur] and writes [cur ^ 1] for whatever the
+  // face work gathers (FP64: U, W; mixed: the cell records); arrays a cell
+  // only updates in place have [1] == [0], and so has every array of the
+  // two-kernel step.
+  int cur = 0;
+  int last_in = -1;  // buffer the last step read (the tiled step's face fluxes are recomputed from it)
+  double4 *U4b[2] = {nullptr, nullptr};
+  double *UEb[2] = {nullptr, nullptr};
+  double4 *W4b[2] = {nullptr, nullptr};
+  double *Wpb[2] = {nullptr, nullptr};
+  float4 *Wvab[2] = {nullptr, nullptr};
+  float4 *Wsdb[2] = {nullptr, nullptr};
+  int32_t n_tiles = 0, tile_slots = 0;
   std::vector<void *> allocs;
   std::vector<int32_t> cell_new_to_old, face_new_to_old;
   Scalars *host_sc = nullptr;  // pinned staging of the device scalars
+  unsigned long long fix_fail_seen = 0;  // fix_fail already reported by rhllc_step / rhllc_sync
   float4 *rad_FU = nullptr;
+  float4 *rad_FU_add = nullptr;  // moments of rhllc_add_rad_flux
+  float *rad_T = nullptr;        // pressure tensor of the fused source (rhllc_set_radiation_tensor)
+  float *rad_T_add = nullptr;    // pressure tensor of rhllc_add_rad_flux_tensor
   float2 *rad_k = nullptr;
   double *tab = nullptr;  // opacity table (nx*ny), owned
-  hipGraphExec_t graph = nullptr;
-  hipGraph_t graph_src = nullptr;
+  // Captured step graphs: the two-kernel step keeps one step in [0]; the
+  // tiled step keeps two steps starting from buffer k in [k], so a replay
+  // leaves the buffer parity as it was.
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  hipGraph_t graph_src[2] = {nullptr, nullptr};
   float *mlp_P = nullptr;
   short *mlp_W2b = nullptr;
 
@@ -74,10 +97,12 @@ struct rhllc_solver {
   }
 
   void drop_graph() {
-    if (graph) (void)hipGraphExecDestroy(graph);
-    if (graph_src) (void)hipGraphDestroy(graph_src);
-    graph = nullptr;
-    graph_src = nullptr;
+    for (int k = 0; k < 2; ++k) {
+      if (graph[k]) (void)hipGraphExecDestroy(graph[k]);
+      if (graph_src[k]) (void)hipGraphDestroy(graph_src[k]);
+      graph[k] = nullptr;
+      graph_src[k] = nullptr;
+    }
   }
 
   ~rhllc_solver() {
@@ -85,6 +110,9 @@ struct rhllc_solver {
     if (stream) (void)hipStreamSynchronize(stream);
     for (void *p : allocs) (void)hipFree(p);
     if (rad_FU) (void)hipFree(rad_FU);
+    if (rad_FU_add) (void)hipFree(rad_FU_add);
+    if (rad_T) (void)hipFree(rad_T);
+    if (rad_T_add) (void)hipFree(rad_T_add);
     if (rad_k) (void)hipFree(rad_k);
     if (tab) (void)hipFree(tab);
     if (mlp_P) (void)hipFree(mlp_P);
@@ -95,18 +123,62 @@ struct rhllc_solver {
 
   bool ready() const { return has_state && (variant != kMixedMlp || mlp_P != nullptr); }
 
-  // Cell kernel, plus the list kernel of the surrogate-start variant.
-  hipError_t enqueue_cells() {
-    hipError_t e = launch_cell(d, variant, rad, prm.collect_stats != 0, stream);
-    if (e == hipSuccess && variant == kMixedMlp) e = launch_cell_mlp(d, rad, prm.collect_stats != 0, stream);
+  // The kernel arguments with input buffers [in] and output buffers [out].
+  Dev dev(int in, int out) const {
+    Dev e = d;
+    e.U4 = U4b[in], e.UE = UEb[in], e.W4 = W4b[in], e.Wp = Wpb[in], e.Wva = Wvab[in], e.Wsd = Wsdb[in];
+    e.U4o = U4b[out], e.UEo = UEb[out], e.W4o = W4b[out], e.Wpo = Wpb[out], e.Wvao = Wvab[out], e.Wsdo = Wsdb[out];
     return e;
   }
+  // The current state, updated in place.
+  Dev now() const { return dev(cur, cur); }
+
+  // Cell kernel, plus the list kernel of the surrogate-start variant.
+  hipError_t enqueue_cells(const Dev &e) {
+    hipError_t r = launch_cell(e, variant, rad, prm.collect_stats != 0, stream);
+    if (r == hipSuccess && variant == kMixedMlp) r = launch_cell_mlp(e, rad, prm.collect_stats != 0, stream);
+    return r;
+  }
+  // One step from buffer cur; the tiled step flips cur.
   hipError_t enqueue_step() {
-    hipError_t e = launch_face(d, mixed, stream);
+    hipError_t e;
+    if (tiled) {
+      const Dev x = dev(cur, cur ^ 1);
+      e = launch_tile(x, mixed, variant, rad, prm.collect_stats != 0, n_tiles, tile_slots, stream);
+      if (e == hipSuccess && variant == kMixedMlp) e = launch_cell_mlp(x, rad, prm.collect_stats != 0, stream);
+      if (e == hipSuccess) e = launch_dt(x, stream);
+      if (e == hipSuccess) {
+        last_in = cur;
+        cur ^= 1;
+      }
+      return e;
+    }
+    const Dev x = now();
+    e = launch_face(x, mixed, stream);
+    if (e == hipSuccess) e = enqueue_cells(x);
+    if (e == hipSuccess) e = launch_dt(x, stream);
+    if (e == hipSuccess) last_in = cur;
+    return e;
+  }
+  // Captures graph[cur]: two steps (tiled) or one (two-kernel).
+  hipError_t capture() {
+    const int k = cur, saved_in = last_in;
+    hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
-    e = enqueue_cells();
-    if (e != hipSuccess) return e;
-    return launch_dt(d, stream);
+    for (int s = 0; s < (tiled ? 2 : 1) && e == hipSuccess; ++s) e = enqueue_step();
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(stream, &g);
+    cur = k;  // capture enqueues nothing; replays set the parity
+    last_in = saved_in;
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&graph[k], g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      graph[k] = nullptr;
+      return e;
+    }
+    graph_src[k] = g;
+    return hipSuccess;
   }
 };
 
@@ -127,6 +199,8 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
   if (!(p.cfl > 0.0) || !(p.h_min > 0.0)) return fail(RHLLC_ERR_ARG, "CFL and h_min must be > 0");
   if (p.precision != RHLLC_PREC_FP64 && p.precision != RHLLC_PREC_MIXED)
     return fail(RHLLC_ERR_ARG, "unknown precision");
+  if (p.schedule != RHLLC_SCHED_TILED && p.schedule != RHLLC_SCHED_TWO_KERNEL)
+    return fail(RHLLC_ERR_ARG, "unknown schedule");
   int variant;
   if (p.precision == RHLLC_PREC_FP64) {
     if (p.c2p_guess == RHLLC_GUESS_REFERENCE)
@@ -143,93 +217,34 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
     else
       return fail(RHLLC_ERR_ARG, "the mixed path supports the previous-pressure and MLP starts");
   }
+  const bool mixed = p.precision == RHLLC_PREC_MIXED, tiled = p.schedule == RHLLC_SCHED_TILED;
 
+  // Host layout (validation, relabelling, SoA arrays, tiles) before the device is touched.
+  rhllc_layout::Options opt;
+  opt.mixed = mixed;
+  opt.reorder = p.reorder != 0;
+  opt.tiled = tiled;
+  opt.has_out_source = p.has_out_source != 0;
+  opt.tile_cells = kTileCells;
+  rhllc_layout::Layout L;
+  {
+    std::string err;
+    if (!rhllc_layout::build(opt, n_cells, geo_cells, n_faces, geo_faces, L, err)) return fail(RHLLC_ERR_ARG, err);
+  }
   const int32_t N = (int32_t)n_cells, F = (int32_t)n_faces;
-  const uint8_t *gc = static_cast<const uint8_t *>(geo_cells);
-  const uint8_t *gf = static_cast<const uint8_t *>(geo_faces);
-
-  // Decode and validate so every device gather stays in bounds.
-  std::vector<int32_t> fl(F), fr(F);
-  for (int32_t f = 0; f < F; ++f) {
-    const uint8_t *r = gf + (size_t)f * RHLLC_GEO_FACE_BYTES;
-    const int32_t l = (int32_t)rhllc_mesh::face_id_l(r);
-    const int32_t rr = rhllc_mesh::face_id_r(r);
-    if (l >= N) return fail(RHLLC_ERR_ARG, "face " + std::to_string(f) + ": id_l out of range");
-    if (rr >= N) return fail(RHLLC_ERR_ARG, "face " + std::to_string(f) + ": id_r out of range");
-    if (rr < RHLLC_BOUND_OUTER_SURFACE)
-      return fail(RHLLC_ERR_ARG, "face " + std::to_string(f) + ": unknown boundary code " + std::to_string(rr));
-    if (rr == RHLLC_BOUND_OUT_SOURCE && !p.has_out_source)
-      return fail(RHLLC_ERR_ARG, "mesh has e_bound_out_source (-3) faces but no out_source state was given");
-    fl[f] = l;
-    fr[f] = rr;
-  }
-  for (int32_t c = 0; c < N; ++c) {
-    const uint8_t *r = gc + (size_t)c * RHLLC_GEO_CELL_BYTES;
-    int32_t ids[4];
-    rhllc_mesh::cell_ids(r, ids);
-    const uint8_t sign = r[48];
-    for (int j = 0; j < 4; ++j) {
-      if (ids[j] < 0 || ids[j] >= F) return fail(RHLLC_ERR_ARG, "cell " + std::to_string(c) + ": face id out of range");
-      const bool is_left = (sign >> j) & 1;
-      if (is_left ? fl[ids[j]] != c : fr[ids[j]] != c)
-        return fail(RHLLC_ERR_ARG, "cell " + std::to_string(c) + ": slot " + std::to_string(j) +
-                                       " disagrees with its face's id_l/id_r and sign bit");
-    }
-    if (!(rhllc_mesh::cell_V(r) > 0.0)) return fail(RHLLC_ERR_ARG, "cell " + std::to_string(c) + ": V <= 0");
-  }
 
   std::unique_ptr<rhllc_solver> h(new rhllc_solver());
   h->prm = p;
   h->n_cells = N;
   h->n_faces = F;
-  h->mixed = p.precision == RHLLC_PREC_MIXED;
+  h->mixed = mixed;
+  h->tiled = tiled;
   h->variant = variant;
   HIP_TRY(hipSetDevice(p.device));
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIP_TRY(hipHostMalloc((void **)&h->host_sc, sizeof(Scalars), hipHostMallocDefault));
-
-  // Relabel (or keep) the numbering.
-  h->cell_new_to_old.resize(N);
-  h->face_new_to_old.resize(F);
-  if (p.reorder) {
-    rhllc_mesh::curve_order_impl(N, geo_cells, F, geo_faces, h->cell_new_to_old.data(), h->face_new_to_old.data());
-  } else {
-    for (int32_t i = 0; i < N; ++i) h->cell_new_to_old[i] = i;
-    for (int32_t i = 0; i < F; ++i) h->face_new_to_old[i] = i;
-  }
-  std::vector<int32_t> cnew(N), fnew(F);
-  for (int32_t i = 0; i < N; ++i) cnew[h->cell_new_to_old[i]] = i;
-  for (int32_t i = 0; i < F; ++i) fnew[h->face_new_to_old[i]] = i;
-
-  std::vector<int2> lr(F);
-  std::vector<double4> nS(h->mixed ? 0 : F);
-  std::vector<float3> Sn32(h->mixed ? F : 0);
-  for (int32_t i = 0; i < F; ++i) {
-    const int32_t f = h->face_new_to_old[i];
-    const uint8_t *r = gf + (size_t)f * RHLLC_GEO_FACE_BYTES;
-    double n[3], S;
-    std::memcpy(n, r + 8, 24);
-    std::memcpy(&S, r + 32, 8);
-    const int32_t l = cnew[fl[f]];
-    // Copy-type boundaries (Cone geometry: every code but -3) read the cell itself.
-    const int32_t rr = fr[f] >= 0 ? cnew[fr[f]] : (fr[f] == RHLLC_BOUND_OUT_SOURCE ? -1 : l);
-    lr[i] = make_int2(l, rr);
-    if (h->mixed)
-      Sn32[i] = make_float3((float)(n[0] * S), (float)(n[1] * S), (float)(n[2] * S));
-    else
-      nS[i] = make_double4(n[0], n[1], n[2], S);
-  }
-  std::vector<int4> cf(N);
-  std::vector<double> V(N);
-  for (int32_t i = 0; i < N; ++i) {
-    const int32_t c = h->cell_new_to_old[i];
-    const uint8_t *r = gc + (size_t)c * RHLLC_GEO_CELL_BYTES;
-    int32_t ids[4], e[4];
-    rhllc_mesh::cell_ids(r, ids);
-    for (int j = 0; j < 4; ++j) e[j] = fnew[ids[j]] | (((r[48] >> j) & 1) ? 0 : (int32_t)0x80000000u);
-    cf[i] = make_int4(e[0], e[1], e[2], e[3]);
-    V[i] = rhllc_mesh::cell_V(r);
-  }
+  h->cell_new_to_old = std::move(L.cell_new_to_old);
+  h->face_new_to_old = std::move(L.face_new_to_old);
 
   Dev &d = h->d;
   d.n_cells = N;
@@ -240,61 +255,72 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
   d.cfl_hmin = p.cfl * p.h_min;
   for (int k = 0; k < 5; ++k) d.ghostW[k] = p.out_source_W[k], d.ghostU[k] = p.out_source_U[k];
 
+  // Uploads count elements of a host array into a new device array (at least
+  // one element is allocated; an empty array copies nothing).
+  auto upload = [&](auto **dst, const void *src, size_t count) -> hipError_t {
+    hipError_t e = h->alloc(dst, count ? count : 1);
+    if (e == hipSuccess && count) e = hipMemcpy(*dst, src, sizeof(**dst) * count, hipMemcpyHostToDevice);
+    return e;
+  };
   int2 *lr_d;
-  int4 *cf_d;
-  double *V_d;
-  HIP_TRY(h->alloc(&lr_d, F));
-  HIP_TRY(h->alloc(&cf_d, N));
-  HIP_TRY(hipMemcpy(lr_d, lr.data(), sizeof(int2) * F, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(cf_d, cf.data(), sizeof(int4) * N, hipMemcpyHostToDevice));
+  HIP_TRY(upload(&lr_d, L.lr.data(), F));
   d.face_lr = lr_d;
-  d.cell_faces = cf_d;
-  if (!h->mixed) {
-    HIP_TRY(h->alloc(&V_d, N));
-    HIP_TRY(hipMemcpy(V_d, V.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+  if (!mixed) {
+    double *V_d;
+    double4 *nS_d;
+    HIP_TRY(upload(&V_d, L.V.data(), N));
+    HIP_TRY(upload(&nS_d, L.nS.data(), F));
     d.cell_V = V_d;
+    d.face_nS = nS_d;
   } else {
-    // (V, delta): delta = sum over the cell's faces of +-S n as the face
-    // kernel sees it (FP32), summed in FP64; the cell kernel subtracts the
-    // cell's own flux through it.
-    std::vector<float4> Vd(N);
-    for (int32_t i = 0; i < N; ++i) {
-      const int ids[4] = {cf[i].x, cf[i].y, cf[i].z, cf[i].w};
-      double dl[3] = {0.0, 0.0, 0.0};
-      for (int j = 0; j < 4; ++j) {
-        const float3 q = Sn32[ids[j] & 0x7FFFFFFF];
-        const double sg = ids[j] >= 0 ? 1.0 : -1.0;
-        dl[0] += sg * (double)q.x, dl[1] += sg * (double)q.y, dl[2] += sg * (double)q.z;
-      }
-      Vd[i] = make_float4((float)V[i], (float)dl[0], (float)dl[1], (float)dl[2]);
-    }
     float4 *vd_d;
-    HIP_TRY(h->alloc(&vd_d, N));
-    HIP_TRY(hipMemcpy(vd_d, Vd.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
+    float3 *sn_d;
+    HIP_TRY(upload(&vd_d, L.Vd.data(), N));
+    HIP_TRY(upload(&sn_d, L.Sn32.data(), F));
     d.cell_Vd = vd_d;
+    d.face_Sn32 = sn_d;
   }
-  if (h->mixed) {
-    float3 *q;
-    HIP_TRY(h->alloc(&q, F));
-    HIP_TRY(hipMemcpy(q, Sn32.data(), sizeof(float3) * F, hipMemcpyHostToDevice));
-    d.face_Sn32 = q;
-    HIP_TRY(h->alloc(&d.F4f, F));
-    HIP_TRY(h->alloc(&d.FXf, F));
+  if (tiled) {
+    int4 *tm;
+    int32_t *te;
+    uint2 *cs;
+    HIP_TRY(upload(&tm, L.tile_meta.data(), L.n_tiles));
+    HIP_TRY(upload(&te, L.tile_extra.data(), L.tile_extra.size()));
+    HIP_TRY(upload(&cs, L.cell_slots.data(), N));
+    d.tile_meta = tm;
+    d.tile_extra = te;
+    d.cell_slots = cs;
+    h->n_tiles = L.n_tiles;
+    h->tile_slots = L.max_slots > 0 ? L.max_slots : 1;
   } else {
-    double4 *q;
-    HIP_TRY(h->alloc(&q, F));
-    HIP_TRY(hipMemcpy(q, nS.data(), sizeof(double4) * F, hipMemcpyHostToDevice));
-    d.face_nS = q;
-    HIP_TRY(h->alloc(&d.F4, F));
-    HIP_TRY(h->alloc(&d.FE, F));
+    int4 *cf_d;
+    HIP_TRY(upload(&cf_d, L.cf.data(), N));
+    d.cell_faces = cf_d;
+    if (mixed) {
+      HIP_TRY(h->alloc(&d.F4f, F));
+      HIP_TRY(h->alloc(&d.FXf, F));
+    } else {
+      HIP_TRY(h->alloc(&d.F4, F));
+      HIP_TRY(h->alloc(&d.FE, F));
+    }
   }
-  HIP_TRY(h->alloc(&d.U4, N));
-  HIP_TRY(h->alloc(&d.UE, N));
-  HIP_TRY(h->alloc(&d.W4, N));
-  HIP_TRY(h->alloc(&d.Wp, N));
-  if (h->mixed) {
-    HIP_TRY(h->alloc(&d.Wva, N));
-    HIP_TRY(h->alloc(&d.Wsd, N));
+  // State buffers: double-buffered where the tiled step gathers them.
+  const bool twoU = tiled && !mixed, twoRec = tiled && mixed;
+  for (int k = 0; k < 2; ++k) {
+    if (k == 0 || twoU) {
+      HIP_TRY(h->alloc(&h->U4b[k], N));
+      HIP_TRY(h->alloc(&h->UEb[k], N));
+      HIP_TRY(h->alloc(&h->W4b[k], N));
+      HIP_TRY(h->alloc(&h->Wpb[k], N));
+    } else {
+      h->U4b[k] = h->U4b[0], h->UEb[k] = h->UEb[0], h->W4b[k] = h->W4b[0], h->Wpb[k] = h->Wpb[0];
+    }
+    if (mixed && (k == 0 || twoRec)) {
+      HIP_TRY(h->alloc(&h->Wvab[k], N));
+      HIP_TRY(h->alloc(&h->Wsdb[k], N));
+    } else {
+      h->Wvab[k] = h->Wvab[0], h->Wsdb[k] = h->Wsdb[0];
+    }
   }
   if (variant == kMixedMlp) HIP_TRY(h->alloc(&d.mlp_list, N));
   HIP_TRY(h->alloc(&d.sc, 1));
@@ -303,7 +329,7 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
   h->host_sc->tau = d.cfl_hmin / 1.0;
   h->host_sc->last_speed = 1.0;
   HIP_TRY(hipMemcpy(d.sc, h->host_sc, sizeof(Scalars), hipMemcpyHostToDevice));
-  HIP_TRY(launch_clear_scalars(d, h->stream));
+  HIP_TRY(launch_clear_scalars(h->now(), h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   *out = h.release();
   return RHLLC_OK;
@@ -320,12 +346,16 @@ extern "C" int rhllc_upload_state(rhllc_solver *h, const void *recs, int64_t str
   if (off_phys < 0 || stride < 40 || off_phys + 40 > stride || (off_conv >= 0 && off_conv + 40 > stride))
     return fail(RHLLC_ERR_ARG, "bad record stride/offsets");
   HIP_TRY(hipSetDevice(h->prm.device));
+  // The copies below are synchronous with the host but not with the
+  // handle's (non-blocking) stream: finish what rhllc_run left in flight.
+  HIP_TRY(hipStreamSynchronize(h->stream));
   const int32_t N = h->n_cells;
   const uint8_t *base = static_cast<const uint8_t *>(recs);
   std::vector<double4> w4(N), u4(off_conv >= 0 ? N : 0);
   std::vector<double> wp(N), ue(off_conv >= 0 ? N : 0);
+  const int32_t *order = h->cell_new_to_old.data();
   for (int32_t i = 0; i < N; ++i) {
-    const uint8_t *r = base + (size_t)h->cell_new_to_old[i] * (size_t)stride;
+    const uint8_t *r = base + (size_t)order[i] * (size_t)stride;
     double w[5];
     std::memcpy(w, r + off_phys, 40);
     w4[i] = make_double4(w[0], w[1], w[2], w[3]);
@@ -337,7 +367,7 @@ extern "C" int rhllc_upload_state(rhllc_solver *h, const void *recs, int64_t str
       ue[i] = u[4];
     }
   }
-  Dev &d = h->d;
+  const Dev d = h->now();
   HIP_TRY(hipMemcpy(d.W4, w4.data(), sizeof(double4) * N, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(d.Wp, wp.data(), sizeof(double) * N, hipMemcpyHostToDevice));
   if (off_conv >= 0) {
@@ -358,6 +388,7 @@ extern "C" int rhllc_upload_state(rhllc_solver *h, const void *recs, int64_t str
     HIP_TRY(hipMemcpy(d.sc, h->host_sc, sizeof(Scalars), hipMemcpyHostToDevice));
   }
   h->has_state = true;
+  h->last_in = -1;
   return RHLLC_OK;
 }
 
@@ -370,8 +401,9 @@ extern "C" int rhllc_download_state(rhllc_solver *h, void *recs, int64_t stride,
   HIP_TRY(hipSetDevice(h->prm.device));
   HIP_TRY(hipStreamSynchronize(h->stream));
   const int32_t N = h->n_cells;
-  const Dev &d = h->d;
+  const Dev d = h->now();
   uint8_t *base = static_cast<uint8_t *>(recs);
+  const int32_t *order = h->cell_new_to_old.data();
   if (off_phys >= 0) {
     if (h->mixed) {
       // The mixed step keeps W in FP32; rebuild FP64 W from the FP64 U.
@@ -384,7 +416,7 @@ extern "C" int rhllc_download_state(rhllc_solver *h, void *recs, int64_t stride,
     HIP_TRY(hipMemcpy(w4.data(), d.W4, sizeof(double4) * N, hipMemcpyDeviceToHost));
     for (int32_t i = 0; i < N; ++i) {
       const double w[5] = {w4[i].x, w4[i].y, w4[i].z, w4[i].w, wp[i]};
-      std::memcpy(base + (size_t)h->cell_new_to_old[i] * stride + off_phys, w, 40);
+      std::memcpy(base + (size_t)order[i] * stride + off_phys, w, 40);
     }
   }
   if (off_conv >= 0) {
@@ -394,9 +426,21 @@ extern "C" int rhllc_download_state(rhllc_solver *h, void *recs, int64_t stride,
     HIP_TRY(hipMemcpy(ue.data(), d.UE, sizeof(double) * N, hipMemcpyDeviceToHost));
     for (int32_t i = 0; i < N; ++i) {
       const double u[5] = {u4[i].x, u4[i].y, u4[i].z, u4[i].w, ue[i]};
-      std::memcpy(base + (size_t)h->cell_new_to_old[i] * stride + off_conv, u, 40);
+      std::memcpy(base + (size_t)order[i] * stride + off_conv, u, 40);
     }
   }
+  return RHLLC_OK;
+}
+
+// PhysPressureFix failures and a non-finite speed since the last report.
+static int report_step_errors(rhllc_solver *h) {
+  const unsigned long long ff = h->host_sc->fix_fail;
+  const unsigned long long before = h->fix_fail_seen;
+  h->fix_fail_seen = ff;
+  if (ff > before)
+    return fail(RHLLC_ERR_C2P, std::to_string(ff - before) + " cells failed PhysPressureFix");
+  if (!std::isfinite(h->host_sc->last_speed) || !std::isfinite(h->host_sc->tau))
+    return fail(RHLLC_ERR_NUMERIC, "non-finite max signal speed");
   return RHLLC_OK;
 }
 
@@ -404,7 +448,6 @@ extern "C" int rhllc_step(rhllc_solver *h, double tau, double *max_speed) {
   if (!h) return fail(RHLLC_ERR_ARG, "null handle");
   if (!h->ready()) return fail(RHLLC_ERR_STATE, "no state uploaded (or no MLP model set for RHLLC_GUESS_MLP)");
   HIP_TRY(hipSetDevice(h->prm.device));
-  const unsigned long long fix_fail_before = h->host_sc->fix_fail;
   h->host_sc->tau = tau;
   HIP_TRY(hipMemcpyAsync(&h->d.sc->tau, &h->host_sc->tau, sizeof(double), hipMemcpyHostToDevice, h->stream));
   HIP_TRY(h->enqueue_step());
@@ -412,11 +455,7 @@ extern "C" int rhllc_step(rhllc_solver *h, double tau, double *max_speed) {
   HIP_TRY(hipStreamSynchronize(h->stream));
   const double s = h->host_sc->last_speed;
   if (max_speed) *max_speed = s;
-  if (!std::isfinite(s)) return fail(RHLLC_ERR_NUMERIC, "non-finite max signal speed");
-  if (h->host_sc->fix_fail > fix_fail_before)
-    return fail(RHLLC_ERR_C2P, std::to_string(h->host_sc->fix_fail - fix_fail_before) +
-                                   " cells failed PhysPressureFix in this step");
-  return RHLLC_OK;
+  return report_step_errors(h);
 }
 
 extern "C" int rhllc_run(rhllc_solver *h, int32_t nsteps, int32_t use_graph) {
@@ -428,20 +467,14 @@ extern "C" int rhllc_run(rhllc_solver *h, int32_t nsteps, int32_t use_graph) {
     for (int32_t k = 0; k < nsteps; ++k) HIP_TRY(h->enqueue_step());
     return RHLLC_OK;
   }
-  if (!h->graph) {
-    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = h->enqueue_step();
-    hipGraph_t g = nullptr;
-    hipError_t e2 = hipStreamEndCapture(h->stream, &g);
-    if (e != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      return hip_fail(e, "step capture");
-    }
-    if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
-    h->graph_src = g;
-    HIP_TRY(hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
+  const int per = h->tiled ? 2 : 1;
+  const int32_t reps = nsteps / per;
+  if (reps > 0) {
+    if (!h->graph[h->cur]) HIP_TRY(h->capture());
+    for (int32_t k = 0; k < reps; ++k) HIP_TRY(hipGraphLaunch(h->graph[h->cur], h->stream));
+    h->last_in = h->tiled ? h->cur ^ 1 : h->cur;
   }
-  for (int32_t k = 0; k < nsteps; ++k) HIP_TRY(hipGraphLaunch(h->graph, h->stream));
+  for (int32_t k = reps * per; k < nsteps; ++k) HIP_TRY(h->enqueue_step());
   return RHLLC_OK;
 }
 
@@ -449,6 +482,16 @@ extern "C" int rhllc_sync(rhllc_solver *h) {
   if (!h) return fail(RHLLC_ERR_ARG, "null handle");
   HIP_TRY(hipSetDevice(h->prm.device));
   HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipMemcpy(h->host_sc, h->d.sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+  return report_step_errors(h);
+}
+
+extern "C" int rhllc_set_collect_stats(rhllc_solver *h, int32_t on) {
+  if (!h) return fail(RHLLC_ERR_ARG, "null handle");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if ((h->prm.collect_stats != 0) != (on != 0)) h->drop_graph();  // other kernels
+  h->prm.collect_stats = on != 0;
   return RHLLC_OK;
 }
 
@@ -489,8 +532,9 @@ extern "C" int rhllc_get_stats(rhllc_solver *h, rhllc_stats *out) {
 extern "C" int rhllc_reset_stats(rhllc_solver *h) {
   if (!h) return fail(RHLLC_ERR_ARG, "null handle");
   HIP_TRY(hipSetDevice(h->prm.device));
-  HIP_TRY(launch_clear_scalars(h->d, h->stream));
+  HIP_TRY(launch_clear_scalars(h->now(), h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
+  h->fix_fail_seen = 0;
   return RHLLC_OK;
 }
 
@@ -499,21 +543,40 @@ extern "C" int rhllc_download_face_flux(rhllc_solver *h, double *F) {
   HIP_TRY(hipSetDevice(h->prm.device));
   HIP_TRY(hipStreamSynchronize(h->stream));
   const int32_t nf = h->n_faces;
-  if (!h->mixed) {
-    std::vector<double4> a(nf);
-    std::vector<double> e(nf);
+  if (h->last_in < 0) {  // no step since the state was set
+    std::memset(F, 0, sizeof(double) * 5 * (size_t)nf);
+    return RHLLC_OK;
+  }
+  std::vector<double4> a(nf);
+  std::vector<double> e(nf);
+  if (h->tiled) {
+    // The tiled step keeps its fluxes in LDS; they are recomputed, to the same
+    // bits, from the state the last step read (its input buffer is intact).
+    double4 *F4 = nullptr;
+    double *FE = nullptr;
+    HIP_TRY(hipMalloc((void **)&F4, sizeof(double4) * nf));
+    hipError_t r = hipMalloc((void **)&FE, sizeof(double) * nf);
+    if (r == hipSuccess) r = launch_face_fp64_out(h->dev(h->last_in, h->last_in), h->mixed, F4, FE, h->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(h->stream);
+    if (r == hipSuccess) r = hipMemcpy(a.data(), F4, sizeof(double4) * nf, hipMemcpyDeviceToHost);
+    if (r == hipSuccess) r = hipMemcpy(e.data(), FE, sizeof(double) * nf, hipMemcpyDeviceToHost);
+    (void)hipFree(F4);
+    if (FE) (void)hipFree(FE);
+    HIP_TRY(r);
+  } else if (!h->mixed) {
     HIP_TRY(hipMemcpy(a.data(), h->d.F4, sizeof(double4) * nf, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(e.data(), h->d.FE, sizeof(double) * nf, hipMemcpyDeviceToHost));
-    for (int32_t i = 0; i < nf; ++i) {
-      double *o = F + 5 * (size_t)h->face_new_to_old[i];
-      o[0] = a[i].x, o[1] = a[i].y, o[2] = a[i].z, o[3] = a[i].w, o[4] = e[i];
-    }
   } else {
-    std::vector<float4> a(nf);
-    std::vector<uint2> e(nf);
-    HIP_TRY(hipMemcpy(a.data(), h->d.F4f, sizeof(float4) * nf, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(e.data(), h->d.FXf, sizeof(uint2) * nf, hipMemcpyDeviceToHost));
-    for (int32_t i = 0; i < nf; ++i) rhllc_ext::unpack(a[i], e[i], F + 5 * (size_t)h->face_new_to_old[i]);
+    std::vector<float4> pa(nf);
+    std::vector<uint2> pe(nf);
+    HIP_TRY(hipMemcpy(pa.data(), h->d.F4f, sizeof(float4) * nf, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(pe.data(), h->d.FXf, sizeof(uint2) * nf, hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < nf; ++i) rhllc_ext::unpack(pa[i], pe[i], F + 5 * (size_t)h->face_new_to_old[i]);
+    return RHLLC_OK;
+  }
+  for (int32_t i = 0; i < nf; ++i) {
+    double *o = F + 5 * (size_t)h->face_new_to_old[i];
+    o[0] = a[i].x, o[1] = a[i].y, o[2] = a[i].z, o[3] = a[i].w, o[4] = e[i];
   }
   return RHLLC_OK;
 }
@@ -550,7 +613,32 @@ extern "C" int rhllc_set_radiation(rhllc_solver *h, const float *Ur, const float
   return RHLLC_OK;
 }
 
-extern "C" int rhllc_add_rad_flux(rhllc_solver *h, double tau, const float *Ur, const float *Fr3) {
+// Per-cell FP32 tensors (9 values, caller's numbering) into device order.
+static int upload_tensor(rhllc_solver *h, const float *Tr9, float **dst) {
+  const int32_t N = h->n_cells;
+  std::vector<float> t(9 * (size_t)N);
+  for (int32_t i = 0; i < N; ++i)
+    std::memcpy(&t[9 * (size_t)i], Tr9 + 9 * (size_t)h->cell_new_to_old[i], 9 * sizeof(float));
+  if (!*dst) HIP_TRY(hipMalloc((void **)dst, sizeof(float) * 9 * N));
+  HIP_TRY(hipMemcpy(*dst, t.data(), sizeof(float) * 9 * N, hipMemcpyHostToDevice));
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_set_radiation_tensor(rhllc_solver *h, const float *Tr9) {
+  if (!h) return fail(RHLLC_ERR_ARG, "null handle");
+  HIP_TRY(hipSetDevice(h->prm.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->drop_graph();  // the graph holds the kernel arguments by value
+  if (!Tr9) {
+    h->d.rad_T = nullptr;
+    return RHLLC_OK;
+  }
+  if (int e = upload_tensor(h, Tr9, &h->rad_T)) return e;
+  h->d.rad_T = h->rad_T;
+  return RHLLC_OK;
+}
+
+static int add_rad_flux(rhllc_solver *h, double tau, const float *Ur, const float *Fr3, const float *Tr9) {
   if (!h) return fail(RHLLC_ERR_ARG, "null handle");
   if (!Ur || !Fr3) return fail(RHLLC_ERR_ARG, "null radiation array");
   if (!(tau >= 0.0) || !std::isfinite(tau)) return fail(RHLLC_ERR_ARG, "tau must be finite and >= 0");
@@ -564,18 +652,37 @@ extern "C" int rhllc_add_rad_flux(rhllc_solver *h, double tau, const float *Ur, 
     const int32_t c = h->cell_new_to_old[i];
     fu[i] = make_float4(Fr3[3 * (size_t)c], Fr3[3 * (size_t)c + 1], Fr3[3 * (size_t)c + 2], Ur[c]);
   }
-  if (!h->rad_FU) HIP_TRY(hipMalloc((void **)&h->rad_FU, sizeof(float4) * N));
-  HIP_TRY(hipMemcpy(h->rad_FU, fu.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
-  h->d.rad_FU = h->rad_FU;
+  // Its own buffers: the moments of rhllc_set_radiation (the step's fused
+  // source, also held by a captured graph) stay as they were.
+  if (!h->rad_FU_add) HIP_TRY(hipMalloc((void **)&h->rad_FU_add, sizeof(float4) * N));
+  HIP_TRY(hipMemcpy(h->rad_FU_add, fu.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
+  Dev x = h->now();
+  x.rad_FU = h->rad_FU_add;
+  x.rad_T = nullptr;
+  if (Tr9) {
+    if (int e = upload_tensor(h, Tr9, &h->rad_T_add)) return e;
+    x.rad_T = h->rad_T_add;
+  }
   HIP_TRY(hipMemcpy(h->host_sc, h->d.sc, sizeof(Scalars), hipMemcpyDeviceToHost));
   const unsigned long long fix_fail_before = h->host_sc->fix_fail;
-  HIP_TRY(launch_add_rad(h->d, h->mixed, tau, h->stream));
+  HIP_TRY(launch_add_rad(x, h->mixed, tau, h->stream));
   HIP_TRY(hipMemcpyAsync(h->host_sc, h->d.sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
-  if (h->host_sc->fix_fail > fix_fail_before)
-    return fail(RHLLC_ERR_C2P, std::to_string(h->host_sc->fix_fail - fix_fail_before) +
-                                   " cells failed PhysPressureFix after the radiation source");
+  const unsigned long long now_failed = h->host_sc->fix_fail - fix_fail_before;
+  h->fix_fail_seen += now_failed;  // reported here; earlier steps' failures stay for rhllc_sync
+  if (now_failed)
+    return fail(RHLLC_ERR_C2P, std::to_string(now_failed) + " cells failed PhysPressureFix after the radiation source");
   return RHLLC_OK;
+}
+
+extern "C" int rhllc_add_rad_flux(rhllc_solver *h, double tau, const float *Ur, const float *Fr3) {
+  return add_rad_flux(h, tau, Ur, Fr3, nullptr);
+}
+
+extern "C" int rhllc_add_rad_flux_tensor(rhllc_solver *h, double tau, const float *Ur, const float *Fr3,
+                                         const float *Tr9) {
+  if (!Tr9) return fail(RHLLC_ERR_ARG, "null pressure tensor");
+  return add_rad_flux(h, tau, Ur, Fr3, Tr9);
 }
 
 extern "C" int rhllc_set_radiation_emission(rhllc_solver *h, int32_t on) {
@@ -659,15 +766,28 @@ extern "C" int rhllc_time_kernels(rhllc_solver *h, int32_t nsteps, double out[4]
   for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
   double acc[4] = {0, 0, 0, 0};
   int rc = RHLLC_OK;
+  const bool stats = h->prm.collect_stats != 0;
   for (int32_t k = 0; k < nsteps && rc == RHLLC_OK; ++k) {
+    const Dev x = h->tiled ? h->dev(h->cur, h->cur ^ 1) : h->now();
     hipError_t e = hipEventRecord(ev[0], h->stream);
-    if (e == hipSuccess) e = launch_face(h->d, h->mixed, h->stream);
+    if (e == hipSuccess)
+      e = h->tiled ? launch_tile(x, h->mixed, h->variant, h->rad, stats, h->n_tiles, h->tile_slots, h->stream)
+                   : launch_face(x, h->mixed, h->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[1], h->stream);
-    if (e == hipSuccess) e = h->enqueue_cells();
+    if (e == hipSuccess) {
+      if (!h->tiled)
+        e = h->enqueue_cells(x);
+      else if (h->variant == kMixedMlp)
+        e = launch_cell_mlp(x, h->rad, stats, h->stream);
+    }
     if (e == hipSuccess) e = hipEventRecord(ev[2], h->stream);
-    if (e == hipSuccess) e = launch_dt(h->d, h->stream);
+    if (e == hipSuccess) e = launch_dt(x, h->stream);
     if (e == hipSuccess) e = hipEventRecord(ev[3], h->stream);
     if (e == hipSuccess) e = hipEventSynchronize(ev[3]);
+    if (e == hipSuccess) {
+      h->last_in = h->cur;
+      if (h->tiled) h->cur ^= 1;
+    }
     float ms[4] = {0, 0, 0, 0};
     if (e == hipSuccess) e = hipEventElapsedTime(&ms[0], ev[0], ev[1]);
     if (e == hipSuccess) e = hipEventElapsedTime(&ms[1], ev[1], ev[2]);
@@ -691,11 +811,39 @@ extern "C" int rhllc_info(rhllc_solver *h, int64_t *n_cells, int64_t *n_faces, i
   if (n_faces) *n_faces = h->n_faces;
   if (device_bytes) {
     const int64_t N = h->n_cells, F = h->n_faces;
-    // Algorithmic bytes of one step (both kernels), each array element counted
-    // once per kernel that touches it.
-    *device_bytes = h->mixed ? 180 * N + 68 * F : 233 * N + 120 * F;
+    // The two-kernel structure's compulsory bytes at the variant's storage
+    // precision: FP64 face 80 N + 80 F, cell 153 N + 40 F; mixed face
+    // 20 N + 44 F, cell 141 N + 20 F; the radiation fields 24 N.
+    *device_bytes = h->mixed ? 161 * N + 64 * F : 233 * N + 120 * F;
     if (h->rad) *device_bytes += 24 * N;
   }
+  return RHLLC_OK;
+}
+
+extern "C" int rhllc_tile_layout(int64_t n_cells, const void *geo_cells, int64_t n_faces, const void *geo_faces,
+                                 int32_t reorder, int32_t *n_tiles, int64_t *n_extra, int32_t *max_slots,
+                                 int32_t *cell_new_to_old, int32_t *face_new_to_old, int32_t *tile_meta,
+                                 int32_t *tile_extra, uint32_t *cell_slots) {
+  if (!geo_cells || !geo_faces || !n_tiles || !n_extra || !max_slots) return fail(RHLLC_ERR_ARG, "null argument");
+  if (n_cells < 1 || n_cells >= ((int64_t)1 << 25) || n_faces < 1 || n_faces > 0x7FFFFFFF)
+    return fail(RHLLC_ERR_ARG, "mesh size out of range");
+  rhllc_layout::Options opt;
+  opt.reorder = reorder != 0;
+  opt.tiled = true;
+  opt.has_out_source = true;
+  opt.tile_cells = kTileCells;
+  rhllc_layout::Layout L;
+  std::string err;
+  if (!rhllc_layout::build(opt, n_cells, geo_cells, n_faces, geo_faces, L, err)) return fail(RHLLC_ERR_ARG, err);
+  *n_tiles = L.n_tiles;
+  *n_extra = (int64_t)L.tile_extra.size();
+  *max_slots = L.max_slots;
+  if (cell_new_to_old) std::memcpy(cell_new_to_old, L.cell_new_to_old.data(), sizeof(int32_t) * n_cells);
+  if (face_new_to_old) std::memcpy(face_new_to_old, L.face_new_to_old.data(), sizeof(int32_t) * n_faces);
+  if (tile_meta) std::memcpy(tile_meta, L.tile_meta.data(), sizeof(int32_t) * L.tile_meta.size());
+  if (tile_extra && !L.tile_extra.empty())
+    std::memcpy(tile_extra, L.tile_extra.data(), sizeof(int32_t) * L.tile_extra.size());
+  if (cell_slots) std::memcpy(cell_slots, L.cell_slots.data(), sizeof(uint32_t) * L.cell_slots.size());
   return RHLLC_OK;
 }
 
@@ -791,9 +939,11 @@ extern "C" int rhllc_eval_pressure_fix(int32_t eos, double gamma, int64_t n, dou
 }
 
 extern "C" int rhllc_eval_rad_source(int64_t n, const double *W, const double *Ur, const double *Fr3,
-                                     const double *ka, const double *ks, int32_t emission, double *G) {
+                                     const double *ka, const double *ks, int32_t emission, const float *Tr9,
+                                     double *G) {
   if (n < 0 || !W || !Ur || !Fr3 || !ka || !ks || !G) return fail(RHLLC_ERR_ARG, "bad argument");
-  DevBuf a, b, c, dd, e, f;
+  DevBuf a, b, c, dd, e, f, t;
+  if (Tr9) HIP_TRY(to_dev(t, Tr9, sizeof(float) * 9 * n));
   HIP_TRY(to_dev(a, W, sizeof(double) * 5 * n));
   HIP_TRY(to_dev(b, Ur, sizeof(double) * n));
   HIP_TRY(to_dev(c, Fr3, sizeof(double) * 3 * n));
@@ -801,18 +951,19 @@ extern "C" int rhllc_eval_rad_source(int64_t n, const double *W, const double *U
   HIP_TRY(to_dev(e, ks, sizeof(double) * n));
   HIP_TRY(to_dev(f, nullptr, sizeof(double) * 4 * n));
   HIP_TRY(launch_eval_rad(n, (double *)a.p, (double *)b.p, (double *)c.p, (double *)dd.p, (double *)e.p,
-                          emission, (double *)f.p, nullptr));
+                          emission, (const float *)t.p, (double *)f.p, nullptr));
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(G, f.p, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
   return RHLLC_OK;
 }
 
 extern "C" int rhllc_eval_rad_source_opt(int32_t nx, int32_t ny, const double header[6], const double *table,
-                                         int64_t n, const double *W, const double *Ur, const double *Fr3, double *G,
-                                         double *cell) {
+                                         int64_t n, const double *W, const double *Ur, const double *Fr3,
+                                         const float *Tr9, double *G, double *cell) {
   if (n < 0 || !header || !table || !W || !Ur || !Fr3 || !G || !cell) return fail(RHLLC_ERR_ARG, "bad argument");
   if (int e = check_table(nx, ny, header)) return e;
-  DevBuf t, a, b, c, g, o;
+  DevBuf t, a, b, c, g, o, tt;
+  if (Tr9) HIP_TRY(to_dev(tt, Tr9, sizeof(float) * 9 * n));
   HIP_TRY(to_dev(t, table, sizeof(double) * nx * ny));
   HIP_TRY(to_dev(a, W, sizeof(double) * 5 * n));
   HIP_TRY(to_dev(b, Ur, sizeof(double) * n));
@@ -820,7 +971,7 @@ extern "C" int rhllc_eval_rad_source_opt(int32_t nx, int32_t ny, const double he
   HIP_TRY(to_dev(g, nullptr, sizeof(double) * 4 * n));
   HIP_TRY(to_dev(o, nullptr, sizeof(double) * 5 * n));
   HIP_TRY(launch_eval_rad_opt(nx, ny, header, (double *)t.p, n, (double *)a.p, (double *)b.p, (double *)c.p,
-                              (double *)g.p, (double *)o.p, nullptr));
+                              (const float *)tt.p, (double *)g.p, (double *)o.p, nullptr));
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(G, g.p, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(cell, o.p, sizeof(double) * 5 * n, hipMemcpyDeviceToHost));

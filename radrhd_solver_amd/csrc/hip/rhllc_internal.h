@@ -12,6 +12,8 @@ constexpr int kFacesPerThread = 1;  // faces per k_face thread (2 measured slowe
 constexpr int kSpeedSlots = 1024;  // per-step max-speed partials (one atomic slot per block group)
 constexpr int kSlotStride = 16;    // doubles between slots: one 128-byte line each
 constexpr int kHistBins = 41;
+constexpr int kTileCells = 256;     // cells per tile of the tiled step (one workgroup, one cell per thread)
+constexpr int kTileMaxSlots = 4 * kTileCells;  // faces a tile can touch (4 per cell)
 
 // Device-resident scalars of the time loop and the step counters.
 struct Scalars {
@@ -47,15 +49,40 @@ struct Dev {
   const int4 *cell_faces;  // face index | 0x80000000 when the cell is id_r (sign bit 0)
   const double *cell_V;    // FP64 path
   const float4 *cell_Vd;   // mixed path: (V, closure defect sum_j sign_j face_Sn32_j)
+  // State at the start of the step (read by the face work and by each cell's
+  // own update) and where the cell update writes it.  The two-kernel step
+  // updates in place (out == in); the tiled step gathers neighbours' state
+  // while other tiles already write theirs, so whatever the face work reads
+  // there is double-buffered: U and W on the FP64 path, the cell records on
+  // the mixed path (whose U is read by its own cell only and stays in place).
   double4 *U4;   // (D, mx, my, mz), FP64 on both paths (c2p amplifies errors in U)
   double *UE;    // E
   double4 *W4;   // (rho, vx, vy, vz), FP64 path
   double *Wp;    // p (both paths: the FP64 pressure)
   float4 *Wva;   // mixed path: (vx, vy, vz, p) in FP32 (mixed_record)
   float4 *Wsd;   // mixed path: (s - D, D, cs2, extension counts of s - D and D), s = rho h W^2
+  double4 *U4o;
+  double *UEo;
+  double4 *W4o;
+  double *Wpo;
+  float4 *Wvao;
+  float4 *Wsdo;
+  // Tiled step: tile t owns cells [t kTileCells, (t + 1) kTileCells) and
+  // computes every face of them: its own faces (those whose earlier cell is in
+  // the tile, a contiguous range since faces are grouped by their earlier
+  // cell) and the extra faces it shares with an earlier tile (recomputed, same
+  // bits).  tile_meta = (first own face, own count, first extra, extra count);
+  // cell_slots = four 16-bit tile-local face slots in slot order, bit 15 set
+  // when the cell is the face's id_r (the flux is subtracted).
+  const int4 *tile_meta;
+  const int32_t *tile_extra;
+  const uint2 *cell_slots;
   // radiation source (optional): (Fx, Fy, Fz, Ur) and (kappa_a, kappa_s)
   const float4 *rad_FU;
   const float2 *rad_k;
+  // Radiation pressure tensor Tr per cell (9 FP32, row-major), the reference's
+  // compiled ISOTROPIC_INTENSITY 0 branch; null: the M1 closure.
+  const float *rad_T;
   int32_t rad_emission;  // thermal emission term of the source (0: off)
   // Opacity table (ln Lambda over ln D x ln T, TableFunc of the reference):
   // when set, each cell's opacities come from its state and rad_k is unused.
@@ -83,6 +110,13 @@ hipError_t launch_p2c_init(const Dev &d, hipStream_t s);
 hipError_t launch_face(const Dev &d, bool mixed, hipStream_t s);
 hipError_t launch_cell(const Dev &d, int variant, bool rad, bool stats, hipStream_t s);
 hipError_t launch_cell_mlp(const Dev &d, bool rad, bool stats, hipStream_t s);
+// The tiled step: faces into LDS, then the cells of the tile from LDS.
+// lds_slots = the largest face count of any tile.
+hipError_t launch_tile(const Dev &d, bool mixed, int variant, bool rad, bool stats, int n_tiles, int lds_slots,
+                       hipStream_t s);
+// Face fluxes of the state in d's input buffers as the step computes them
+// (FP64 S * F, energy slot = E flux), into F4/FE: for rhllc_download_face_flux.
+hipError_t launch_face_fp64_out(const Dev &d, bool mixed, double4 *F4, double *FE, hipStream_t s);
 hipError_t launch_dt(const Dev &d, hipStream_t s);
 hipError_t launch_clear_scalars(const Dev &d, hipStream_t s);
 // Mixed path: W (FP64) recovered from U by c2p (start from the FP32 pressure).
@@ -102,9 +136,10 @@ hipError_t launch_eval_c2p(bool mixed, int eos, double gamma, int64_t n, const d
 hipError_t launch_eval_p2c(int eos, double gamma, int64_t n, double *W, double *U, hipStream_t s);
 hipError_t launch_eval_fix(int eos, double gamma, int64_t n, double *U, double *W, int32_t *codes, hipStream_t s);
 hipError_t launch_eval_rad(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
-                           const double *ks, int emission, double *G, hipStream_t s);
+                           const double *ks, int emission, const float *Tr9, double *G, hipStream_t s);
 // hdr = {min_x, max_x, step_x, min_y, max_y, step_y}; host pointer, read at launch.
 hipError_t launch_eval_rad_opt(int nx, int ny, const double *hdr, const double *table, int64_t n, const double *W,
-                               const double *Ur, const double *Fr3, double *G, double *cell, hipStream_t s);
+                               const double *Ur, const double *Fr3, const float *Tr9, double *G, double *cell,
+                               hipStream_t s);
 
 }  // namespace rhllc_dev

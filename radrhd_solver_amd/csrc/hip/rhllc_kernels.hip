@@ -55,11 +55,11 @@ __device__ __forceinline__ void load_W32(const Dev &d, int c, State<double> &W) 
   W.d = r.D * sqrt(1.0 - dot3(W.v, W.v));
 }
 template <int EOS>
-__device__ __forceinline__ void store_W32(const Dev &d, int c, const State<double> &W) {
+__device__ __forceinline__ void store_W32(const Dev &d, float4 *va_out, float4 *sd_out, int c, const State<double> &W) {
   float4 va, sd;
   mixed_record<EOS>(d.gamma, d.Gg, W, va, sd);
-  d.Wva[c] = va;
-  d.Wsd[c] = sd;
+  va_out[c] = va;
+  sd_out[c] = sd;
 }
 
 }  // namespace rhllc_dev
@@ -88,14 +88,15 @@ __device__ __forceinline__ void fold_speed(const Dev &d, double speed) {
   }
 }
 
-// One face: gather, HLLC flux, flux store; returns max(lL, lR).
+// One face: gather of the two cells' state at the start of the step, HLLC
+// flux into F (S * flux in the global frame; on the mixed path F.p is the
+// flux of tau = E - D); returns max(lL, lR).
 template <bool MIXED, int EOS>
-__device__ __forceinline__ double face_flux(const Dev &d, int i) {
-  double speed = 0.0;  // rhllc_3d_stab.cpp:18 -- per-thread max starts at 0
+__device__ __forceinline__ double face_flux(const Dev &d, int i, State<double> &F) {
   const int2 lr = d.face_lr[i];
   if (!MIXED) {
     const double4 nS = d.face_nS[i];
-    State<double> UL, WL, UR, WR, F;
+    State<double> UL, WL, UR, WR;
     load_U(d, lr.x, UL);
     load_W(d, lr.x, WL);
     if (lr.y >= 0) {
@@ -105,15 +106,11 @@ __device__ __forceinline__ double face_flux(const Dev &d, int i) {
       UR.d = d.ghostU[0], UR.v[0] = d.ghostU[1], UR.v[1] = d.ghostU[2], UR.v[2] = d.ghostU[3], UR.p = d.ghostU[4];
       WR.d = d.ghostW[0], WR.v[0] = d.ghostW[1], WR.v[1] = d.ghostW[2], WR.v[2] = d.ghostW[3], WR.p = d.ghostW[4];
     }
-    const double s = hllc_flux<double, Ieee, EOS>(d.gamma, d.Gg, UL, UR, WL, WR, nS.x, nS.y, nS.z, nS.w, F);
-    speed = smax(speed, s);
-    d.F4[i] = make_double4(F.d, F.v[0], F.v[1], F.v[2]);
-    d.FE[i] = F.p;
+    return hllc_flux<double, Ieee, EOS>(d.gamma, d.Gg, UL, UR, WL, WR, nS.x, nS.y, nS.z, nS.w, F);
   } else {
-    // FP32 storage (cell records, geometry, flux), FP64 arithmetic with the
-    // fast divide/sqrt policy.
+    // FP32 storage (cell records, geometry), FP64 arithmetic with the fast
+    // divide/sqrt policy.
     const float3 Sn = d.face_Sn32[i];
-    State<double> F;
     const MixedIn WL = mixed_in(d.Wva[lr.x], d.Wsd[lr.x]);
     MixedIn WR;
     if (lr.y >= 0) {
@@ -125,8 +122,21 @@ __device__ __forceinline__ double face_flux(const Dev &d, int i) {
       mixed_record<EOS>(d.gamma, d.Gg, g, va, sd);
       WR = mixed_in(va, sd);
     }
-    const double s = mixed_face_flux(WL, WR, Sn.x, Sn.y, Sn.z, F);
-    speed = smax(speed, s);
+    return mixed_face_flux(WL, WR, Sn.x, Sn.y, Sn.z, F);
+  }
+}
+
+// Face flux stored for the two-kernel step: FP64 (D, m) + E on the FP64
+// path; the packed 24-byte record (flux_ext.h) on the mixed path.
+template <bool MIXED, int EOS>
+__device__ __forceinline__ double face_flux_store(const Dev &d, int i) {
+  State<double> F;
+  const double s = face_flux<MIXED, EOS>(d, i, F);
+  const double speed = smax(0.0, s);  // rhllc_3d_stab.cpp:18 -- per-thread max starts at 0
+  if (!MIXED) {
+    d.F4[i] = make_double4(F.d, F.v[0], F.v[1], F.v[2]);
+    d.FE[i] = F.p;
+  } else {
     const rhllc_ext::Packed pk = rhllc_ext::pack(F.d, F.v[0], F.v[1], F.v[2], F.p);
     d.F4f[i] = pk.a;
     d.FXf[i] = pk.b;
@@ -143,9 +153,21 @@ __global__ __launch_bounds__(kBlock) void k_face(Dev d) {
 #pragma unroll
   for (int k = 0; k < kFacesPerThread; ++k) {
     const int i = base + k * kBlock;
-    if (i < d.n_faces) speed = smax(speed, face_flux<MIXED, EOS>(d, i));
+    if (i < d.n_faces) speed = smax(speed, face_flux_store<MIXED, EOS>(d, i));
   }
   fold_speed(d, speed);
+}
+
+// The step's face fluxes in FP64 (energy slot = E flux) for a download; on
+// the mixed path the same arithmetic as the step, E flux = D flux + tau flux.
+template <bool MIXED, int EOS>
+__global__ __launch_bounds__(kBlock) void k_face_fp64_out(Dev d, double4 *F4, double *FE) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= d.n_faces) return;
+  State<double> F;
+  (void)face_flux<MIXED, EOS>(d, i, F);
+  F4[i] = make_double4(F.d, F.v[0], F.v[1], F.v[2]);
+  FE[i] = MIXED ? F.d + F.p : F.p;
 }
 
 template <bool MIXED>
@@ -192,6 +214,35 @@ __device__ __forceinline__ double mlp_guess(const Dev &d, bool live, const State
 // change the previous pressure is as good a start and costs nothing.
 constexpr double kMlpGate = 0.02;
 
+// Where a cell's four face fluxes come from.  GlobalFlux: the arrays the
+// face kernel of the two-kernel step wrote (face index | bit 31 when the cell
+// is id_r).  TileFlux: the LDS of the tiled step, FP64 (D, m0), (m1, m2), E
+// per slot (16-bit slot | bit 15 when the cell is id_r); on the mixed path the
+// energy slot holds the tau = E - D flux, so E's flux is D's plus tau's, as
+// the packed record's unpack gives it.
+template <bool MIXED>
+struct GlobalFlux {
+  const Dev &d;
+  int4 cf;
+  __device__ __forceinline__ bool neg(int j) const { return (&cf.x)[j] < 0; }
+  __device__ __forceinline__ void load(int j, State<double> &F) const {
+    load_flux<MIXED>(d, (&cf.x)[j] & 0x7FFFFFFF, F);
+  }
+};
+template <bool MIXED>
+struct TileFlux {
+  const double2 *A, *B;
+  const double *C;
+  uint32_t sv[4];
+  __device__ __forceinline__ bool neg(int j) const { return (sv[j] & 0x8000u) != 0; }
+  __device__ __forceinline__ void load(int j, State<double> &F) const {
+    const uint32_t k = sv[j] & 0x7FFFu;
+    const double2 a = A[k], b = B[k];
+    F.d = a.x, F.v[0] = a.y, F.v[1] = b.x, F.v[2] = b.y;
+    F.p = MIXED ? a.x + C[k] : C[k];
+  }
+};
+
 // Gather of the four face fluxes in slot order with their signs
 // (rhllc_3d_stab.cpp:30-45) and U -= tau/V * sum.  Also returns the change of
 // tau = E - D (for the surrogate gate) and, on the mixed path, the cell's
@@ -205,18 +256,16 @@ constexpr double kMlpGate = 0.02;
 // (D v.delta, m v.delta + p delta, m.delta) with v.delta = m.delta / (E + p),
 // restores that balance; with exact geometry delta = 0 and the update is
 // unchanged.
-template <bool MIXED>
-__device__ __forceinline__ void update_U(const Dev &d, int c, double tau, State<double> &U, double &dtau,
-                                         float &p32) {
-  const int4 cf = d.cell_faces[c];
-  const int fidx[4] = {cf.x, cf.y, cf.z, cf.w};
+template <bool MIXED, typename Src>
+__device__ __forceinline__ void update_U(const Dev &d, int c, double tau, const Src &src, State<double> &U,
+                                         double &dtau, float &p32) {
   State<double> s;
   s.d = 0.0, s.v[0] = 0.0, s.v[1] = 0.0, s.v[2] = 0.0, s.p = 0.0;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     State<double> F;
-    load_flux<MIXED>(d, fidx[j] & 0x7FFFFFFF, F);
-    if (fidx[j] >= 0) {
+    src.load(j, F);
+    if (!src.neg(j)) {
       s.d += F.d, s.v[0] += F.v[0], s.v[1] += F.v[1], s.v[2] += F.v[2], s.p += F.p;
     } else {
       s.d -= F.d, s.v[0] -= F.v[0], s.v[1] -= F.v[1], s.v[2] -= F.v[2], s.p -= F.p;
@@ -256,14 +305,15 @@ __device__ __forceinline__ void apply_rad(const Dev &d, int c, double tau, State
   using A = typename std::conditional<MIXED, Fast, Ieee>::type;
   const float4 fu = d.rad_FU[c];
   const double Fr[3] = {(double)fu.x, (double)fu.y, (double)fu.z};
+  const float *Tr9 = d.rad_T ? d.rad_T + 9 * (size_t)c : nullptr;
   double G[4];
   if (d.tab) {
     const OpacityTable t{d.tab,        d.tab_nx,    d.tab_ny,    d.tab_min_x, d.tab_max_x,
                          d.tab_step_x, d.tab_min_y, d.tab_max_y, d.tab_step_y};
-    rad_source_opt<A>(W, cell_rad<A>(t, W), (double)fu.w, Fr, G);
+    rad_source_opt<A>(W, cell_rad<A>(t, W), (double)fu.w, Fr, G, Tr9);
   } else {
     const float2 kk = d.rad_k[c];
-    rad_source<A>(W, (double)fu.w, Fr, (double)kk.x, (double)kk.y, G, d.rad_emission != 0);
+    rad_source<A>(W, (double)fu.w, Fr, (double)kk.x, (double)kk.y, G, d.rad_emission != 0, Tr9);
   }
   const double ds = 1;
   U.p += ds * tau * G[0];
@@ -329,13 +379,13 @@ __device__ __forceinline__ void finish_cell(const Dev &d, int c, bool live, Stat
     }
     if (RAD) apply_rad<MIXED, EOS>(d, c, tau, U, W, iters, fixed, fix_failed);
     if (STATS) resid = c2p_residual<EOS>(d.Gg, U, W.p);
-    d.U4[c] = make_double4(U.d, U.v[0], U.v[1], U.v[2]);
-    d.UE[c] = U.p;
+    d.U4o[c] = make_double4(U.d, U.v[0], U.v[1], U.v[2]);
+    d.UEo[c] = U.p;
     if (!MIXED) {
-      d.W4[c] = make_double4(W.d, W.v[0], W.v[1], W.v[2]);
-      d.Wp[c] = W.p;
+      d.W4o[c] = make_double4(W.d, W.v[0], W.v[1], W.v[2]);
+      d.Wpo[c] = W.p;
     } else {
-      store_W32<EOS>(d, c, W);
+      store_W32<EOS>(d, d.Wvao, d.Wsdo, c, W);
     }
   }
   if (__ballot(live && (code != 0 || fixed || fell_back)) != 0) {  // rare: skip eight ballots otherwise
@@ -372,11 +422,32 @@ __device__ __forceinline__ void finish_cell(const Dev &d, int c, bool live, Stat
   }
 }
 
+// The surrogate-start variant defers cells whose tau changed by more than
+// kMlpGate: they are appended to d.mlp_list (one atomic per wave), their
+// updated U is stored, and k_cell_mlp finishes them on full waves.  Returns
+// whether this lane's cell stays live here.  Called by all lanes of a wave.
+__device__ __forceinline__ bool defer_stale(const Dev &d, int c, bool live, const State<double> &U, double dtau) {
+  const bool stale = live && !(fabs(dtau) <= kMlpGate * fabs(U.p - U.d));
+  const unsigned long long m = __ballot(stale);
+  if (m) {
+    const int lane = threadIdx.x & 63;
+    const int first = __ffsll((long long)m) - 1;
+    unsigned int base = 0;
+    if (lane == first) base = atomicAdd(&d.sc->mlp_count, (unsigned int)__popcll(m));
+    base = __shfl(base, first, 64);
+    if (stale) {
+      d.mlp_list[base + __popcll(m & ((1ull << lane) - 1ull))] = c;
+      d.U4o[c] = make_double4(U.d, U.v[0], U.v[1], U.v[2]);
+      d.UEo[c] = U.p;
+      return false;
+    }
+  }
+  return live;
+}
+
 // One thread per cell.  Start of the Newton iteration: the reference start
 // (kFp64Ref), the FP64 previous pressure (kFp64Prev) or the FP32 one (mixed).
-// kMixedMlp: cells whose tau changed by more than kMlpGate are not finished
-// here; they are appended to d.mlp_list (one atomic per wave) and finished by
-// k_cell_mlp, which starts their c2p from the surrogate on full waves.
+// kMixedMlp: stale cells go to k_cell_mlp (defer_stale).
 template <int VARIANT, bool RAD, bool STATS, int EOS>
 __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
   constexpr bool MIXED = VARIANT >= kMixedPrev;
@@ -386,24 +457,58 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
   State<double> U;
   double dtau = 0.0;
   float p32 = -1.f;
-  if (live) update_U<MIXED>(d, c, tau, U, dtau, p32);
-  if (VARIANT == kMixedMlp) {
-    const bool stale = live && !(fabs(dtau) <= kMlpGate * fabs(U.p - U.d));
-    const unsigned long long m = __ballot(stale);
-    if (m) {
-      const int lane = threadIdx.x & 63;
-      const int first = __ffsll((long long)m) - 1;
-      unsigned int base = 0;
-      if (lane == first) base = atomicAdd(&d.sc->mlp_count, (unsigned int)__popcll(m));
-      base = __shfl(base, first, 64);
-      if (stale) {
-        d.mlp_list[base + __popcll(m & ((1ull << lane) - 1ull))] = c;
-        d.U4[c] = make_double4(U.d, U.v[0], U.v[1], U.v[2]);
-        d.UE[c] = U.p;
-        live = false;
-      }
-    }
+  if (live) {
+    const GlobalFlux<MIXED> src{d, d.cell_faces[c]};
+    update_U<MIXED>(d, c, tau, src, U, dtau, p32);
   }
+  if (VARIANT == kMixedMlp) live = defer_stale(d, c, live, U, dtau);
+  double guess = -1.0;
+  if (VARIANT == kFp64Prev && live) guess = d.Wp[c];
+  if (MIXED && live) guess = (double)p32;
+  finish_cell<VARIANT, RAD, STATS, EOS>(d, c, live, U, guess, false, tau);
+}
+
+// The tiled step, one workgroup per tile of kTileCells cells (Hilbert order).
+// Face phase: every face of the tile's cells, own faces then extra faces,
+// block-strided; the flux goes to LDS in FP64 and the speed into the per-step
+// maximum.  Cell phase, after the barrier of the speed fold: one thread per
+// cell sums its four fluxes from LDS in slot order with their signs and
+// finishes the cell exactly as k_cell does.  The state the face phase gathers
+// is the step's input buffer; the cell phase writes the output buffer, so no
+// tile reads what another tile of the same step writes.  A face shared by two
+// tiles is evaluated by both from the same inputs, to the same bits.
+template <int VARIANT, bool RAD, bool STATS, int EOS>
+__global__ __launch_bounds__(kBlock) void k_tile(Dev d) {
+  static_assert(kTileCells == kBlock, "one cell per thread in the cell phase");
+  constexpr bool MIXED = VARIANT >= kMixedPrev;
+  extern __shared__ double2 lds_flux[];
+  const int t = (int)xcd_block(blockIdx.x, gridDim.x);
+  const int4 tm = d.tile_meta[t];
+  const int nf = tm.y + tm.w;
+  double2 *A = lds_flux, *B = lds_flux + nf;
+  double *C = reinterpret_cast<double *>(lds_flux + 2 * nf);
+  double speed = 0.0;
+  for (int k = threadIdx.x; k < nf; k += kBlock) {
+    const int f = k < tm.y ? tm.x + k : d.tile_extra[tm.z + (k - tm.y)];
+    State<double> F;
+    speed = smax(speed, face_flux<MIXED, EOS>(d, f, F));
+    A[k] = make_double2(F.d, F.v[0]);
+    B[k] = make_double2(F.v[1], F.v[2]);
+    C[k] = F.p;
+  }
+  fold_speed(d, speed);  // its barrier also publishes the LDS fluxes
+  const int c = t * kTileCells + (int)threadIdx.x;
+  bool live = c < d.n_cells;
+  const double tau = d.sc->tau;
+  State<double> U;
+  double dtau = 0.0;
+  float p32 = -1.f;
+  if (live) {
+    const uint2 sl = d.cell_slots[c];
+    const TileFlux<MIXED> src{A, B, C, {sl.x & 0xFFFFu, sl.x >> 16, sl.y & 0xFFFFu, sl.y >> 16}};
+    update_U<MIXED>(d, c, tau, src, U, dtau, p32);
+  }
+  if (VARIANT == kMixedMlp) live = defer_stale(d, c, live, U, dtau);
   double guess = -1.0;
   if (VARIANT == kFp64Prev && live) guess = d.Wp[c];
   if (MIXED && live) guess = (double)p32;
@@ -480,7 +585,7 @@ __global__ __launch_bounds__(kBlock) void k_seed_mixed(Dev d) {
   if (c >= d.n_cells) return;
   State<double> W;
   load_W(d, c, W);
-  store_W32<EOS>(d, c, W);
+  store_W32<EOS>(d, d.Wva, d.Wsd, c, W);
 }
 
 // Mixed path: FP64 primitives from the FP64 conserved state (for download).
@@ -526,7 +631,7 @@ __global__ __launch_bounds__(kBlock) void k_add_rad(Dev d, double tau) {
       d.W4[c] = make_double4(W.d, W.v[0], W.v[1], W.v[2]);
       d.Wp[c] = W.p;
     } else {
-      store_W32<EOS>(d, c, W);
+      store_W32<EOS>(d, d.Wva, d.Wsd, c, W);
     }
   }
   if (__ballot(fixed) != 0) {
@@ -638,6 +743,70 @@ hipError_t launch_cell_mlp(const Dev &d, bool rad, bool stats, hipStream_t s) {
       hipLaunchKernelGGL((k_cell_mlp<false, true>), g, b, 0, s, d);
     else
       hipLaunchKernelGGL((k_cell_mlp<false, false>), g, b, 0, s, d);
+  }
+  return hipGetLastError();
+}
+
+template <int V, int EOS>
+static void launch_tile_v(const Dev &d, bool rad, bool stats, dim3 g, size_t lds, hipStream_t s) {
+  const dim3 b(kBlock);
+  if (rad) {
+    if (stats)
+      hipLaunchKernelGGL((k_tile<V, true, true, EOS>), g, b, lds, s, d);
+    else
+      hipLaunchKernelGGL((k_tile<V, true, false, EOS>), g, b, lds, s, d);
+  } else {
+    if (stats)
+      hipLaunchKernelGGL((k_tile<V, false, true, EOS>), g, b, lds, s, d);
+    else
+      hipLaunchKernelGGL((k_tile<V, false, false, EOS>), g, b, lds, s, d);
+  }
+}
+
+// One workgroup per tile; LDS for the largest tile's fluxes (40 bytes a face).
+hipError_t launch_tile(const Dev &d, bool mixed, int variant, bool rad, bool stats, int n_tiles, int lds_slots,
+                       hipStream_t s) {
+  if (d.n_cells == 0) return hipSuccess;
+  if (n_tiles < 1 || lds_slots < 1 || lds_slots > kTileMaxSlots) return hipErrorInvalidValue;
+  if (mixed != (variant >= kMixedPrev)) return hipErrorInvalidValue;
+  const dim3 g((unsigned int)n_tiles);
+  const size_t lds = (size_t)lds_slots * 40;
+  const bool taub = d.eos == kTaub;
+  switch (variant) {
+    case kFp64Ref:
+      taub ? launch_tile_v<kFp64Ref, kTaub>(d, rad, stats, g, lds, s) : launch_tile_v<kFp64Ref, kIdeal>(d, rad, stats, g, lds, s);
+      break;
+    case kFp64Prev:
+      taub ? launch_tile_v<kFp64Prev, kTaub>(d, rad, stats, g, lds, s)
+           : launch_tile_v<kFp64Prev, kIdeal>(d, rad, stats, g, lds, s);
+      break;
+    case kMixedPrev:
+      taub ? launch_tile_v<kMixedPrev, kTaub>(d, rad, stats, g, lds, s)
+           : launch_tile_v<kMixedPrev, kIdeal>(d, rad, stats, g, lds, s);
+      break;
+    case kMixedMlp:
+      if (taub) return hipErrorInvalidValue;
+      launch_tile_v<kMixedMlp, kIdeal>(d, rad, stats, g, lds, s);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_face_fp64_out(const Dev &d, bool mixed, double4 *F4, double *FE, hipStream_t s) {
+  if (d.n_faces == 0) return hipSuccess;
+  const dim3 g(blocks_for(d.n_faces)), b(kBlock);
+  if (d.eos == kTaub) {
+    if (mixed)
+      hipLaunchKernelGGL((k_face_fp64_out<true, kTaub>), g, b, 0, s, d, F4, FE);
+    else
+      hipLaunchKernelGGL((k_face_fp64_out<false, kTaub>), g, b, 0, s, d, F4, FE);
+  } else {
+    if (mixed)
+      hipLaunchKernelGGL((k_face_fp64_out<true, kIdeal>), g, b, 0, s, d, F4, FE);
+    else
+      hipLaunchKernelGGL((k_face_fp64_out<false, kIdeal>), g, b, 0, s, d, F4, FE);
   }
   return hipGetLastError();
 }
@@ -759,22 +928,22 @@ __global__ void k_eval_fix(double gamma, int64_t n, double *U, double *W, int32_
 }
 
 __global__ void k_eval_rad(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
-                           const double *ks, int emission, double *G) {
+                           const double *ks, int emission, const float *Tr9, double *G) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   State<double> w;
   ld5(W + 5 * i, w);
-  rad_source(w, Ur[i], Fr3 + 3 * i, ka[i], ks[i], G + 4 * i, emission != 0);
+  rad_source(w, Ur[i], Fr3 + 3 * i, ka[i], ks[i], G + 4 * i, emission != 0, Tr9 ? Tr9 + 9 * i : nullptr);
 }
 
 __global__ void k_eval_rad_opt(OpacityTable t, int64_t n, const double *W, const double *Ur, const double *Fr3,
-                               double *G, double *cell) {
+                               const float *Tr9, double *G, double *cell) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   State<double> w;
   ld5(W + 5 * i, w);
   const CellRad c = cell_rad(t, w);
-  rad_source_opt(w, c, Ur[i], Fr3 + 3 * i, G + 4 * i);
+  rad_source_opt(w, c, Ur[i], Fr3 + 3 * i, G + 4 * i, Tr9 ? Tr9 + 9 * i : nullptr);
   double *o = cell + 5 * i;
   o[0] = c.T, o[1] = c.vel, o[2] = c.lorenz, o[3] = c.alpha, o[4] = c.betta;
 }
@@ -844,17 +1013,18 @@ hipError_t launch_eval_fix(int eos, double gamma, int64_t n, double *U, double *
   return hipGetLastError();
 }
 hipError_t launch_eval_rad(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
-                           const double *ks, int emission, double *G, hipStream_t s) {
+                           const double *ks, int emission, const float *Tr9, double *G, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_eval_rad, eval_grid(n), dim3(256), 0, s, n, W, Ur, Fr3, ka, ks, emission, G);
+  hipLaunchKernelGGL(k_eval_rad, eval_grid(n), dim3(256), 0, s, n, W, Ur, Fr3, ka, ks, emission, Tr9, G);
   return hipGetLastError();
 }
 
 hipError_t launch_eval_rad_opt(int nx, int ny, const double *hdr, const double *table, int64_t n, const double *W,
-                               const double *Ur, const double *Fr3, double *G, double *cell, hipStream_t s) {
+                               const double *Ur, const double *Fr3, const float *Tr9, double *G, double *cell,
+                               hipStream_t s) {
   if (n == 0) return hipSuccess;
   const OpacityTable t{table, nx, ny, hdr[0], hdr[1], hdr[2], hdr[3], hdr[4], hdr[5]};
-  hipLaunchKernelGGL(k_eval_rad_opt, eval_grid(n), dim3(256), 0, s, t, n, W, Ur, Fr3, G, cell);
+  hipLaunchKernelGGL(k_eval_rad_opt, eval_grid(n), dim3(256), 0, s, t, n, W, Ur, Fr3, Tr9, G, cell);
   return hipGetLastError();
 }
 

@@ -1,0 +1,205 @@
+// Host preparation of a solver handle's device layout (rhllc_layout.h).
+#include "rhllc_layout.h"
+
+#include <algorithm>
+#include <cstring>
+
+#include "../mesh/mesh_common.h"
+
+namespace rhllc_layout {
+
+using rhllc_mesh::cell_ids;
+using rhllc_mesh::cell_V;
+using rhllc_mesh::RawVec;
+
+bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n_faces, const void *geo_faces,
+           Layout &L, std::string &err) {
+  const int32_t N = (int32_t)n_cells, F = (int32_t)n_faces;
+  const uint8_t *gc = static_cast<const uint8_t *>(geo_cells);
+  const uint8_t *gf = static_cast<const uint8_t *>(geo_faces);
+  auto frec = [&](int64_t f) { return gf + (size_t)f * RHLLC_GEO_FACE_BYTES; };
+  auto crec = [&](int64_t c) { return gc + (size_t)c * RHLLC_GEO_CELL_BYTES; };
+
+  // Validate so every device gather stays in bounds.
+  RawVec<int32_t> fl(F), fr(F);
+  if (!rhllc_mesh::validate_geo(N, geo_cells, F, geo_faces, opt.has_out_source, err, fl.data(), fr.data()))
+    return false;
+
+  // Numbering on the device.  Faces are grouped by the earlier new index of
+  // their two cells (a boundary face goes with its id_l): the Hilbert order
+  // does that itself; without it the tiles need the same grouping.
+  L.cell_new_to_old.resize(N);
+  L.face_new_to_old.resize(F);
+  if (opt.reorder) {
+    rhllc_mesh::curve_order_impl(N, geo_cells, F, geo_faces, L.cell_new_to_old.data(), L.face_new_to_old.data());
+  } else {
+    for (int32_t i = 0; i < N; ++i) L.cell_new_to_old[i] = i;
+    if (!opt.tiled) {
+      for (int32_t i = 0; i < F; ++i) L.face_new_to_old[i] = i;
+    } else {
+      std::vector<int64_t> start((size_t)N + 1, 0);  // counting sort, stable in the old face order
+      auto owner = [&](int32_t f) { return fr[f] >= 0 ? std::min(fl[f], fr[f]) : fl[f]; };
+      for (int32_t f = 0; f < F; ++f) start[owner(f) + 1]++;
+      for (int32_t i = 0; i < N; ++i) start[i + 1] += start[i];
+      for (int32_t f = 0; f < F; ++f) L.face_new_to_old[start[owner(f)]++] = f;
+    }
+  }
+  RawVec<int32_t> cnew(N), fnew(F);
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < N; ++i) cnew[L.cell_new_to_old[i]] = i;
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < F; ++i) fnew[L.face_new_to_old[i]] = i;
+
+  // Faces: cells, geometry.
+  L.lr.resize(2 * (size_t)F);
+  if (opt.mixed)
+    L.Sn32.resize(3 * (size_t)F);
+  else
+    L.nS.resize(4 * (size_t)F);
+  RawVec<int32_t> nb(F);  // the neighbour across the face (device numbering), -1 on the boundary
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < F; ++i) {
+    const int32_t f = L.face_new_to_old[i];
+    const uint8_t *r = frec(f);
+    double n[3], S;
+    std::memcpy(n, r + 8, 24);
+    std::memcpy(&S, r + 32, 8);
+    const int32_t l = cnew[fl[f]];
+    nb[i] = fr[f] >= 0 ? cnew[fr[f]] : -1;
+    // Copy-type boundaries (Cone geometry: every code but -3) read the cell itself.
+    L.lr[2 * (size_t)i] = l;
+    L.lr[2 * (size_t)i + 1] = fr[f] >= 0 ? nb[i] : (fr[f] == RHLLC_BOUND_OUT_SOURCE ? -1 : l);
+    if (opt.mixed) {
+      float *q = &L.Sn32[3 * (size_t)i];
+      q[0] = (float)(n[0] * S), q[1] = (float)(n[1] * S), q[2] = (float)(n[2] * S);
+    } else {
+      double *q = &L.nS[4 * (size_t)i];
+      q[0] = n[0], q[1] = n[1], q[2] = n[2], q[3] = S;
+    }
+  }
+
+  // Cells: face slots (device face index, bit 31 when the cell is id_r) and V.
+  RawVec<int32_t> cf(4 * (size_t)N);
+  if (opt.mixed)
+    L.Vd.resize(4 * (size_t)N);
+  else
+    L.V.resize(N);
+#pragma omp parallel for schedule(static)
+  for (int32_t i = 0; i < N; ++i) {
+    const uint8_t *r = crec(L.cell_new_to_old[i]);
+    int32_t ids[4];
+    cell_ids(r, ids);
+    for (int j = 0; j < 4; ++j)
+      cf[4 * (size_t)i + j] = fnew[ids[j]] | (((r[48] >> j) & 1) ? 0 : (int32_t)0x80000000u);
+    const double V = cell_V(r);
+    if (!opt.mixed) {
+      L.V[i] = V;
+    } else {
+      // delta = sum over the cell's faces of +-S n as the face kernel sees it
+      // (FP32), summed in FP64; the cell update subtracts the cell's own flux
+      // through it.
+      double dl[3] = {0.0, 0.0, 0.0};
+      for (int j = 0; j < 4; ++j) {
+        const int32_t e = cf[4 * (size_t)i + j];
+        const float *q = &L.Sn32[3 * (size_t)(e & 0x7FFFFFFF)];
+        const double sg = e >= 0 ? 1.0 : -1.0;
+        dl[0] += sg * (double)q[0], dl[1] += sg * (double)q[1], dl[2] += sg * (double)q[2];
+      }
+      float *o = &L.Vd[4 * (size_t)i];
+      o[0] = (float)V, o[1] = (float)dl[0], o[2] = (float)dl[1], o[3] = (float)dl[2];
+    }
+  }
+  if (!opt.tiled) {
+    L.cf.assign(cf.begin(), cf.end());
+    return true;
+  }
+
+  // Tiles of T consecutive cells.  A face belongs to the tile of its earlier
+  // cell (own faces: contiguous, since faces are grouped by that cell) and is
+  // also an extra face of the tile of its later cell when that is another one.
+  const int32_t T = opt.tile_cells;
+  const int32_t nt = (N + T - 1) / T;
+  L.n_tiles = nt;
+  // The own faces of tile t are [first[t], first[t + 1]); own_tile is
+  // non-decreasing in the face index.
+  auto own_tile = [&](int32_t i) {
+    const int32_t l = L.lr[2 * (size_t)i], b = nb[i];
+    return (b >= 0 ? std::min(l, b) : l) / T;
+  };
+  std::vector<int32_t> first(nt + 1);
+  first[nt] = F;
+#pragma omp parallel for schedule(static)
+  for (int32_t t = 0; t < nt; ++t) {
+    int32_t lo = 0, hi = F;  // first face with own_tile >= t
+    while (lo < hi) {
+      const int32_t mid = lo + (hi - lo) / 2;
+      if (own_tile(mid) < t)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    first[t] = lo;
+  }
+  // Extra faces of tile t: the faces its cells reach whose own tile is an
+  // earlier one, each met in exactly one slot (that of its later cell), listed
+  // by face index.  Per tile, in parallel: counts, then the lists.
+  auto extras_of = [&](int32_t t, int32_t *out) {
+    int n = 0;
+    const int32_t c1 = std::min(N, (t + 1) * T);
+    for (int32_t c = t * T; c < c1; ++c)
+      for (int j = 0; j < 4; ++j) {
+        const int32_t i = cf[4 * (size_t)c + j] & 0x7FFFFFFF;
+        if (i < first[t]) out[n++] = i;
+      }
+    std::sort(out, out + n);
+    return n;
+  };
+  L.tile_meta.resize(4 * (size_t)nt);
+#pragma omp parallel
+  {
+    std::vector<int32_t> buf(4 * (size_t)T);
+#pragma omp for schedule(static)
+    for (int32_t t = 0; t < nt; ++t) {
+      L.tile_meta[4 * (size_t)t + 1] = first[t + 1] - first[t];
+      L.tile_meta[4 * (size_t)t + 3] = extras_of(t, buf.data());
+    }
+  }
+  int64_t ne = 0;
+  int32_t max_slots = 0;
+  for (int32_t t = 0; t < nt; ++t) {
+    int32_t *m = &L.tile_meta[4 * (size_t)t];
+    m[0] = first[t];
+    m[2] = (int32_t)ne;
+    ne += m[3];
+    max_slots = std::max(max_slots, m[1] + m[3]);
+  }
+  // A tile's cells touch at most 4 T distinct faces; slots are 15 bits.
+  if (max_slots > 4 * T || max_slots > 0x7FFF || ne > 0x7FFFFFFF) {
+    err = "tile layout: a tile has " + std::to_string(max_slots) + " faces";
+    return false;
+  }
+  L.max_slots = max_slots;
+  L.tile_extra.resize(ne);
+  L.cell_slots.resize(2 * (size_t)N);
+#pragma omp parallel for schedule(static)
+  for (int32_t t = 0; t < nt; ++t) {
+    const int32_t *m = &L.tile_meta[4 * (size_t)t];
+    int32_t *ex = &L.tile_extra[m[2]];
+    extras_of(t, ex);
+    const int32_t c1 = std::min(N, (t + 1) * T);
+    for (int32_t c = t * T; c < c1; ++c) {
+      uint32_t s[4];
+      for (int j = 0; j < 4; ++j) {
+        const int32_t e = cf[4 * (size_t)c + j], i = e & 0x7FFFFFFF;
+        const uint32_t slot = i >= first[t] ? (uint32_t)(i - first[t])
+                                            : (uint32_t)(m[1] + (std::lower_bound(ex, ex + m[3], i) - ex));
+        s[j] = slot | (e < 0 ? 0x8000u : 0u);
+      }
+      L.cell_slots[2 * (size_t)c] = s[0] | (s[1] << 16);
+      L.cell_slots[2 * (size_t)c + 1] = s[2] | (s[3] << 16);
+    }
+  }
+  return true;
+}
+
+}  // namespace rhllc_layout

@@ -1,0 +1,45 @@
+// Host preparation of the device layout of one solver handle (plain C++,
+// no HIP types): mesh validation, the Hilbert relabelling, the SoA arrays of
+// both paths and the tiles of the tiled step.  Built with g++ and OpenMP and
+// linked into librhllc_hip.so.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../mesh/mesh_common.h"
+
+namespace rhllc_layout {
+
+struct Options {
+  bool mixed = false;           // FP32 face vectors and (V, closure defect) instead of FP64 (n, S) and V
+  bool reorder = true;          // Hilbert relabelling of cells (faces are grouped either way when tiled)
+  bool tiled = true;            // build the tiles of the tiled step (else the face index per cell slot)
+  bool has_out_source = false;  // boundary code -3 faces are allowed (fixed ghost state)
+  int tile_cells = 256;
+};
+
+struct Layout {
+  std::vector<int32_t> cell_new_to_old, face_new_to_old;
+  rhllc_mesh::RawVec<int32_t> lr;      // per face (left, right): right = cell, -1 ghost (-3), left (copy-type codes)
+  rhllc_mesh::RawVec<double> nS;       // per face (nx, ny, nz, S), FP64 path
+  rhllc_mesh::RawVec<float> Sn32;      // per face S (nx, ny, nz) in FP32, mixed path
+  rhllc_mesh::RawVec<double> V;        // per cell, FP64 path
+  rhllc_mesh::RawVec<float> Vd;        // per cell (V, delta) in FP32, mixed path
+  rhllc_mesh::RawVec<int32_t> cf;      // per cell 4 face indices | bit 31 when the cell is id_r (two-kernel step)
+  // tiled step
+  int32_t n_tiles = 0, max_slots = 0;
+  std::vector<int32_t> tile_meta;    // per tile (first own face, own count, first extra, extra count)
+  std::vector<int32_t> tile_extra;   // extra faces of all tiles, tile by tile, by face index
+  rhllc_mesh::RawVec<uint32_t> cell_slots;  // per cell two words: four 16-bit slots, bit 15 = id_r
+};
+
+// Validates the reference's geo_cell_t / geo_face_t records (every id in
+// range, boundary codes known, each cell slot agreeing with its face's
+// id_l / id_r and sign bit, V > 0) and builds the layout.  Returns false with
+// a message naming the first offending record.
+bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n_faces, const void *geo_faces,
+           Layout &out, std::string &err);
+
+}  // namespace rhllc_layout

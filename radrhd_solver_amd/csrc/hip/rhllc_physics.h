@@ -710,16 +710,28 @@ __device__ __forceinline__ double b_planck(double T) {
 }
 
 // The common tail of GetRadSource and GetRadSourceOpt (radRHD_utils.cpp:
-// 74-100 and 131-157) once gamma, gamma^2 and u^2 are known.
+// 74-100 and 131-157) once gamma, gamma^2 and u^2 are known.  D(j, i) is the
+// M1 Eddington tensor EddTensor(j, i, Ur, Fr) (ISOTROPIC_INTENSITY == 1), or,
+// when the caller passes the radiation pressure tensor Tr (row-major,
+// Tr(j, i) = Tr9[3 j + i]), D(j, i) = Tr(j, i) as the reference is compiled
+// (radRHD_config.h:8, ISOTROPIC_INTENSITY 0; radRHD_utils.cpp:80-86, 137-143),
+// multiplied by Ur below as there.
 template <typename A>
 __device__ __forceinline__ void rad_source_tail(const State<double> &W, double Ur, const double *Fr, double abs_op,
                                                 double scat_op, double q, double gamma, double gamma2, double u2,
-                                                double *G) {
+                                                double *G, const float *Tr9 = nullptr) {
   const double tot_op = abs_op + scat_op;
   const double rho = W.d;
   const double u[3] = {gamma * W.v[0], gamma * W.v[1], gamma * W.v[2]};
   double Dm[3][3];
-  edd_matrix<A>(Ur, Fr, Dm);
+  if (Tr9) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) Dm[a][b] = (double)Tr9[3 * a + b];
+  } else {
+    edd_matrix<A>(Ur, Fr, Dm);
+  }
   double uuD = 0.;
   const double uF = dot3(u, Fr);
   double uD[3];
@@ -745,14 +757,15 @@ __device__ __forceinline__ void rad_source_tail(const State<double> &W, double U
 // emission q = -kappa_a rho B(T(rho, p)) / kRadiation when `emission`.
 template <typename A = Ieee>
 __device__ __forceinline__ void rad_source(const State<double> &W, double Ur, const double *Fr, double abs_op,
-                                           double scat_op, double *G, bool emission = false) {
+                                           double scat_op, double *G, bool emission = false,
+                                           const float *Tr9 = nullptr) {
   double gamma = dot3(W.v, W.v);
   double u2 = gamma;
   const double gamma2 = A::rcp(1.0 - gamma);
   u2 *= gamma2;
   gamma = A::sqrt_(gamma2);
   const double q = emission ? -abs_op * W.d * b_planck(gas_temperature(W.d, W.p)) / radc::kRadiation : 0.0;
-  rad_source_tail<A>(W, Ur, Fr, abs_op, scat_op, q, gamma, gamma2, u2, G);
+  rad_source_tail<A>(W, Ur, Fr, abs_op, scat_op, q, gamma, gamma2, u2, G, Tr9);
 }
 
 // The reference's TableFunc (lib/grids/include/solvers_struct.h:24-44) over
@@ -814,12 +827,12 @@ __device__ __forceinline__ CellRad cell_rad(const OpacityTable &t, const State<d
 // temperature, gamma from its Lorentz factor, thermal emission always on.
 template <typename A = Ieee>
 __device__ __forceinline__ void rad_source_opt(const State<double> &W, const CellRad &c, double Ur, const double *Fr,
-                                               double *G) {
+                                               double *G, const float *Tr9 = nullptr) {
   const double gamma = c.lorenz;
   const double gamma2 = gamma * gamma;
   const double u2 = gamma2 * c.vel * c.vel;
   const double q = -c.alpha * W.d * b_planck(c.T) / radc::kRadiation;
-  rad_source_tail<A>(W, Ur, Fr, c.alpha, c.betta, q, gamma, gamma2, u2, G);
+  rad_source_tail<A>(W, Ur, Fr, c.alpha, c.betta, q, gamma, gamma2, u2, G, Tr9);
 }
 
 }  // namespace rhllc_dev

@@ -3,7 +3,10 @@
 
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../../include/rhllc_mesh.h"
 
@@ -34,10 +37,41 @@ inline bool fail(const std::string &msg) {
 }
 
 // Cells sorted by the Hilbert index of their centres; faces by the earlier new
-// index of their two cells (id_l for a boundary face), stable in the old face
-// order (counting sort).
+// index of their two cells (id_l for a boundary face), ties in the old face
+// order.  The mesh must have passed validate_geo.
 void curve_order_impl(int64_t n_cells, const void *geo_cells, int64_t n_faces, const void *geo_faces,
                       int32_t *cell_new_to_old, int32_t *face_new_to_old);
+
+// Checks what every consumer of the records indexes with: id_l / id_r in
+// range, boundary codes known (-3 only when allow_out_source), each cell's
+// face ids in range and agreeing with their face's id_l / id_r and sign bit,
+// V > 0.  On success fl / fr (n_faces each, may be null) hold id_l and the
+// sign-extended id_r; otherwise err names the first offending record.
+bool validate_geo(int64_t n_cells, const void *geo_cells, int64_t n_faces, const void *geo_faces,
+                  bool allow_out_source, std::string &err, int32_t *fl = nullptr, int32_t *fr = nullptr);
+
+// Allocator whose value-less construct leaves trivial elements uninitialised,
+// so large host arrays are first touched by the parallel loops that fill them.
+template <typename T>
+struct NoInit : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInit<U>;
+  };
+  NoInit() = default;
+  template <typename U>
+  NoInit(const NoInit<U> &) {}
+  template <typename U>
+  void construct(U *p) {
+    ::new (static_cast<void *>(p)) U;
+  }
+  template <typename U, typename... Args>
+  void construct(U *p, Args &&...args) {
+    ::new (static_cast<void *>(p)) U(std::forward<Args>(args)...);
+  }
+};
+template <typename T>
+using RawVec = std::vector<T, NoInit<T>>;
 
 inline int num_threads() {
 #ifdef _OPENMP

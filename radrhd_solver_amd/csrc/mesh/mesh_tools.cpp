@@ -13,7 +13,12 @@ using namespace rhllc_mesh;
 extern "C" const char *rhllc_mesh_last_error(void) { return err_storage().c_str(); }
 
 // utils/src/get_average_volume.cpp:31-46: r = (3/4) V / sum_j S_j, min over cells.
-extern "C" double rhllc_mesh_h_min(int64_t n_cells, const void *geo_cells, const void *geo_faces) {
+extern "C" double rhllc_mesh_h_min(int64_t n_cells, const void *geo_cells, int64_t n_faces, const void *geo_faces) {
+  if (!geo_cells || !geo_faces || n_cells < 1) return fail("null array or empty mesh"), -1.0;
+  {
+    std::string err;
+    if (!validate_geo(n_cells, geo_cells, n_faces, geo_faces, true, err)) return fail(err), -1.0;
+  }
   const uint8_t *cells = static_cast<const uint8_t *>(geo_cells);
   const uint8_t *faces = static_cast<const uint8_t *>(geo_faces);
   double best = 1e10;
@@ -114,6 +119,10 @@ extern "C" int rhllc_init_radiation(int64_t n_cells, uint64_t seed, double ur_lo
 extern "C" int rhllc_curve_order(int64_t n_cells, const void *geo_cells, int64_t n_faces, const void *geo_faces,
                                  int32_t *cell_new_to_old, int32_t *face_new_to_old) {
   if (!geo_cells || !geo_faces || !cell_new_to_old || !face_new_to_old) return fail("null array"), 1;
+  {
+    std::string err;
+    if (!validate_geo(n_cells, geo_cells, n_faces, geo_faces, true, err)) return fail(err), 1;
+  }
   curve_order_impl(n_cells, geo_cells, n_faces, geo_faces, cell_new_to_old, face_new_to_old);
   return 0;
 }

@@ -83,7 +83,10 @@ class Mesh:
 
     def h_min(self) -> float:
         lib = load_mesh_lib()
-        return float(lib.rhllc_mesh_h_min(self.n_cells, self.cells.ctypes.data, self.faces.ctypes.data))
+        h = float(lib.rhllc_mesh_h_min(self.n_cells, self.cells.ctypes.data, self.n_faces, self.faces.ctypes.data))
+        if h < 0:
+            raise ValueError(lib.rhllc_mesh_last_error().decode())
+        return h
 
     def write(self, faces_path: str, cells_path: str) -> None:
         write_records(faces_path, self.faces)

@@ -209,3 +209,92 @@ def test_table_argument_checks(table):
         s.set_opacity_table(None)
     finally:
         s.close()
+
+
+# ------------------------------------------------ the pressure-tensor branch
+def _tensors(Ur, Fr, rng):
+    """A pressure tensor per cell (FP32): Ur times the M1 Eddington tensor of the moments,
+    plus a non-symmetric perturbation of a tenth of Ur."""
+    n = Ur.size
+    F = Fr.astype(np.float64)
+    U = Ur.astype(np.float64)
+    fm = np.linalg.norm(F, axis=1)
+    nvec = F / np.maximum(fm, 1e-300)[:, None]
+    f2 = (fm / U) ** 2
+    chi = (1.5 + 2 * f2) / (5 + 2 * np.sqrt(np.maximum(4 - 3 * f2, 0)))
+    D = (3 * chi - 0.5)[:, None, None] * nvec[:, :, None] * nvec[:, None, :] + (0.5 - chi)[:, None, None] * np.eye(3)
+    return (U[:, None, None] * (D + 0.1 * rng.normal(size=(n, 3, 3)))).astype(np.float32)
+
+
+def test_eval_rad_source_tensor_bitwise():
+    """GetRadSource with D(j, i) = Tr(j, i) (ISOTROPIC_INTENSITY 0): device FP64 bitwise the oracle."""
+    rng = np.random.default_rng(SEED + 9)
+    n = 4000
+    W = np.column_stack([10 ** rng.uniform(-2, 1, n), 0.5 * rng.uniform(-1, 1, (n, 3)), 10 ** rng.uniform(-2, 1, n)])
+    Ur = 10 ** rng.uniform(-3, 0, n)
+    Fr = rng.normal(size=(n, 3)) * Ur[:, None] * 0.3
+    ka, ks = rng.uniform(0.1, 10, n), rng.uniform(0.1, 10, n)
+    Tr = _tensors(Ur.astype(np.float32), Fr.astype(np.float32), rng)
+    G = capi.eval_rad_source(W, Ur, Fr, ka, ks, Tr=Tr)
+    Go = np.array([O.rad_source(W[i], Ur[i], Fr[i], ka[i], ks[i], Tr=Tr[i]) for i in range(n)])
+    assert np.array_equal(G, Go)
+    G1 = capi.eval_rad_source(W, Ur, Fr, ka, ks)
+    assert not np.array_equal(G, G1)
+
+
+def test_step_radiation_tensor_bitwise_and_mixed():
+    """The fused source with the pressure tensor: FP64 step bitwise the oracle (30 steps of the
+    blast with the seeded radiation field); the mixed step within 1e-5 per-field L1."""
+    m = M.kuhn_box(16, 16, 16, seed=SEED)
+    W0 = M.init_state(m, M.INIT_BLAST)
+    Ur, Fr, ka, ks = M.init_radiation(m.n_cells, seed=SEED)
+    Tr = _tensors(Ur, Fr, np.random.default_rng(SEED + 10))
+    h = m.h_min()
+    st = O.State(m, W0)
+    st.run_rad(0.4, h, 30, Ur, Fr, ka, ks, Tr=Tr)
+    out = {}
+    for prec, guess in ((capi.PREC_FP64, capi.GUESS_REFERENCE), (capi.PREC_MIXED, capi.GUESS_PREVIOUS)):
+        s = capi.Solver(m, cfl=0.4, h_min=h, precision=prec, guess=guess)
+        s.upload_W(W0)
+        s.set_radiation(Ur, Fr, ka, ks)
+        s.set_radiation_tensor(Tr)
+        s.run(30)
+        s.sync()
+        out[prec] = s.download()
+        s.close()
+    W, U = out[capi.PREC_FP64]
+    assert np.array_equal(st.U, U) and np.array_equal(st.W, W)
+    st_m1 = O.State(m, W0)
+    st_m1.run_rad(0.4, h, 30, Ur, Fr, ka, ks)
+    assert not np.array_equal(st_m1.U, U)  # the tensor branch is not the M1 closure here
+    U32 = out[capi.PREC_MIXED][1]
+    for k in range(5):
+        assert np.abs(U32[:, k] - U[:, k]).sum() <= 1e-5 * np.abs(U[:, k]).sum(), k
+
+
+def test_add_rad_flux_tensor_vs_oracle(table):
+    """AddRadFlux as compiled (table opacities, D = Tr(j, i)) + HllcConvToPhys after each hydro
+    step: 10 steps against the oracle within 1e-10 (the table path's exp/log, as above)."""
+    (hdr, data), t = table
+    m = M.kuhn_box(10, 10, 10, seed=SEED)
+    W0, Ur, Fr = physical_blast(m, seed=8, ur_scale=30.0)
+    Tr = _tensors(Ur, Fr, np.random.default_rng(SEED + 11))
+    h = m.h_min()
+    st = O.State(m, W0)
+    st.run_rad(0.4, h, 0, Ur, Fr, table=t)  # p2c only
+    s = capi.Solver(m, cfl=0.4, h_min=h)
+    s.upload_W(W0)
+    s.set_opacity_table((hdr, data))
+    tau = 0.4 * h
+    for _ in range(10):
+        sp_o = st.step_rad(tau, Ur, Fr, table=t, Tr=Tr)
+        sp_g = s.step(tau)
+        s.add_rad_flux(tau, Ur, Fr, Tr=Tr)
+        assert sp_o == sp_g
+        tau = 0.4 * h / sp_o
+    W, U = s.download()
+    s.close()
+    for name, a, b in (("U", U, st.U), ("W", W, st.W)):
+        for k in range(5):
+            ref = np.abs(b[:, k]).max()
+            assert np.abs(a[:, k] - b[:, k]).max() <= 1e-10 * ref, (name, k)

@@ -80,11 +80,12 @@ def test_eval_c2p_p2c_fix_bitwise():
     assert np.array_equal(fo, fg) and np.array_equal(Uo2, Ug2) and np.array_equal(Wo2, Wg2)
 
 
-def _run_both(mesh, W0, cfl, steps, gamma=5.0 / 3.0, graph=True, reorder=True, eos=capi.EOS_IDEAL):
+def _run_both(mesh, W0, cfl, steps, gamma=5.0 / 3.0, graph=True, reorder=True, eos=capi.EOS_IDEAL,
+              schedule=capi.SCHED_TILED):
     h = mesh.h_min()
     st = O.State(mesh, W0, gamma, eos=eos)
     t_o, taus_o, speeds_o = st.run(cfl, h, steps)
-    s = capi.Solver(mesh, gamma=gamma, cfl=cfl, h_min=h, reorder=reorder, eos=eos)
+    s = capi.Solver(mesh, gamma=gamma, cfl=cfl, h_min=h, reorder=reorder, eos=eos, schedule=schedule)
     s.upload_W(W0)
     s.run(steps, graph=graph)
     s.sync()
@@ -94,12 +95,14 @@ def _run_both(mesh, W0, cfl, steps, gamma=5.0 / 3.0, graph=True, reorder=True, e
     return st, (t_o, taus_o, speeds_o), (Wg, Ug, stats)
 
 
-def test_step_shock_tube_200_steps_bitwise():
-    """Config 0: relativistic shock tube, 120k jittered/permuted tets, CFL 0.4, 200 steps."""
+@pytest.mark.parametrize("schedule", [capi.SCHED_TILED, capi.SCHED_TWO_KERNEL])
+def test_step_shock_tube_200_steps_bitwise(schedule):
+    """Config 0: relativistic shock tube, 120k jittered/permuted tets, CFL 0.4, 200 steps,
+    under both kernel schedules."""
     m = M.kuhn_box(200, 10, 10, length=(1.0, 0.05, 0.05), seed=SEED)
     assert (m.n_cells, m.n_faces) == (120000, 248200)
     W0 = M.init_state(m, M.INIT_SHOCK_TUBE)
-    st, (t_o, taus, speeds), (Wg, Ug, stats) = _run_both(m, W0, 0.4, 200)
+    st, (t_o, taus, speeds), (Wg, Ug, stats) = _run_both(m, W0, 0.4, 200, schedule=schedule)
     assert stats["steps"] == 200
     assert np.array_equal(st.U, Ug)
     assert np.array_equal(st.W, Wg)
@@ -187,7 +190,8 @@ def test_mixed_precision_blast_l1():
     assert max(l1) < 1e-5
 
 
-def test_step_radiation_source_bitwise():
+@pytest.mark.parametrize("schedule", [capi.SCHED_TILED, capi.SCHED_TWO_KERNEL])
+def test_step_radiation_source_bitwise(schedule):
     """Full-form coupling: hydro step, U += tau*G(W) (M1 closure), c2p -- as radRHD_main.cpp:99-132."""
     m = M.kuhn_box(16, 16, 16, seed=SEED)
     W0 = M.init_state(m, M.INIT_BLAST)
@@ -195,7 +199,7 @@ def test_step_radiation_source_bitwise():
     h = m.h_min()
     st = O.State(m, W0)
     st.run_rad(0.4, h, 30, Ur, Fr, ka, ks)
-    s = capi.Solver(m, cfl=0.4, h_min=h)
+    s = capi.Solver(m, cfl=0.4, h_min=h, schedule=schedule)
     s.upload_W(W0)
     s.set_radiation(Ur, Fr, ka, ks)
     s.run(30)

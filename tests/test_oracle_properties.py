@@ -262,6 +262,51 @@ def test_rad_source_is_the_covariant_four_force():
     assert np.allclose(G, [2.0 * 3.0 * 0.5, 2 * 4.5 * 0.1, -2 * 4.5 * 0.2, 2 * 4.5 * 0.05], rtol=1e-14)
 
 
+def test_rad_source_tensor_branch_is_the_covariant_four_force():
+    """The reference as compiled (radRHD_config.h:8, ISOTROPIC_INTENSITY 0) takes D(j, i) = Tr(j, i)
+    of the moments' pressure tensor and multiplies it by Ur as in the M1 branch
+    (radRHD_utils.cpp:80-86, 137-143).  That is the covariant G^mu of the M1 test with
+    R^{ij} = Ur Tr(j, i): checked with non-symmetric tensors, which tell Tr(j, i) from Tr(i, j).
+    Tr = the M1 tensor where that is exact in FP32 (|F| > Ur along an axis: D = e e^T) gives the
+    M1 source to the bit, and the GetRadSourceOpt form takes the same tensor."""
+    rng = np.random.default_rng(22)
+    eta = np.diag([-1.0, 1.0, 1.0, 1.0])
+    for k in range(300):
+        rho = 10 ** rng.uniform(-2, 1)
+        v = rng.normal(size=3)
+        v *= rng.uniform(0, 0.95) / np.linalg.norm(v)
+        Ur = 10 ** rng.uniform(-3, 0)
+        F = rng.normal(size=3) * Ur * 0.4
+        ka, ks = rng.uniform(0.1, 10, 2)
+        Tr = rng.normal(size=(3, 3)).astype(np.float32)  # not symmetric
+        g = 1.0 / np.sqrt(1.0 - v @ v)
+        u_up = np.concatenate([[g], g * v])
+        u_dn = eta @ u_up
+        R = np.zeros((4, 4))
+        R[0, 0] = Ur
+        R[0, 1:] = R[1:, 0] = F
+        R[1:, 1:] = Ur * Tr.astype(np.float64).T  # R^{ij} = Ur Tr(j, i)
+        G_cov = -rho * (ka + ks) * (R @ u_dn) - rho * ks * (u_dn @ R @ u_dn) * u_up
+        W = np.concatenate([[rho], v, [1.0]])
+        G = O.rad_source(W, Ur, F, ka, ks, Tr=Tr)
+        scale = rho * (ka + ks) * Ur * g**3 * 4 * (1 + np.abs(Tr).max())
+        assert np.abs(G - G_cov).max() <= 1e-12 * scale, (k, G, G_cov)
+        R[1:, 1:] = Ur * Tr.astype(np.float64)  # the other index order differs
+        G_t = -rho * (ka + ks) * (R @ u_dn) - rho * ks * (u_dn @ R @ u_dn) * u_up
+        assert np.abs(G - G_t).max() > 1e-9 * scale
+        # GetRadSourceOpt's form with the same opacities, no emission term (T = 0)
+        cell = [0.0, np.linalg.norm(v), g, ka, ks]
+        Go = O.rad_source_opt(W, cell, Ur, F, Tr=Tr)
+        assert np.abs(Go - G).max() <= 1e-12 * scale
+    for axis in range(3):
+        F = np.zeros(3)
+        F[axis] = 0.8
+        Tr = np.zeros((3, 3), np.float32)
+        Tr[axis, axis] = 1.0
+        W = [1.5, 0.3, -0.2, 0.4, 2.0]
+        assert np.array_equal(O.rad_source(W, 0.5, F, 2.0, 3.0, Tr=Tr), O.rad_source(W, 0.5, F, 2.0, 3.0))
+
+
 # ------------------------------------------------ Taub-Mathews EOS (closed forms)
 def test_taub_enthalpy_identity_and_sound_speed():
     """Taub-Mathews: (h - theta)(h - 4 theta) = 1, and c_s^2 = (dp/drho)_s / h along an
