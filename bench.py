@@ -304,7 +304,7 @@ def main():
             "start, tiled step (faces into LDS, cells from LDS), hipGraph-replayed",
             "n_cells": N,
             "n_faces": F,
-            "c2p_start": "MLP surrogate where tau changed > 2% in the step, previous pressure elsewhere",
+            "c2p_start": "MLP surrogate where tau changed > 5% in the step, previous pressure elsewhere",
             "parallelism": "single GPU",
         },
         "roofline": {

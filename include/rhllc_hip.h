@@ -55,7 +55,7 @@ extern "C" {
 #define RHLLC_GUESS_REFERENCE 0 /* max(|m|-E, 0, pmin) as rhllc_flux_stab.cpp:139-151 */
 #define RHLLC_GUESS_PREVIOUS 1  /* the previous step's pressure */
 #define RHLLC_GUESS_MLP 2       /* the MLP surrogate (rhllc_set_mlp) in cells whose tau = E - D
-                                   changed by more than 2% in the step (the previous pressure
+                                   changed by more than 5% in the step (the previous pressure
                                    elsewhere); then the previous pressure, the reference start
                                    and PhysPressureFix as fallbacks */
 

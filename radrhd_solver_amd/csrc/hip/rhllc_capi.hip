@@ -190,7 +190,10 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
   *out = nullptr;
   if (n_cells < 1 || n_cells >= ((int64_t)1 << 25))
     return fail(RHLLC_ERR_ARG, "n_cells must be in [1, 2^25) (geo_face_t::id_l is 25 bits)");
-  if (n_faces < 1 || n_faces > 0x7FFFFFFF) return fail(RHLLC_ERR_ARG, "n_faces out of range");
+  // Faces below 2^27 (a mesh of 2^25 tets has at most 4 * 2^25 faces): the
+  // kernels address the face arrays (at most 32 bytes an element) with 32-bit
+  // byte offsets.
+  if (n_faces < 1 || n_faces > ((int64_t)1 << 27)) return fail(RHLLC_ERR_ARG, "n_faces out of range [1, 2^27]");
   const rhllc_params &p = *params;
   if (p.eos != RHLLC_EOS_IDEAL && p.eos != RHLLC_EOS_TAUB) return fail(RHLLC_ERR_ARG, "unknown eos");
   if (p.eos == RHLLC_EOS_IDEAL && !(p.gamma > 1.0)) return fail(RHLLC_ERR_ARG, "gamma must be > 1");

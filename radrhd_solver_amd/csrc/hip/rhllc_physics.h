@@ -68,6 +68,7 @@ struct Ieee {
   __device__ static __forceinline__ double div(double a, double b) { return a / b; }
   __device__ static __forceinline__ double rcp(double x) { return 1.0 / x; }
   __device__ static __forceinline__ double sqrt_(double x) { return sqrt(x); }
+  __device__ static __forceinline__ double rsqrt(double x) { return 1.0 / sqrt(x); }
 };
 struct Fast {
   static constexpr bool kFast = true;
@@ -299,15 +300,18 @@ __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<doub
       lor = A::sqrt_(lor2);
       mlor3 = A::div(m2 * lor2 * lor, alpha2 * alpha);
     } else {
+      // One refined reciprocal root y = 1/sqrt(alpha^2 - m^2) gives
+      // Gamma = alpha y (clamped to >= 1: rounding takes it to 1 - ulp for gas
+      // at rest) and m^2 Gamma^3 / alpha^3 = m^2 y^3.
       const double q = alpha2 - m2;
       if (!(q > 0.0)) {
         iters = iter + 1;
         return 5;
       }
-      const double r = A::rcp(alpha * q);
-      lor2 = smax(alpha2 * alpha * r, 1.0);
-      lor = A::sqrt_(lor2);
-      mlor3 = m2 * lor * r;
+      const double y = A::rsqrt(q);
+      lor = smax(alpha * y, 1.0);
+      lor2 = lor * lor;
+      mlor3 = m2 * (y * y) * y;
     }
     tau = lor * D_1;
     theta = p * tau;
@@ -323,7 +327,9 @@ __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<doub
     }
     yp = D * h * lor - E - p;
     dyp = D * lor * dh_dp - mlor3 * (lor * dh_dtau + D * h) - 1.0;
-    dp = A::div(yp, dyp);
+    // Fast: the quotient to one Newton step of the reciprocal (4e-15): the
+    // iteration's fixed point is yp = 0 whatever the accuracy of dyp.
+    dp = A::kFast ? yp * FaceFast::rcp(dyp) : A::div(yp, dyp);
     p -= dp;
     if (p < pmin) p = pmin;
     if (fabs(dp) < tol * E) break;
@@ -674,6 +680,30 @@ __device__ __forceinline__ void edd_matrix(double U, const double *F, double D[3
     }
 }
 
+// The M1 tensor of edd_matrix as D = c n n^T + dg I: |F| or U below 1e-150:
+// c = 0, dg = 0.33333333333333 (n = 0); |F| > U: c = 1, dg = 0 (free
+// streaming); else c = 3 chi - 1/2, dg = 1/2 - chi with the same chi.
+template <typename A>
+__device__ __forceinline__ void m1_closure(double U, const double *F, double &c, double &dg, double n[3]) {
+  double fm = dot3(F, F);
+  if (fm < 1e-150 || U < 1e-150) {
+    c = 0.0, dg = 0.33333333333333, n[0] = n[1] = n[2] = 0.0;
+    return;
+  }
+  fm = A::sqrt_(fm);
+  const double r = A::rcp(fm);
+  n[0] = F[0] * r, n[1] = F[1] * r, n[2] = F[2] * r;
+  if (fm > U) {
+    c = 1.0, dg = 0.0;
+    return;
+  }
+  double f2 = A::div(fm, U);
+  f2 = f2 * f2;
+  const double chi_2 = A::div(1.5 + 2.0 * f2, 5.0 + 2.0 * A::sqrt_(4.0 - 3.0 * f2));
+  c = 3.0 * chi_2 - 0.5;
+  dg = 0.5 - chi_2;
+}
+
 // Physical constants of the source, lib/global_consts.h:12-83 (CGS branch)
 // evaluated as the reference's constexpr chain, and the three logarithms its
 // header generator writes into log_global_consts.h (resources/parser_header.cpp,
@@ -723,26 +753,40 @@ __device__ __forceinline__ void rad_source_tail(const State<double> &W, double U
   const double tot_op = abs_op + scat_op;
   const double rho = W.d;
   const double u[3] = {gamma * W.v[0], gamma * W.v[1], gamma * W.v[2]};
-  double Dm[3][3];
-  if (Tr9) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b) Dm[a][b] = (double)Tr9[3 * a + b];
-  } else {
-    edd_matrix<A>(Ur, Fr, Dm);
-  }
   double uuD = 0.;
   const double uF = dot3(u, Fr);
   double uD[3];
+  if (A::kFast && !Tr9) {
+    // The M1 tensor is c n n^T + dg I (n = F/|F|), so u.D = c (u.n) n + dg u and
+    // u.D.u = c (u.n)^2 + dg u^2: the products the tensor loops below form,
+    // without the tensor.
+    double c, dg, n[3];
+    m1_closure<A>(Ur, Fr, c, dg, n);
+    const double un = dot3(u, n);
+    const double cun = c * un;
+    uD[0] = cun * n[0] + dg * u[0];
+    uD[1] = cun * n[1] + dg * u[1];
+    uD[2] = cun * n[2] + dg * u[2];
+    uuD = cun * un + dg * dot3(u, u);
+  } else {
+    double Dm[3][3];
+    if (Tr9) {
 #pragma unroll
-  for (int i = 0; i < 3; i++) {
-    uD[i] = 0.;
+      for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-      const double D = Dm[j][i];  // EddTensor(j, i, Ur, Fr)
-      uD[i] += u[j] * D;
-      uuD += u[i] * u[j] * D;
+        for (int b = 0; b < 3; ++b) Dm[a][b] = (double)Tr9[3 * a + b];
+    } else {
+      edd_matrix<A>(Ur, Fr, Dm);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      uD[i] = 0.;
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        const double D = Dm[j][i];  // EddTensor(j, i, Ur, Fr) or Tr(j, i)
+        uD[i] += u[j] * D;
+        uuD += u[i] * u[j] * D;
+      }
     }
   }
   const double rhogamma = rho * gamma;
