@@ -287,6 +287,20 @@ int rhllc_mlp_eval(int32_t device, const float *model, int64_t n, const double *
 int rhllc_mlp_grad(int32_t device, const float *model, int32_t n, const float *X, const float *y, const float *ok,
                    float *grad, float *loss);
 
+/* rhllc_mlp_grad on device arrays, allocating nothing: model (RHLLC_MLP_MODEL
+ * floats), X (n x 3), y, ok (n each), grad (RHLLC_MLP_PARAMS floats) and loss
+ * (1 float) are device pointers; the work is enqueued on `stream` (a
+ * hipStream_t; NULL = the null stream) and the caller synchronizes.  The
+ * packed weights, per-wave partials and losses live in a scratch array the
+ * library holds on each GPU (18.4 MB, for n <= RHLLC_MLP_NOALLOC_MAX_N; larger
+ * n is RHLLC_ERR_ARG); grad and loss are the same bits as rhllc_mlp_grad's.
+ * Calls that overlap on one GPU share that scratch and race.  No counterpart
+ * in the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight training loops). */
+#define RHLLC_MLP_NOALLOC_MAX_N 65536
+int rhllc_mlp_grad_noalloc(int32_t device, const float *model, int32_t n, const float *X, const float *y,
+                           const float *ok, float *grad, float *loss, void *stream);
+
 /* Runs one v_mfma_f32_16x16x16_bf16 on known operands; out[64*4] = the
  * accumulator registers per lane, to check the lane layout the kernels use. */
 int rhllc_mlp_selftest(int32_t device, float *out);

@@ -118,6 +118,7 @@ SIGNATURES = {
     "rhllc_mlp_eval": (C.c_int, [_i32, _vp, _i64, _vp, _vp]),
     "rhllc_mlp_grad": (C.c_int, [_i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_mlp_selftest": (C.c_int, [_i32, _vp]),
+    "rhllc_mlp_grad_noalloc": (C.c_int, [_i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_set_mlp": (C.c_int, [_vp, _vp]),
     "rhllc_last_error": (C.c_char_p, []),
 }

@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -213,5 +214,65 @@ extern "C" int rhllc_mlp_train(const rhllc_mlp_config *cfg, float *model_out, fl
     HIP_TRY(hipMemcpy(loss_hist, H.p, sizeof(float) * cfg->steps, hipMemcpyDeviceToHost));
     for (int k = 0; k < cfg->steps; ++k) loss_hist[k] /= (float)B;
   }
+  return RHLLC_OK;
+}
+
+// ---------------------------------------------------------------- no-alloc
+// Scratch of rhllc_mlp_grad_noalloc: the packed BF16 weights (2 x 64 x 64
+// shorts = 4096 floats), the per-wave gradient partials (kParams floats a
+// wave) and the per-wave losses, for batches of up to kNoAllocMaxN samples.
+// Each GPU holds its own copy until the process ends.  Host code uses the
+// device address hipGetSymbolAddress returns, never the array's name.
+namespace {
+constexpr int kNoAllocMaxN = 65536;
+constexpr int kNoAllocMaxWaves = kNoAllocMaxN / 64;
+constexpr size_t kScratchW2 = 0;                                              // floats
+constexpr size_t kScratchGrad = kScratchW2 + (size_t)kH * kH;                 // 2 x 4096 shorts
+constexpr size_t kScratchLoss = kScratchGrad + (size_t)kParams * kNoAllocMaxWaves;
+constexpr size_t kScratchFloats = kScratchLoss + kNoAllocMaxWaves;
+}  // namespace
+
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<float *> g_scratch_ptr;  // per device; null until its first call
+
+// The device address of `scratch` on the current device (looked up once per device).
+int scratch_on(int device, float **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return fail(RHLLC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<float *>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return RHLLC_OK;
+}
+}  // namespace
+
+extern "C" int rhllc_mlp_grad_noalloc(int32_t device, const float *model, int32_t n, const float *X, const float *y,
+                                      const float *ok, float *grad, float *loss, void *stream) {
+  if (!model || !X || !y || !ok || !grad || !loss || n <= 0 || n % 256)
+    return fail(RHLLC_ERR_ARG, "bad argument (n must be a positive multiple of 256)");
+  if (n > kNoAllocMaxN)
+    return fail(RHLLC_ERR_ARG, "n = " + std::to_string(n) + " exceeds the no-alloc scratch (at most " +
+                                   std::to_string(kNoAllocMaxN) + " samples a call)");
+  if (device < 0 || device > 4096) return fail(RHLLC_ERR_ARG, "bad device ordinal");
+  HIP_TRY(hipSetDevice(device));
+  float *sc = nullptr;
+  if (int e = scratch_on(device, &sc)) return e;
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  const int nw = n / 64;
+  short *W2b = reinterpret_cast<short *>(sc + kScratchW2);
+  float *part = sc + kScratchGrad, *lpart = sc + kScratchLoss;
+  // Every scratch element these kernels read is written first in the same call
+  // (pack: all of W2; grad: nw partial rows and nw losses), so nothing is cleared.
+  HIP_TRY(launch_pack(model, W2b, W2b + kH * kH, st));
+  HIP_TRY(launch_grad(model, W2b, W2b + kH * kH, X, y, ok, 1.0f / (float)n, n, part, lpart, st));
+  HIP_TRY(launch_sum_partials(part, nw, kParams, grad, st));
+  HIP_TRY(launch_sum_partials(lpart, nw, 1, loss, st));
   return RHLLC_OK;
 }

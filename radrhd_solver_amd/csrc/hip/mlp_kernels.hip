@@ -289,6 +289,16 @@ __global__ void k_mlp_adam(float *P, float *Mo, float *Vo, short *W2b, short *W2
 
 __global__ void k_mlp_tick(uint32_t *step) { *step += 1u; }
 
+// out[p] = sum over w < nw of part[w * np + p], added in w order in FP32 (the
+// order of the host reduction of rhllc_mlp_grad).
+__global__ void k_mlp_sum_partials(const float *part, int nw, int np, float *out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  float s = 0.f;
+  for (int w = 0; w < nw; ++w) s += part[(size_t)w * np + p];
+  out[p] = s;
+}
+
 __global__ void k_mlp_pack(const float *P, short *W2b, short *W2Tb) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= kH * kH) return;
@@ -355,6 +365,10 @@ hipError_t launch_adam(float *P, float *M, float *V, short *W2b, short *W2Tb, co
                        hipStream_t s) {
   hipLaunchKernelGGL(k_mlp_adam, dim3((kParams + 255) / 256), dim3(256), 0, s, P, M, V, W2b, W2Tb, grad, nwaves, step,
                      total_steps, lr0, loss, loss_hist);
+  return hipGetLastError();
+}
+hipError_t launch_sum_partials(const float *part, int nw, int np, float *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_mlp_sum_partials, dim3((np + 255) / 256), dim3(256), 0, s, part, nw, np, out);
   return hipGetLastError();
 }
 hipError_t launch_tick(uint32_t *step, hipStream_t s) {
