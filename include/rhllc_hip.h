@@ -233,8 +233,8 @@ int rhllc_eval_c2p(int32_t precision, int32_t eos, double gamma, int64_t n, cons
                    double *W, int32_t *codes, int32_t *iters);
 int rhllc_eval_p2c(int32_t eos, double gamma, int64_t n, double *W, double *U);
 int rhllc_eval_pressure_fix(int32_t eos, double gamma, int64_t n, double *U, double *W, int32_t *codes);
-int rhllc_eval_rad_source(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
-                          const double *ks, int32_t emission, const float *Tr9, double *G);
+int rhllc_eval_rad_source(int32_t precision, int64_t n, const double *W, const double *Ur, const double *Fr3,
+                          const double *ka, const double *ks, int32_t emission, const float *Tr9, double *G);
 /* GetRadSourceOpt with the table's opacities: G (4 per state) and the cell
  * data (T, vel, lorenz, alpha, betta: 5 per state).  For both: Tr9 (9 FP32
  * per state, row-major) selects the pressure-tensor branch, NULL the M1

@@ -110,7 +110,7 @@ SIGNATURES = {
     "rhllc_eval_c2p": (C.c_int, [_i32, _i32, _dbl, _i64, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_eval_p2c": (C.c_int, [_i32, _dbl, _i64, _vp, _vp]),
     "rhllc_eval_pressure_fix": (C.c_int, [_i32, _dbl, _i64, _vp, _vp, _vp]),
-    "rhllc_eval_rad_source": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp]),
+    "rhllc_eval_rad_source": (C.c_int, [_i32, _i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp]),
     "rhllc_eval_rad_source_opt": (C.c_int, [_i32, _i32, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_device_query": (C.c_int, [_i32, C.c_char_p, _i32, _vp, _vp]),
     "rhllc_copy_probe": (C.c_int, [_i32, _i64, _i32, _vp]),
@@ -354,14 +354,14 @@ def _tensor(Tr, n):
     return None if Tr is None else np.ascontiguousarray(Tr, dtype=np.float32).reshape(n, 9)
 
 
-def eval_rad_source(W, Ur, Fr, ka, ks, emission=False, Tr=None):
+def eval_rad_source(W, Ur, Fr, ka, ks, emission=False, Tr=None, precision=PREC_FP64):
     W = np.ascontiguousarray(W, dtype=np.float64).reshape(-1, 5)
     n = W.shape[0]
     Ur, ka, ks = (np.ascontiguousarray(x, dtype=np.float64).reshape(n) for x in (Ur, ka, ks))
     Fr = np.ascontiguousarray(Fr, dtype=np.float64).reshape(n, 3)
     T = _tensor(Tr, n)
     G = np.zeros((n, 4))
-    check(_lib().rhllc_eval_rad_source(n, W.ctypes.data, Ur.ctypes.data, Fr.ctypes.data, ka.ctypes.data,
+    check(_lib().rhllc_eval_rad_source(precision, n, W.ctypes.data, Ur.ctypes.data, Fr.ctypes.data, ka.ctypes.data,
                                        ks.ctypes.data, 1 if emission else 0, _ptr(T), G.ctypes.data))
     return G
 

@@ -941,10 +941,11 @@ extern "C" int rhllc_eval_pressure_fix(int32_t eos, double gamma, int64_t n, dou
   return RHLLC_OK;
 }
 
-extern "C" int rhllc_eval_rad_source(int64_t n, const double *W, const double *Ur, const double *Fr3,
-                                     const double *ka, const double *ks, int32_t emission, const float *Tr9,
-                                     double *G) {
+extern "C" int rhllc_eval_rad_source(int32_t precision, int64_t n, const double *W, const double *Ur,
+                                     const double *Fr3, const double *ka, const double *ks, int32_t emission,
+                                     const float *Tr9, double *G) {
   if (n < 0 || !W || !Ur || !Fr3 || !ka || !ks || !G) return fail(RHLLC_ERR_ARG, "bad argument");
+  if (precision != RHLLC_PREC_FP64 && precision != RHLLC_PREC_MIXED) return fail(RHLLC_ERR_ARG, "bad precision");
   DevBuf a, b, c, dd, e, f, t;
   if (Tr9) HIP_TRY(to_dev(t, Tr9, sizeof(float) * 9 * n));
   HIP_TRY(to_dev(a, W, sizeof(double) * 5 * n));
@@ -953,8 +954,8 @@ extern "C" int rhllc_eval_rad_source(int64_t n, const double *W, const double *U
   HIP_TRY(to_dev(dd, ka, sizeof(double) * n));
   HIP_TRY(to_dev(e, ks, sizeof(double) * n));
   HIP_TRY(to_dev(f, nullptr, sizeof(double) * 4 * n));
-  HIP_TRY(launch_eval_rad(n, (double *)a.p, (double *)b.p, (double *)c.p, (double *)dd.p, (double *)e.p,
-                          emission, (const float *)t.p, (double *)f.p, nullptr));
+  HIP_TRY(launch_eval_rad(precision == RHLLC_PREC_MIXED, n, (double *)a.p, (double *)b.p, (double *)c.p,
+                          (double *)dd.p, (double *)e.p, emission, (const float *)t.p, (double *)f.p, nullptr));
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(G, f.p, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
   return RHLLC_OK;

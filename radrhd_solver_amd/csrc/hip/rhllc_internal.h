@@ -135,8 +135,9 @@ hipError_t launch_eval_c2p(bool mixed, int eos, double gamma, int64_t n, const d
                            double *W, int32_t *codes, int32_t *iters, hipStream_t s);
 hipError_t launch_eval_p2c(int eos, double gamma, int64_t n, double *W, double *U, hipStream_t s);
 hipError_t launch_eval_fix(int eos, double gamma, int64_t n, double *U, double *W, int32_t *codes, hipStream_t s);
-hipError_t launch_eval_rad(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
-                           const double *ks, int emission, const float *Tr9, double *G, hipStream_t s);
+hipError_t launch_eval_rad(bool mixed, int64_t n, const double *W, const double *Ur, const double *Fr3,
+                           const double *ka, const double *ks, int emission, const float *Tr9, double *G,
+                           hipStream_t s);
 // hdr = {min_x, max_x, step_x, min_y, max_y, step_y}; host pointer, read at launch.
 hipError_t launch_eval_rad_opt(int nx, int ny, const double *hdr, const double *table, int64_t n, const double *W,
                                const double *Ur, const double *Fr3, const float *Tr9, double *G, double *cell,

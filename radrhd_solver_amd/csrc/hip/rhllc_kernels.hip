@@ -277,15 +277,16 @@ __device__ __forceinline__ void update_U(const Dev &d, int c, double tau, const 
                                          double &dtau, float &p32) {
   State<double> s;
   s.d = 0.0, s.v[0] = 0.0, s.v[1] = 0.0, s.v[2] = 0.0, s.p = 0.0;
+  // s +- F as fma(+-1, F, s): the product is exact, so each sum is rounded
+  // once, as the reference's += and -= are (signed zeros included), without
+  // a branch on the sign.
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     State<double> F;
     src.load(j, F);
-    if (!src.neg(j)) {
-      s.d += F.d, s.v[0] += F.v[0], s.v[1] += F.v[1], s.v[2] += F.v[2], s.p += F.p;
-    } else {
-      s.d -= F.d, s.v[0] -= F.v[0], s.v[1] -= F.v[1], s.v[2] -= F.v[2], s.p -= F.p;
-    }
+    const double sg = src.neg(j) ? -1.0 : 1.0;
+    s.d = fma(sg, F.d, s.d), s.v[0] = fma(sg, F.v[0], s.v[0]), s.v[1] = fma(sg, F.v[1], s.v[1]);
+    s.v[2] = fma(sg, F.v[2], s.v[2]), s.p = fma(sg, F.p, s.p);
   }
   load_U(d, c, U);
   double k;
@@ -493,8 +494,10 @@ __global__ __launch_bounds__(kBlock) void k_cell(Dev d) {
 // is the step's input buffer; the cell phase writes the output buffer, so no
 // tile reads what another tile of the same step writes.  A face shared by two
 // tiles is evaluated by both from the same inputs, to the same bits.
+// At most 96 VGPRs (five waves a SIMD): without the floor the compiler takes
+// 116 for the mixed variants and the kernel runs at four waves.
 template <int VARIANT, bool RAD, bool STATS, int EOS>
-__global__ __launch_bounds__(kBlock) void k_tile(Dev d) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_tile(Dev d) {
   static_assert(kTileCells == kBlock, "one cell per thread in the cell phase");
   constexpr bool MIXED = VARIANT >= kMixedPrev;
   extern __shared__ double2 lds_flux[];
@@ -943,13 +946,14 @@ __global__ void k_eval_fix(double gamma, int64_t n, double *U, double *W, int32_
   st5(W + 5 * i, w);
 }
 
+template <typename A>
 __global__ void k_eval_rad(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
                            const double *ks, int emission, const float *Tr9, double *G) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   State<double> w;
   ld5(W + 5 * i, w);
-  rad_source(w, Ur[i], Fr3 + 3 * i, ka[i], ks[i], G + 4 * i, emission != 0, Tr9 ? Tr9 + 9 * i : nullptr);
+  rad_source<A>(w, Ur[i], Fr3 + 3 * i, ka[i], ks[i], G + 4 * i, emission != 0, Tr9 ? Tr9 + 9 * i : nullptr);
 }
 
 __global__ void k_eval_rad_opt(OpacityTable t, int64_t n, const double *W, const double *Ur, const double *Fr3,
@@ -1028,10 +1032,14 @@ hipError_t launch_eval_fix(int eos, double gamma, int64_t n, double *U, double *
     hipLaunchKernelGGL(k_eval_fix<kIdeal>, eval_grid(n), dim3(256), 0, s, gamma, n, U, W, codes);
   return hipGetLastError();
 }
-hipError_t launch_eval_rad(int64_t n, const double *W, const double *Ur, const double *Fr3, const double *ka,
-                           const double *ks, int emission, const float *Tr9, double *G, hipStream_t s) {
+hipError_t launch_eval_rad(bool mixed, int64_t n, const double *W, const double *Ur, const double *Fr3,
+                           const double *ka, const double *ks, int emission, const float *Tr9, double *G,
+                           hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_eval_rad, eval_grid(n), dim3(256), 0, s, n, W, Ur, Fr3, ka, ks, emission, Tr9, G);
+  if (mixed)
+    hipLaunchKernelGGL(k_eval_rad<Fast>, eval_grid(n), dim3(256), 0, s, n, W, Ur, Fr3, ka, ks, emission, Tr9, G);
+  else
+    hipLaunchKernelGGL(k_eval_rad<Ieee>, eval_grid(n), dim3(256), 0, s, n, W, Ur, Fr3, ka, ks, emission, Tr9, G);
   return hipGetLastError();
 }
 

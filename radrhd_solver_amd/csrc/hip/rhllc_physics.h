@@ -507,8 +507,8 @@ __device__ __forceinline__ void mixed_record(double gamma, double Gg, const Stat
   const double rho = (float)W.d, p = va.w;
   const double v[3] = {va.x, va.y, va.z};
   const double vv = dot3(v, v);
-  const double lor2 = A::rcp(1.0 - vv);
-  const double lor = A::sqrt_(lor2);
+  const double lor = A::rsqrt(1.0 - vv);  // one reciprocal root: 4.2e-15, far below the record's 2^-31
+  const double lor2 = lor * lor;
   double rhm1, cs2;  // rho (h - 1), sound speed^2
   if (EOS == kIdeal) {
     rhm1 = Gg * p;
@@ -680,30 +680,6 @@ __device__ __forceinline__ void edd_matrix(double U, const double *F, double D[3
     }
 }
 
-// The M1 tensor of edd_matrix as D = c n n^T + dg I: |F| or U below 1e-150:
-// c = 0, dg = 0.33333333333333 (n = 0); |F| > U: c = 1, dg = 0 (free
-// streaming); else c = 3 chi - 1/2, dg = 1/2 - chi with the same chi.
-template <typename A>
-__device__ __forceinline__ void m1_closure(double U, const double *F, double &c, double &dg, double n[3]) {
-  double fm = dot3(F, F);
-  if (fm < 1e-150 || U < 1e-150) {
-    c = 0.0, dg = 0.33333333333333, n[0] = n[1] = n[2] = 0.0;
-    return;
-  }
-  fm = A::sqrt_(fm);
-  const double r = A::rcp(fm);
-  n[0] = F[0] * r, n[1] = F[1] * r, n[2] = F[2] * r;
-  if (fm > U) {
-    c = 1.0, dg = 0.0;
-    return;
-  }
-  double f2 = A::div(fm, U);
-  f2 = f2 * f2;
-  const double chi_2 = A::div(1.5 + 2.0 * f2, 5.0 + 2.0 * A::sqrt_(4.0 - 3.0 * f2));
-  c = 3.0 * chi_2 - 0.5;
-  dg = 0.5 - chi_2;
-}
-
 // Physical constants of the source, lib/global_consts.h:12-83 (CGS branch)
 // evaluated as the reference's constexpr chain, and the three logarithms its
 // header generator writes into log_global_consts.h (resources/parser_header.cpp,
@@ -746,6 +722,17 @@ __device__ __forceinline__ double b_planck(double T) {
 // Tr(j, i) = Tr9[3 j + i]), D(j, i) = Tr(j, i) as the reference is compiled
 // (radRHD_config.h:8, ISOTROPIC_INTENSITY 0; radRHD_utils.cpp:80-86, 137-143),
 // multiplied by Ur below as there.
+//
+// The mixed path (A = Fast) with the M1 tensor evaluates the same source in a
+// compact form.  With D = c n n^T + dg I and n = F/|F|, u.D = cf (u.F) F + dg u
+// and u.D.u = cf (u.F)^2 + dg u.u, cf = c/|F|^2, so no unit vector, |F| or
+// tensor is formed, and each spatial component is G_i = a u_i + b F_i with
+//   a = q - rho Ur (k_t dg + k_s (gamma^2 + uuD)) + 2 rho gamma k_s (u.F),
+//   b = rho k_t (gamma - Ur cf (u.F)),
+// the reference's terms regrouped.  The closure's cases compare squares:
+// F.F < 1e-150 (as the reference does) or Ur < 1e-150: cf = 0, dg =
+// 0.33333333333333; |F|^2 > Ur^2: cf = 1/|F|^2, dg = 0; else chi of
+// f^2 = |F|^2 / Ur^2 as EddTensor has it.
 template <typename A>
 __device__ __forceinline__ void rad_source_tail(const State<double> &W, double Ur, const double *Fr, double abs_op,
                                                 double scat_op, double q, double gamma, double gamma2, double u2,
@@ -753,40 +740,52 @@ __device__ __forceinline__ void rad_source_tail(const State<double> &W, double U
   const double tot_op = abs_op + scat_op;
   const double rho = W.d;
   const double u[3] = {gamma * W.v[0], gamma * W.v[1], gamma * W.v[2]};
-  double uuD = 0.;
   const double uF = dot3(u, Fr);
-  double uD[3];
   if (A::kFast && !Tr9) {
-    // The M1 tensor is c n n^T + dg I (n = F/|F|), so u.D = c (u.n) n + dg u and
-    // u.D.u = c (u.n)^2 + dg u^2: the products the tensor loops below form,
-    // without the tensor.
-    double c, dg, n[3];
-    m1_closure<A>(Ur, Fr, c, dg, n);
-    const double un = dot3(u, n);
-    const double cun = c * un;
-    uD[0] = cun * n[0] + dg * u[0];
-    uD[1] = cun * n[1] + dg * u[1];
-    uD[2] = cun * n[2] + dg * u[2];
-    uuD = cun * un + dg * dot3(u, u);
-  } else {
-    double Dm[3][3];
-    if (Tr9) {
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) Dm[a][b] = (double)Tr9[3 * a + b];
-    } else {
-      edd_matrix<A>(Ur, Fr, Dm);
-    }
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-      uD[i] = 0.;
-#pragma unroll
-      for (int j = 0; j < 3; j++) {
-        const double D = Dm[j][i];  // EddTensor(j, i, Ur, Fr) or Tr(j, i)
-        uD[i] += u[j] * D;
-        uuD += u[i] * u[j] * D;
+#pragma clang fp contract(fast)
+    const double fm2 = dot3(Fr, Fr);
+    double cf = 0.0, dg = 0.33333333333333;
+    if (!(fm2 < 1e-150 || Ur < 1e-150)) {
+      const double ifm2 = A::rcp(fm2);
+      const double U2 = Ur * Ur;
+      if (fm2 > U2) {
+        cf = ifm2, dg = 0.0;
+      } else {
+        const double f2 = fm2 * A::rcp(U2);
+        const double chi_2 = A::div(1.5 + 2.0 * f2, 5.0 + 2.0 * A::sqrt_(4.0 - 3.0 * f2));
+        cf = (3.0 * chi_2 - 0.5) * ifm2, dg = 0.5 - chi_2;
       }
+    }
+    const double cuF = cf * uF;
+    const double uuD = cuF * uF + dg * u2;
+    const double rhogamma = rho * gamma;
+    const double rUr = rho * Ur;
+    G[0] = q * gamma + rhogamma * (abs_op - scat_op * (u2 + uuD)) * Ur - rho * (abs_op - scat_op * (u2 + gamma2)) * uF;
+    const double a = q - rUr * (tot_op * dg + scat_op * (gamma2 + uuD)) + 2.0 * rhogamma * scat_op * uF;
+    const double b = rho * tot_op * (gamma - Ur * cuF);
+#pragma unroll
+    for (int i = 0; i < 3; i++) G[i + 1] = a * u[i] + b * Fr[i];
+    return;
+  }
+  double uuD = 0.;
+  double uD[3];
+  double Dm[3][3];
+  if (Tr9) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) Dm[a][b] = (double)Tr9[3 * a + b];
+  } else {
+    edd_matrix<A>(Ur, Fr, Dm);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    uD[i] = 0.;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const double D = Dm[j][i];  // EddTensor(j, i, Ur, Fr) or Tr(j, i)
+      uD[i] += u[j] * D;
+      uuD += u[i] * u[j] * D;
     }
   }
   const double rhogamma = rho * gamma;
@@ -805,9 +804,16 @@ __device__ __forceinline__ void rad_source(const State<double> &W, double Ur, co
                                            const float *Tr9 = nullptr) {
   double gamma = dot3(W.v, W.v);
   double u2 = gamma;
-  const double gamma2 = A::rcp(1.0 - gamma);
-  u2 *= gamma2;
-  gamma = A::sqrt_(gamma2);
+  double gamma2;
+  if (!A::kFast) {
+    gamma2 = A::rcp(1.0 - gamma);
+    u2 *= gamma2;
+    gamma = A::sqrt_(gamma2);
+  } else {  // gamma from one reciprocal root
+    gamma = A::rsqrt(1.0 - gamma);
+    gamma2 = gamma * gamma;
+    u2 *= gamma2;
+  }
   const double q = emission ? -abs_op * W.d * b_planck(gas_temperature(W.d, W.p)) / radc::kRadiation : 0.0;
   rad_source_tail<A>(W, Ur, Fr, abs_op, scat_op, q, gamma, gamma2, u2, G, Tr9);
 }

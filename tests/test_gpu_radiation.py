@@ -298,3 +298,26 @@ def test_add_rad_flux_tensor_vs_oracle(table):
         for k in range(5):
             ref = np.abs(b[:, k]).max()
             assert np.abs(a[:, k] - b[:, k]).max() <= 1e-10 * ref, (name, k)
+
+
+def test_eval_rad_source_mixed_form():
+    """The mixed path's compact M1 source (G_i = a u_i + b F_i, squares instead of |F| and n)
+    against the oracle's GetRadSource: the same four-force to rounding, over random states
+    and the closure's three cases (F = 0, |F| > Ur, |F| <= Ur) and gas at rest."""
+    rng = np.random.default_rng(SEED + 12)
+    n = 6000
+    W = np.column_stack([10 ** rng.uniform(-2, 1, n), 0.9 * rng.uniform(-1, 1, (n, 3)) / np.sqrt(3),
+                         10 ** rng.uniform(-2, 1, n)])
+    W[:300, 1:4] = 0.0
+    Ur = 10 ** rng.uniform(-3, 0, n)
+    Fr = rng.normal(size=(n, 3))
+    Fr *= (Ur * rng.uniform(0, 1.5, n) / np.linalg.norm(Fr, axis=1))[:, None]
+    Fr[300:600] = 0.0
+    ka, ks = rng.uniform(0.1, 10, n), rng.uniform(0.1, 10, n)
+    G = capi.eval_rad_source(W, Ur, Fr, ka, ks, precision=capi.PREC_MIXED)
+    Go = np.array([O.rad_source(W[i], Ur[i], Fr[i], ka[i], ks[i]) for i in range(n)])
+    g = 1.0 / np.sqrt(1.0 - (W[:, 1:4] ** 2).sum(1))
+    scale = W[:, 0] * (ka + ks) * Ur * g**3 * 4
+    err = np.abs(G - Go).max(axis=1) / scale
+    print("mixed-form source: max error / scale", err.max())
+    assert err.max() <= 1e-13
