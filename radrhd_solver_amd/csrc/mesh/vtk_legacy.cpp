@@ -24,6 +24,8 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -71,6 +73,18 @@ class Parser {
   }
 
   // ASCII values of the current data block.
+  // Bytes left after the read position.
+  size_t remaining() const { return p_ < b_.size() ? b_.size() - p_ : 0; }
+
+  // Whether n values can follow: w bytes each (BINARY), or at least one
+  // character and a separator each (ASCII).  Checked before a block is sized
+  // from its declared count, so a corrupt count is an error, not an allocation.
+  bool can_hold(int64_t n, int w) const {
+    if (n < 0) return false;
+    const size_t r = remaining();
+    return w > 0 ? (uint64_t)n <= r / (size_t)w : (uint64_t)n <= r / 2 + 1;
+  }
+
   template <typename T>
   bool ascii(T &v) {
     skip_space();
@@ -83,7 +97,7 @@ class Parser {
 
   // Raw bytes of a BINARY data block; a block is followed by a line end.
   const uint8_t *bytes(size_t n) {
-    if (p_ + n > b_.size()) return nullptr;
+    if (n > remaining()) return nullptr;
     const uint8_t *r = reinterpret_cast<const uint8_t *>(b_.data() + p_);
     p_ += n;
     return r;
@@ -131,6 +145,8 @@ uint64_t load_be(const uint8_t *p, int n) {
 
 // n values of type `type` as doubles (points) -- ASCII or BINARY.
 bool read_reals(Parser &P, bool binary, const std::string &type, int64_t n, std::vector<double> &out) {
+  if (!P.can_hold(n, binary ? binary_size(type) : 0))
+    return fail("POINTS declares " + std::to_string(n) + " values, more than the rest of the file holds");
   out.resize((size_t)n);
   const bool f32 = type == "float";
   if (binary) {
@@ -162,6 +178,8 @@ bool read_reals(Parser &P, bool binary, const std::string &type, int64_t n, std:
 // n integers of type `type` -- ASCII or BINARY.
 bool read_ints(Parser &P, bool binary, const std::string &type, int64_t n, std::vector<int64_t> &out,
                const char *what) {
+  if (!P.can_hold(n, binary ? binary_size(type) : 0))
+    return fail(std::string(what) + " declares " + std::to_string(n) + " values, more than the rest of the file holds");
   out.resize((size_t)n);
   if (binary) {
     const int w = binary_size(type);
@@ -226,7 +244,8 @@ bool parse(const char *path, rhllc_vtk_mesh &m) {
     const auto w = words(ln);
     const std::string &k = w[0];
     if (k == "POINTS") {
-      if (w.size() != 3 || !to_i64(w[1], n_points) || n_points < 0) return fail("bad POINTS line '" + ln + "'");
+      if (w.size() != 3 || !to_i64(w[1], n_points) || n_points < 0 || n_points > INT64_MAX / 3)
+        return fail("bad POINTS line '" + ln + "'");
       if (!read_reals(P, binary, w[2], 3 * n_points, pts)) return false;
     } else if (k == "CELLS") {
       int64_t a, b;
@@ -234,6 +253,7 @@ bool parse(const char *path, rhllc_vtk_mesh &m) {
         return fail("bad CELLS line '" + ln + "'");
       if (version < 5.0) {
         // n cells, then `size` integers: a count before each cell's ids.
+        if (b < a) return fail("CELLS size is smaller than its cell count");
         if (!read_ints(P, binary, "int", b, tmp, "CELLS")) return false;
         n_cells = a;
         offsets.assign((size_t)n_cells + 1, 0);
@@ -285,6 +305,7 @@ bool parse(const char *path, rhllc_vtk_mesh &m) {
         int64_t nc, nt;
         if (v.size() != 4 || !to_i64(v[1], nc) || !to_i64(v[2], nt) || nc < 0 || nt < 0)
           return fail("bad FIELD array line '" + ln + "'");
+        if (nt > 0 && nc > (int64_t)P.remaining() / nt) return fail("FIELD block is truncated");
         if (binary) {
           const int bw = binary_size(v[3]);
           if (bw == 0) return fail("unsupported BINARY FIELD type " + v[3]);
@@ -343,8 +364,15 @@ bool parse(const char *path, rhllc_vtk_mesh &m) {
 extern "C" int rhllc_vtk_open(const char *path, rhllc_vtk_mesh **out, int64_t *n_points, int64_t *n_cells) {
   if (!path || !out) return fail("null argument"), 1;
   *out = nullptr;
-  std::unique_ptr<rhllc_vtk_mesh> m(new rhllc_vtk_mesh);
-  if (!parse(path, *m)) return 1;
+  std::unique_ptr<rhllc_vtk_mesh> m;
+  try {
+    m.reset(new rhllc_vtk_mesh);
+    if (!parse(path, *m)) return 1;
+  } catch (const std::bad_alloc &) {
+    return fail(std::string("out of host memory reading ") + path), 1;
+  } catch (const std::exception &e) {
+    return fail(std::string("reading ") + path + ": " + e.what()), 1;
+  }
   if (n_points) *n_points = (int64_t)(m->points.size() / 3);
   if (n_cells) *n_cells = (int64_t)(m->tets.size() / 4);
   *out = m.release();

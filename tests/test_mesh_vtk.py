@@ -361,8 +361,29 @@ def test_reader_errors(tmp_path):
     _expect_error(tmp_path, HEAD + PTS + "CELLS 1 5\n4 0 1 2 3\n", "no CELL_TYPES")
     _expect_error(tmp_path, HEAD + PTS + "CELLS 1 9\n8 0 1 2 3 4 0 1 2\nCELL_TYPES 1\n12\n", "VTK type 12")
     _expect_error(tmp_path, HEAD + PTS + "CELLS 1 5\n4 0 1 2 9\nCELL_TYPES 1\n10\n", "out of range")
-    _expect_error(tmp_path, HEAD + "POINTS 5 double\n0 0 0\n1 0\n", "POINTS value")
+    _expect_error(tmp_path, HEAD + "POINTS 5 double\n0 0 0\n1 0\n", "POINTS declares 15 values, more than")
+    _expect_error(tmp_path, HEAD + "POINTS 2 double\n0 0 0\n1 0 x\n", "POINTS value")
     _expect_error(tmp_path, HEAD + PTS + "CELLS 2 10\n4 0 1 2 3\n", "CELLS")
+
+
+def test_reader_rejects_counts_beyond_the_file(tmp_path):
+    """Declared counts are checked against the bytes left before anything is sized from them:
+    a corrupt count is an error message, not an allocation of its size."""
+    big = 1 << 60
+    _expect_error(tmp_path, HEAD + f"POINTS {big} double\n0 0 0\n", "more than the rest of the file")
+    _expect_error(tmp_path, HEAD + f"POINTS {1 << 62} double\n0 0 0\n", "bad POINTS line")
+    _expect_error(tmp_path, HEAD + PTS + f"CELLS 1 {big}\n4 0 1 2 3\n", "more than the rest of the file")
+    _expect_error(tmp_path, HEAD + PTS + f"CELLS {big} 5\n4 0 1 2 3\n", "smaller than its cell count")
+    _expect_error(tmp_path, HEAD + PTS + "CELLS 1 5\n4 0 1 2 3\n" + f"CELL_TYPES {big}\n10\n",
+                  "more than the rest of the file")
+    v5 = "# vtk DataFile Version 5.1\nt\nASCII\nDATASET UNSTRUCTURED_GRID\n"
+    _expect_error(tmp_path, v5 + PTS + f"CELLS {big} 4\nOFFSETS vtktypeint64\n0 4\n", "OFFSETS declares")
+    _expect_error(tmp_path, HEAD + PTS + f"FIELD f 1\na 3 {big} double\n1 2 3\n", "FIELD block is truncated")
+    path = tmp_path / "bin.vtk"
+    path.write_bytes(b"# vtk DataFile Version 3.0\nt\nBINARY\nDATASET UNSTRUCTURED_GRID\n"
+                     + f"POINTS {1 << 40} double\n".encode() + bytes(48))
+    with pytest.raises(ValueError, match="more than the rest of the file"):
+        M.read_vtk(str(path))
 
 
 def test_geometry_errors():
