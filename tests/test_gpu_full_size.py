@@ -5,7 +5,10 @@ matched step for step at smaller sizes in test_gpu_step.py):
 * config 1: the blast wave on 100^3 hexes x 6 = 6M tets, 100 steps;
 * config 2: the W ~ 10 shear layer on 128^3 hexes x 6 = 12.6M tets with
   periodic faces, gamma = 4/3, 200 steps;
-* config 3: the c2p surrogate inside config 2.
+* config 3: the c2p surrogate inside config 2;
+* config 4: the full form -- the blast wave on 159^3 hexes x 6 = 24.1M tets
+  with the seeded FP32 radiation source, mixed precision, surrogate start,
+  graph-captured, 100 steps.
 
 Properties: mass and energy conservation (on the periodic box for every field;
 on the blast's box while its outer cells are still untouched).  The FP64 step
@@ -14,6 +17,9 @@ only by rounding (bar 1e-13 relative).  The mixed step also subtracts each
 cell's own flux through the closure defect of its FP32 face vectors, a per-cell
 term that keeps a uniform state at rest but does not telescope, so its totals
 move at the FP32 geometry's rounding times the flow's variation (bar 1e-9).
+With the radiation source (config 4) the momenta and energy take the source,
+so only the mass balances: one step changes the total of V*D by tau times the
+D flux through the boundary faces (their outer state is the cell's own).
 Also: the mixed step against the FP64 step (per-field relative L1 <= 1e-5), no
 c2p failure and the c2p residual |f(p)|/(E + p) <= 1e-10 in every cell, and the
 FP64 step reproducing itself bitwise between eager steps and the captured graph.
@@ -172,3 +178,56 @@ def test_config3_surrogate_in_shear_12M():
     mean = {k: float(np.concatenate(v).mean()) for k, v in tot.items()}
     print("mean Newton iterations over the probed states:", mean)
     assert mean["mlp"] < mean["ref"]  # the surrogate is the better cold start
+
+
+def _mass_balance(s, m, V, tau):
+    """One eager step: |change of total V*D + tau * boundary D flux| over the total."""
+    _, U0 = s.download()
+    s.step(tau)
+    s.sync()
+    _, U1 = s.download()
+    F = s.face_flux()
+    bnd = M.face_id_r(m.faces) < 0
+    M0, M1 = (U0[:, 0] * V).sum(), (U1[:, 0] * V).sum()
+    return abs(M1 - M0 + tau * F[bnd, 0].sum()) / M0, (M1 - M0) / M0
+
+
+def test_config4_full_form_24M():
+    """Config 4: 24.1M tets, blast wave + seeded radiation source, mixed precision with the
+    surrogate start (the stored gamma = 5/3 model), graph-captured, 100 steps with the
+    statistics on: no c2p failure, no pressure fix, residual <= 1e-10 in every cell, per-field
+    relative L1 <= 1e-5 against the FP64 step under the same source, and the mass balance of
+    one more step against the boundary flux (FP64 to rounding, mixed to its FP32 geometry)."""
+    m = M.kuhn_box(159, 159, 159, seed=SEED)
+    assert m.n_cells == 24_118_074
+    W0 = M.init_state(m, M.INIT_BLAST)
+    rad = M.init_radiation(m.n_cells, seed=SEED)
+    V = m.cells["V"]
+    model, info = surrogate.load_or_train(5.0 / 3.0)
+    print("surrogate", info.get("source"), flush=True)
+    h = m.h_min()
+    tau = 0.4 * h
+    out = {}
+    for label, prec, guess in (("fp64", capi.PREC_FP64, capi.GUESS_REFERENCE),
+                               ("mixed+mlp", capi.PREC_MIXED, capi.GUESS_MLP)):
+        s = capi.Solver(m, gamma=5.0 / 3.0, cfl=0.4, h_min=h, precision=prec, guess=guess, collect_stats=True)
+        try:
+            s.upload_W(W0)
+            s.set_radiation(*rad)
+            if guess == capi.GUESS_MLP:
+                s.set_mlp(model)
+            s.run(100, graph=True)
+            s.sync()
+            st = s.stats()
+            _clean(st, label)
+            assert st["steps"] == 100
+            print(label, "surrogate starts", st["mlp_starts"], "fallback cells", st["fallback_cells"], flush=True)
+            out[label] = s.download()[1]
+            bal, dm = _mass_balance(s, m, V, tau)
+            print(label, "mass change of one step", dm, "imbalance vs boundary flux", bal, flush=True)
+            assert bal <= (1e-13 if prec == capi.PREC_FP64 else 1e-9)
+        finally:
+            s.close()
+    l1 = _rel_l1(out["mixed+mlp"], out["fp64"], V)
+    print("mixed+mlp vs fp64 rel L1 after 100 steps", l1, flush=True)
+    assert max(l1) <= 1e-5
