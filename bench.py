@@ -61,7 +61,7 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 legs")
     ap.add_argument("--two-kernel-mixed", action="store_true", help="also time the mixed two-kernel schedule")
     ap.add_argument("--cpu-steps", type=int, default=0, help="CPU baseline timed steps (0: auto, >= 5, ~10-30 s)")
-    ap.add_argument("--kernel-steps", type=int, default=10, help="eager steps timed per kernel with HIP events")
+    ap.add_argument("--kernel-steps", type=int, default=20, help="eager steps timed per kernel with HIP events")
     ap.add_argument("--stats-steps", type=int, default=5, help="steps with statistics after the timed region")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the calibration copy kernel (1 GiB read + 1 GiB write per launch)")

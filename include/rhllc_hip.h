@@ -197,7 +197,9 @@ int rhllc_set_radiation_tensor(rhllc_solver *h, const float *Tr9);
 int rhllc_add_rad_flux_tensor(rhllc_solver *h, double tau, const float *Ur, const float *Fr3, const float *Tr9);
 
 /* Average device time (ms) of each kernel of the step, measured with HIP
- * events around each launch on the handle's stream over nsteps eager steps:
+ * events around each launch on the handle's stream over nsteps eager steps
+ * (1..4096, after twenty untimed ones while the clock settles), enqueued
+ * back to back with one wait at the end; the steps advance the state:
  * out[0] face kernel (tiled step: the tile kernel, faces and cells),
  * out[1] cell kernel and surrogate-list kernel (tiled step: the list kernel
  * only), out[2] dt kernel, out[3] whole step. */

@@ -763,46 +763,63 @@ extern "C" int rhllc_set_mlp(rhllc_solver *h, const float *model) {
 extern "C" int rhllc_time_kernels(rhllc_solver *h, int32_t nsteps, double out[4]) {
   if (!h || !out) return fail(RHLLC_ERR_ARG, "null argument");
   if (!h->ready()) return fail(RHLLC_ERR_STATE, "no state uploaded (or no MLP model set)");
-  if (nsteps < 1) return fail(RHLLC_ERR_ARG, "nsteps < 1");
+  if (nsteps < 1 || nsteps > 4096) return fail(RHLLC_ERR_ARG, "nsteps must be in 1..4096");
   HIP_TRY(hipSetDevice(h->prm.device));
-  hipEvent_t ev[4];
-  for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
-  double acc[4] = {0, 0, 0, 0};
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  // Twenty untimed lead-in steps, then nsteps steps with four events each, all
+  // enqueued back to back: the device runs them without the idle gaps that a
+  // host wait per step would leave, as it runs the graph-replayed steps.  After
+  // an idle period the device clock takes some fifteen steps to settle (the
+  // first kernel runs fast, the next few up to 30% slower than steady state).
+  constexpr int kLead = 20;
+  std::vector<hipEvent_t> ev(4 * (size_t)nsteps, nullptr);
   int rc = RHLLC_OK;
+  for (auto &e : ev)
+    if (hipError_t err = hipEventCreate(&e); err != hipSuccess) {
+      rc = hip_fail(err, "kernel timing");
+      break;
+    }
   const bool stats = h->prm.collect_stats != 0;
-  for (int32_t k = 0; k < nsteps && rc == RHLLC_OK; ++k) {
+  for (int32_t k = 0; k < kLead + nsteps && rc == RHLLC_OK; ++k) {
+    hipEvent_t *e4 = k >= kLead ? &ev[4 * (size_t)(k - kLead)] : nullptr;
     const Dev x = h->tiled ? h->dev(h->cur, h->cur ^ 1) : h->now();
-    hipError_t e = hipEventRecord(ev[0], h->stream);
+    hipError_t e = e4 ? hipEventRecord(e4[0], h->stream) : hipSuccess;
     if (e == hipSuccess)
       e = h->tiled ? launch_tile(x, h->mixed, h->variant, h->rad, stats, h->n_tiles, h->tile_slots, h->stream)
                    : launch_face(x, h->mixed, h->stream);
-    if (e == hipSuccess) e = hipEventRecord(ev[1], h->stream);
+    if (e == hipSuccess && e4) e = hipEventRecord(e4[1], h->stream);
     if (e == hipSuccess) {
       if (!h->tiled)
         e = h->enqueue_cells(x);
       else if (h->variant == kMixedMlp)
         e = launch_cell_mlp(x, h->rad, stats, h->stream);
     }
-    if (e == hipSuccess) e = hipEventRecord(ev[2], h->stream);
+    if (e == hipSuccess && e4) e = hipEventRecord(e4[2], h->stream);
     if (e == hipSuccess) e = launch_dt(x, h->stream);
-    if (e == hipSuccess) e = hipEventRecord(ev[3], h->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(ev[3]);
-    if (e == hipSuccess) {
-      h->last_in = h->cur;
-      if (h->tiled) h->cur ^= 1;
-    }
-    float ms[4] = {0, 0, 0, 0};
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms[0], ev[0], ev[1]);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms[1], ev[1], ev[2]);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms[2], ev[2], ev[3]);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms[3], ev[0], ev[3]);
+    if (e == hipSuccess && e4) e = hipEventRecord(e4[3], h->stream);
     if (e != hipSuccess) {
       rc = hip_fail(e, "kernel timing");
       break;
     }
+    h->last_in = h->cur;
+    if (h->tiled) h->cur ^= 1;
+  }
+  double acc[4] = {0, 0, 0, 0};
+  if (rc == RHLLC_OK) {
+    if (hipError_t e = hipStreamSynchronize(h->stream); e != hipSuccess) rc = hip_fail(e, "kernel timing");
+  }
+  for (int32_t k = 0; k < nsteps && rc == RHLLC_OK; ++k) {
+    const hipEvent_t *e4 = &ev[4 * (size_t)k];
+    float ms[4] = {0, 0, 0, 0};
+    hipError_t e = hipEventElapsedTime(&ms[0], e4[0], e4[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[1], e4[1], e4[2]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[2], e4[2], e4[3]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[3], e4[0], e4[3]);
+    if (e != hipSuccess) rc = hip_fail(e, "kernel timing");
     for (int i = 0; i < 4; ++i) acc[i] += ms[i];
   }
-  for (auto &e : ev) (void)hipEventDestroy(e);
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
   if (rc != RHLLC_OK) return rc;
   for (int i = 0; i < 4; ++i) out[i] = acc[i] / nsteps;
   return RHLLC_OK;
