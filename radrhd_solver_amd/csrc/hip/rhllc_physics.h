@@ -117,6 +117,13 @@ struct FaceFast {
     return x == 0.0 ? x : s;
   }
   __device__ static __forceinline__ double rsqrt(double x) { return Fast::rsqrt(x); }
+  // The hardware approximations alone (within 5.2e-8 and 4.6e-8 relative),
+  // for the signal speeds: the HLLC flux depends on them smoothly, and two
+  // equal states give the physical flux whatever they are.
+  __device__ static __forceinline__ double sqrt_rough(double x) {  // x = 0 returns 0
+    return x == 0.0 ? x : x * __builtin_amdgcn_rsq(x);
+  }
+  __device__ static __forceinline__ double rcp_rough(double x) { return __builtin_amdgcn_rcp(x); }
 };
 
 template <typename R>
@@ -569,8 +576,8 @@ __device__ __forceinline__ void mixed_side(const MixedIn &w, const double *n, Mi
   o.mn = o.s * o.vn;
   const double cs2 = w.cs2;
   const double vn2 = o.vn * o.vn;
-  const double sroot = A::sqrt_(cs2 * (1.0 - vn2 - (V2 - vn2) * cs2) * (1.0 - V2));
-  const double r = A::rcp(1.0 - V2 * cs2);
+  const double sroot = A::sqrt_rough(cs2 * (1.0 - vn2 - (V2 - vn2) * cs2) * (1.0 - V2));
+  const double r = A::rcp_rough(1.0 - V2 * cs2);
   const double a = o.vn * (1.0 - cs2);
   o.cmax = (a + sroot) * r;
   o.cmin = (a - sroot) * r;
