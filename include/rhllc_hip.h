@@ -303,8 +303,9 @@ int rhllc_mlp_grad(int32_t device, const float *model, int32_t n, const float *X
 int rhllc_mlp_grad_noalloc(int32_t device, const float *model, int32_t n, const float *X, const float *y,
                            const float *ok, float *grad, float *loss, void *stream);
 
-/* Runs one v_mfma_f32_16x16x16_bf16 on known operands; out[64*4] = the
- * accumulator registers per lane, to check the lane layout the kernels use. */
+/* Runs one v_mfma_f32_16x16x32_bf16 on known operands, with the operand
+ * layout the MLP kernels use; out[64*4] = the accumulator registers per lane,
+ * to check that layout on the device. */
 int rhllc_mlp_selftest(int32_t device, float *out);
 
 /* Installs a model for RHLLC_GUESS_MLP (mixed path); its gamma must be the

@@ -16,15 +16,20 @@
 // index gamma the model was trained for.
 //
 // Wave-level layout.  A wave evaluates 64 samples with the 64x64 layer on
-// v_mfma_f32_16x16x16_bf16 (FP32 accumulate).  For the 16x16x16 instruction
-// the operand/result lane maps are
-//   A[i][k]: lane i + 16*(k/4), element k%4
-//   B[k][j]: lane j + 16*(k/4), element k%4
+// gfx950's v_mfma_f32_16x16x32_bf16 (FP32 accumulate), two instructions per
+// 16x16 output tile.  Its operand/result lane maps are
+//   A[i][k]: lane i + 16*(k/8), element k%8
+//   B[k][j]: lane j + 16*(k/8), element k%8
 //   C[i][j]: lane j + 16*(i/4), element i%4
-// (checked on the device by rhllc_mlp_selftest).  The forward product is
-// Z2^T = W2 * H1^T, so lane l holds, for samples s_t = 16 t + l%16 (t = 0..3),
-// the hidden units 16 q + 4 (l/16) + e of both layers -- layer 1 is computed
-// directly in that layout and no transpose is needed until dW2.
+// The hidden units are held in 16-unit blocks q, four per lane (units
+// 16 q + 4 (l/16) + e, e = 0..3, in lane l); one instruction contracts the
+// blocks 2 q' and 2 q' + 1 by giving it, in both operands, the lane's four
+// units of the first block as elements 0..3 and of the second as 4..7 (the
+// sum over k does not depend on which unit sits at which k, as long as A and
+// B agree; rhllc_mlp_selftest checks this layout on the device).  The forward
+// product is Z2^T = W2 * H1^T, so lane l holds, for samples s_t = 16 t + l%16
+// (t = 0..3), the hidden units 16 q + 4 (l/16) + e of both layers -- layer 1
+// is computed directly in that layout and no transpose is needed until dW2.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -45,7 +50,18 @@ constexpr int kOffNorm = kParams;          // mu_x[3], sd_x[3], mu_y, sd_y, gamm
 constexpr int kModel = kParams + 9;        // 4490
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// acc += A B over 32 k: lo holds the lane's k elements 0..3, hi 4..7 (of A
+// and of B alike).
+__device__ __forceinline__ f32x4 mfma32(bf16x4 a_lo, bf16x4 a_hi, bf16x4 b_lo, bf16x4 b_hi, f32x4 acc) {
+  const s16x8 a = {a_lo[0], a_lo[1], a_lo[2], a_lo[3], a_hi[0], a_hi[1], a_hi[2], a_hi[3]};
+  const s16x8 b = {b_lo[0], b_lo[1], b_lo[2], b_lo[3], b_hi[0], b_hi[1], b_hi[2], b_hi[3]};
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc,
+                                                 0, 0, 0);
+}
 
 __device__ __forceinline__ short f2bf(float x) {  // round to nearest even
   uint32_t u = __float_as_uint(x);
@@ -128,7 +144,7 @@ __device__ __forceinline__ void forward_wave(const float *__restrict__ P, const 
     for (int t = 0; t < 4; ++t) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[q], h1[q][t], acc, 0, 0, 0);
+      for (int q = 0; q < 4; q += 2) acc = mfma32(a[q], a[q + 1], h1[q][t], h1[q + 1][t], acc);
       // lane holds Z2[s = 16 t + c][h2 = 16 u + 4 g + r]
 #pragma unroll
       for (int r = 0; r < 4; ++r) {

@@ -3,7 +3,8 @@
 // Training step (captured in a hipGraph and replayed):
 //   k_mlp_gen    seeded primitives -> FP64 p2c -> FP64 c2p (reference start)
 //                -> normalised features/targets (failed states are masked);
-//   k_mlp_grad   one wave per 64 samples: forward (BF16 MFMA, FP32 accumulate),
+//   k_mlp_grad   one wave per 64 samples: forward (BF16 MFMA with K = 32,
+//                FP32 accumulate),
 //                backward, per-wave gradient partials written to global memory;
 //   k_mlp_adam   fixed-order sum of the partials (deterministic), Adam on the
 //                FP32 master weights, BF16 copies of W2 and W2^T;
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float *__restrict__ P, c
     for (int t = 0; t < 4; ++t) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[q], h1[q][t], acc, 0, 0, 0);
+      for (int q = 0; q < 4; q += 2) acc = mfma32(a[q], a[q + 1], h1[q][t], h1[q + 1][t], acc);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int h2 = 16 * u + 4 * g + r;
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float *__restrict__ P, c
     for (int t = 0; t < 4; ++t) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], dz2b[u][t], acc, 0, 0, 0);
+      for (int u = 0; u < 4; u += 2) acc = mfma32(a[u], a[u + 1], dz2b[u][t], dz2b[u + 1][t], acc);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const float dz = acc[e] * dsilu(z1[q][t][e]);
@@ -217,10 +218,11 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float *__restrict__ P, c
     for (int q = 0; q < 4; ++q) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const bf16x4 a = *reinterpret_cast<const bf16x4 *>(dzT + (16 * u + c) * kH + 16 * t + 4 * g);
-        const bf16x4 b = *reinterpret_cast<const bf16x4 *>(h1T + (16 * q + c) * kH + 16 * t + 4 * g);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+      for (int t = 0; t < 4; t += 2) {
+        const short *ar = dzT + (16 * u + c) * kH + 4 * g, *br = h1T + (16 * q + c) * kH + 4 * g;
+        acc = mfma32(*reinterpret_cast<const bf16x4 *>(ar + 16 * t), *reinterpret_cast<const bf16x4 *>(ar + 16 * t + 16),
+                     *reinterpret_cast<const bf16x4 *>(br + 16 * t), *reinterpret_cast<const bf16x4 *>(br + 16 * t + 16),
+                     acc);
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) G[kOffW2 + (16 * u + 4 * g + r) * kH + 16 * q + c] = acc[r];
@@ -333,19 +335,23 @@ __global__ __launch_bounds__(256) void k_mlp_eval(const float *__restrict__ P, c
   if (i < n) p_out[i] = predicted_pressure(norm, o, p0);
 }
 
-// MFMA lane-layout self test: A[i][k] = i + 0.25 k... written so every
-// product term is exact in bf16/FP32; out[64*4] = C per lane.
+// MFMA lane-layout self test of mfma32 as the kernels use it: units k of two
+// 16-unit blocks, lane group g holding k = 4 g + e (lo) and 16 + 4 g + e (hi).
+// A[i][3] = i + 1 and A[i][23] = 0.5, B[3][j] = 16 (j + 1) and B[23][j] = 2, all
+// other entries 0, so C[i][j] = 16 (i + 1)(j + 1) + 1 exactly; out[64*4] = C
+// per lane (C[i][j] expected in lane j + 16 (i/4), element i%4).
 __global__ void k_mlp_selftest(float *out) {
   const int l = threadIdx.x & 63;
   const int g = l >> 4, c = l & 15;
-  bf16x4 a, b;
-  for (int e = 0; e < 4; ++e) {
-    const int k = 4 * g + e;
-    a[e] = f2bf((float)(c + 1) * (k == 3 ? 1.0f : 0.0f) + (k == 7 ? 0.5f : 0.0f));  // A[i][3] = i+1, A[i][7] = 0.5
-    b[e] = f2bf((k == 3 ? 1.0f : 0.0f) * (float)(c + 1) * 16.0f + (k == 7 ? 2.0f : 0.0f));  // B[3][j] = 16(j+1), B[7][j] = 2
-  }
+  bf16x4 a[2], b[2];
+  for (int h = 0; h < 2; ++h)
+    for (int e = 0; e < 4; ++e) {
+      const int k = 16 * h + 4 * g + e;
+      a[h][e] = f2bf((k == 3 ? (float)(c + 1) : 0.0f) + (k == 23 ? 0.5f : 0.0f));
+      b[h][e] = f2bf((k == 3 ? 16.0f * (float)(c + 1) : 0.0f) + (k == 23 ? 2.0f : 0.0f));
+    }
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  acc = mfma32(a[0], a[1], b[0], b[1], acc);
   for (int r = 0; r < 4; ++r) out[4 * l + r] = acc[r];
 }
 

@@ -13,7 +13,10 @@ pytestmark = pytest.mark.gpu
 SEED = 20261202
 
 
-def test_mfma_16x16x16_bf16_lane_layout():
+def test_mfma_16x16x32_bf16_lane_layout():
+    """The K = 32 BF16 MFMA as the MLP kernels feed it: A[i][3] = i + 1, A[i][23] = 1/2,
+    B[3][j] = 16 (j + 1), B[23][j] = 2 (units 3 and 23 sit in different halves of the
+    lanes' eight elements), so C[i][j] = 16 (i + 1)(j + 1) + 1 in lane j + 16 (i / 4)."""
     out = capi.mlp_selftest()
     lane = np.arange(64)
     g, c = lane // 16, lane % 16
