@@ -290,6 +290,19 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
     HIP_TRY(upload(&tm, L.tile_meta.data(), L.n_tiles));
     HIP_TRY(upload(&te, L.tile_extra.data(), L.tile_extra.size()));
     HIP_TRY(upload(&cs, L.cell_slots.data(), N));
+    const size_t ne = L.tile_extra.size();
+    int2 *elr;
+    HIP_TRY(upload(&elr, L.extra_lr.data(), ne));
+    d.extra_lr = elr;
+    if (mixed) {
+      float3 *esn;
+      HIP_TRY(upload(&esn, L.extra_Sn32.data(), ne));
+      d.extra_Sn32 = esn;
+    } else {
+      double4 *ens;
+      HIP_TRY(upload(&ens, L.extra_nS.data(), ne));
+      d.extra_nS = ens;
+    }
     d.tile_meta = tm;
     d.tile_extra = te;
     d.cell_slots = cs;

@@ -76,6 +76,9 @@ struct Dev {
   // when the cell is the face's id_r (the flux is subtracted).
   const int4 *tile_meta;
   const int32_t *tile_extra;
+  const int2 *extra_lr;        // face_lr of each extra face, in tile_extra's order
+  const double4 *extra_nS;     // face_nS of each extra face (FP64 path)
+  const float3 *extra_Sn32;    // face_Sn32 of each extra face (mixed path)
   const uint2 *cell_slots;
   // radiation source (optional): (Fx, Fy, Fz, Ur) and (kappa_a, kappa_s)
   const float4 *rad_FU;

@@ -102,14 +102,43 @@ __device__ __forceinline__ void fold_speed(const Dev &d, double speed) {
   }
 }
 
+// What a face reads before its cells: the two cell indices and the geometry
+// (FP64 path: n and S; mixed path: the FP32 vector S n).
+template <bool MIXED>
+struct FaceIn {
+  int2 lr;
+  typename std::conditional<MIXED, float3, double4>::type g;
+};
+template <bool MIXED>
+__device__ __forceinline__ FaceIn<MIXED> face_in(const Dev &d, int i) {
+  FaceIn<MIXED> o;
+  o.lr = ld(d.face_lr, i);
+  if constexpr (MIXED)
+    o.g = ld(d.face_Sn32, i);
+  else
+    o.g = ld(d.face_nS, i);
+  return o;
+}
+// The same for extra face k of the tiled step, from the copies in tile order.
+template <bool MIXED>
+__device__ __forceinline__ FaceIn<MIXED> extra_in(const Dev &d, int k) {
+  FaceIn<MIXED> o;
+  o.lr = ld(d.extra_lr, k);
+  if constexpr (MIXED)
+    o.g = ld(d.extra_Sn32, k);
+  else
+    o.g = ld(d.extra_nS, k);
+  return o;
+}
+
 // One face: gather of the two cells' state at the start of the step, HLLC
 // flux into F (S * flux in the global frame; on the mixed path F.p is the
 // flux of tau = E - D); returns max(lL, lR).
 template <bool MIXED, int EOS>
-__device__ __forceinline__ double face_flux(const Dev &d, int i, State<double> &F) {
-  const int2 lr = ld(d.face_lr, i);
-  if (!MIXED) {
-    const double4 nS = ld(d.face_nS, i);
+__device__ __forceinline__ double face_flux(const Dev &d, const FaceIn<MIXED> &fi, State<double> &F) {
+  const int2 lr = fi.lr;
+  if constexpr (!MIXED) {
+    const double4 nS = fi.g;
     State<double> UL, WL, UR, WR;
     load_U(d, lr.x, UL);
     load_W(d, lr.x, WL);
@@ -124,7 +153,7 @@ __device__ __forceinline__ double face_flux(const Dev &d, int i, State<double> &
   } else {
     // FP32 storage (cell records, geometry), FP64 arithmetic with the fast
     // divide/sqrt policy.
-    const float3 Sn = ld(d.face_Sn32, i);
+    const float3 Sn = fi.g;
     const MixedIn WL = mixed_in(ld(d.Wva, lr.x), ld(d.Wsd, lr.x));
     MixedIn WR;
     if (lr.y >= 0) {
@@ -142,6 +171,11 @@ __device__ __forceinline__ double face_flux(const Dev &d, int i, State<double> &
 
 // Face flux stored for the two-kernel step: FP64 (D, m) + E on the FP64
 // path; the packed 24-byte record (flux_ext.h) on the mixed path.
+template <bool MIXED, int EOS>
+__device__ __forceinline__ double face_flux(const Dev &d, int i, State<double> &F) {
+  return face_flux<MIXED, EOS>(d, face_in<MIXED>(d, i), F);
+}
+
 template <bool MIXED, int EOS>
 __device__ __forceinline__ double face_flux_store(const Dev &d, int i) {
   State<double> F;
@@ -508,9 +542,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
   double *C = reinterpret_cast<double *>(lds_flux + 2 * nf);
   double speed = 0.0;
   for (int k = threadIdx.x; k < nf; k += kBlock) {
-    const int f = k < tm.y ? tm.x + k : ld(d.tile_extra, tm.z + (k - tm.y));
+    const FaceIn<MIXED> fi = k < tm.y ? face_in<MIXED>(d, tm.x + k) : extra_in<MIXED>(d, tm.z + (k - tm.y));
     State<double> F;
-    speed = smax(speed, face_flux<MIXED, EOS>(d, f, F));
+    speed = smax(speed, face_flux<MIXED, EOS>(d, fi, F));
     A[k] = make_double2(F.d, F.v[0]);
     B[k] = make_double2(F.v[1], F.v[2]);
     C[k] = F.p;

@@ -32,6 +32,11 @@ struct Layout {
   int32_t n_tiles = 0, max_slots = 0;
   std::vector<int32_t> tile_meta;    // per tile (first own face, own count, first extra, extra count)
   std::vector<int32_t> tile_extra;   // extra faces of all tiles, tile by tile, by face index
+  // The cells and geometry of each extra face, in tile_extra's order (copies
+  // of lr and nS / Sn32), so a tile reads them contiguously.
+  rhllc_mesh::RawVec<int32_t> extra_lr;
+  rhllc_mesh::RawVec<double> extra_nS;
+  rhllc_mesh::RawVec<float> extra_Sn32;
   rhllc_mesh::RawVec<uint32_t> cell_slots;  // per cell two words: four 16-bit slots, bit 15 = id_r
 };
 
