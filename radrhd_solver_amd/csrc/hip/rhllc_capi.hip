@@ -67,7 +67,7 @@ struct rhllc_solver {
   double4 *W4b[2] = {nullptr, nullptr};
   double *Wpb[2] = {nullptr, nullptr};
   float4 *Wvab[2] = {nullptr, nullptr};
-  float4 *Wsdb[2] = {nullptr, nullptr};
+  double2 *Wsgb[2] = {nullptr, nullptr};
   int32_t n_tiles = 0, tile_slots = 0;
   std::vector<void *> allocs;
   std::vector<int32_t> cell_new_to_old, face_new_to_old;
@@ -126,8 +126,8 @@ struct rhllc_solver {
   // The kernel arguments with input buffers [in] and output buffers [out].
   Dev dev(int in, int out) const {
     Dev e = d;
-    e.U4 = U4b[in], e.UE = UEb[in], e.W4 = W4b[in], e.Wp = Wpb[in], e.Wva = Wvab[in], e.Wsd = Wsdb[in];
-    e.U4o = U4b[out], e.UEo = UEb[out], e.W4o = W4b[out], e.Wpo = Wpb[out], e.Wvao = Wvab[out], e.Wsdo = Wsdb[out];
+    e.U4 = U4b[in], e.UE = UEb[in], e.W4 = W4b[in], e.Wp = Wpb[in], e.Wva = Wvab[in], e.Wsg = Wsgb[in];
+    e.U4o = U4b[out], e.UEo = UEb[out], e.W4o = W4b[out], e.Wpo = Wpb[out], e.Wvao = Wvab[out], e.Wsgo = Wsgb[out];
     return e;
   }
   // The current state, updated in place.
@@ -321,21 +321,22 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
     }
   }
   // State buffers: double-buffered where the tiled step gathers them.
+  // One entry more than the cells: entry N is the ghost state of code -3.
   const bool twoU = tiled && !mixed, twoRec = tiled && mixed;
   for (int k = 0; k < 2; ++k) {
     if (k == 0 || twoU) {
-      HIP_TRY(h->alloc(&h->U4b[k], N));
-      HIP_TRY(h->alloc(&h->UEb[k], N));
-      HIP_TRY(h->alloc(&h->W4b[k], N));
-      HIP_TRY(h->alloc(&h->Wpb[k], N));
+      HIP_TRY(h->alloc(&h->U4b[k], N + 1));
+      HIP_TRY(h->alloc(&h->UEb[k], N + 1));
+      HIP_TRY(h->alloc(&h->W4b[k], N + 1));
+      HIP_TRY(h->alloc(&h->Wpb[k], N + 1));
     } else {
       h->U4b[k] = h->U4b[0], h->UEb[k] = h->UEb[0], h->W4b[k] = h->W4b[0], h->Wpb[k] = h->Wpb[0];
     }
     if (mixed && (k == 0 || twoRec)) {
-      HIP_TRY(h->alloc(&h->Wvab[k], N));
-      HIP_TRY(h->alloc(&h->Wsdb[k], N));
+      HIP_TRY(h->alloc(&h->Wvab[k], N + 1));
+      HIP_TRY(h->alloc(&h->Wsgb[k], N + 1));
     } else {
-      h->Wvab[k] = h->Wvab[0], h->Wsdb[k] = h->Wsdb[0];
+      h->Wvab[k] = h->Wvab[0], h->Wsgb[k] = h->Wsgb[0];
     }
   }
   if (variant == kMixedMlp) HIP_TRY(h->alloc(&d.mlp_list, N));
@@ -346,6 +347,7 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
   h->host_sc->last_speed = 1.0;
   HIP_TRY(hipMemcpy(d.sc, h->host_sc, sizeof(Scalars), hipMemcpyHostToDevice));
   HIP_TRY(launch_clear_scalars(h->now(), h->stream));
+  HIP_TRY(launch_ghost_cell(h->dev(0, 1), mixed, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   *out = h.release();
   return RHLLC_OK;

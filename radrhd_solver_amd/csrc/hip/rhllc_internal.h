@@ -38,7 +38,7 @@ struct Dev {
   int32_t eos;  // Eos (rhllc_physics.h): kIdeal or kTaub
   double gamma, Gg, cfl_hmin;
   // faces
-  const int2 *face_lr;      // (left cell, right cell); right < 0: fixed ghost state
+  const int2 *face_lr;      // (left cell, right cell); right = n_cells: the fixed ghost state
   const double4 *face_nS;   // (nx, ny, nz, S), FP64 path
   const float3 *face_Sn32;  // S * (nx, ny, nz) in FP32, mixed path
   double4 *F4;              // flux (d, mx, my, mz) * S, FP64 path
@@ -60,13 +60,13 @@ struct Dev {
   double4 *W4;   // (rho, vx, vy, vz), FP64 path
   double *Wp;    // p (both paths: the FP64 pressure)
   float4 *Wva;   // mixed path: (vx, vy, vz, p) in FP32 (mixed_record)
-  float4 *Wsd;   // mixed path: (s - D, D, cs2, extension counts of s - D and D), s = rho h W^2
+  double2 *Wsg;  // mixed path: (s - D, D) in FP64, s = rho h W^2
   double4 *U4o;
   double *UEo;
   double4 *W4o;
   double *Wpo;
   float4 *Wvao;
-  float4 *Wsdo;
+  double2 *Wsgo;
   // Tiled step: tile t owns cells [t kTileCells, (t + 1) kTileCells) and
   // computes every face of them: its own faces (those whose earlier cell is in
   // the tile, a contiguous range since faces are grouped by their earlier
@@ -92,7 +92,9 @@ struct Dev {
   const double *tab;
   int32_t tab_nx, tab_ny;
   double tab_min_x, tab_max_x, tab_step_x, tab_min_y, tab_max_y, tab_step_y;
-  // fixed ghost state for boundary code -3
+  // Fixed ghost state for boundary code -3.  The state arrays hold it as one
+  // more cell, index n_cells (written by launch_ghost_cell), which no step
+  // updates; faces of code -3 gather it as their right cell.
   double ghostW[5], ghostU[5];
   // surrogate c2p guess (RHLLC_GUESS_MLP): FP32 model and bf16 W2
   const float *mlpP;
@@ -124,6 +126,9 @@ hipError_t launch_dt(const Dev &d, hipStream_t s);
 hipError_t launch_clear_scalars(const Dev &d, hipStream_t s);
 // Mixed path: W (FP64) recovered from U by c2p (start from the FP32 pressure).
 hipError_t launch_seed_mixed(const Dev &d, hipStream_t s);
+// Writes the ghost state into entry n_cells of the input and output buffers
+// of d (U and W on the FP64 path, the cell record on the mixed path).
+hipError_t launch_ghost_cell(const Dev &d, bool mixed, hipStream_t s);
 hipError_t launch_recover_W(const Dev &d, hipStream_t s);
 // The radiation source alone (d.rad_FU set): U += tau G(W), then c2p.
 hipError_t launch_add_rad(const Dev &d, bool mixed, double tau, hipStream_t s);

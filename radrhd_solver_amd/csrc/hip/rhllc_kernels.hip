@@ -64,16 +64,18 @@ __device__ __forceinline__ void load_W(const Dev &d, int c, State<double> &W) {
 }
 // The FP32 primitives of the mixed path, widened (rho = D sqrt(1 - v^2)).
 __device__ __forceinline__ void load_W32(const Dev &d, int c, State<double> &W) {
-  const MixedIn r = mixed_in(ld(d.Wva, c), ld(d.Wsd, c));
+  const MixedIn r = mixed_in(ld(d.Wva, c), ld(d.Wsg, c));
   W.v[0] = r.v[0], W.v[1] = r.v[1], W.v[2] = r.v[2], W.p = r.p;
   W.d = r.D * sqrt(1.0 - dot3(W.v, W.v));
 }
 template <int EOS>
-__device__ __forceinline__ void store_W32(const Dev &d, float4 *va_out, float4 *sd_out, int c, const State<double> &W) {
-  float4 va, sd;
-  mixed_record<EOS>(d.gamma, d.Gg, W, va, sd);
+__device__ __forceinline__ void store_W32(const Dev &d, float4 *va_out, double2 *sg_out, int c,
+                                          const State<double> &W) {
+  float4 va;
+  double2 sg;
+  mixed_record<EOS>(d.Gg, W, va, sg);
   st(va_out, c, va);
-  st(sd_out, c, sd);
+  st(sg_out, c, sg);
 }
 
 }  // namespace rhllc_dev
@@ -142,30 +144,16 @@ __device__ __forceinline__ double face_flux(const Dev &d, const FaceIn<MIXED> &f
     State<double> UL, WL, UR, WR;
     load_U(d, lr.x, UL);
     load_W(d, lr.x, WL);
-    if (lr.y >= 0) {
-      load_U(d, lr.y, UR);
-      load_W(d, lr.y, WR);
-    } else {
-      UR.d = d.ghostU[0], UR.v[0] = d.ghostU[1], UR.v[1] = d.ghostU[2], UR.v[2] = d.ghostU[3], UR.p = d.ghostU[4];
-      WR.d = d.ghostW[0], WR.v[0] = d.ghostW[1], WR.v[1] = d.ghostW[2], WR.v[2] = d.ghostW[3], WR.p = d.ghostW[4];
-    }
+    load_U(d, lr.y, UR);  // lr.y = n_cells on a face of code -3: the ghost state
+    load_W(d, lr.y, WR);
     return hllc_flux<double, Ieee, EOS>(d.gamma, d.Gg, UL, UR, WL, WR, nS.x, nS.y, nS.z, nS.w, F);
   } else {
     // FP32 storage (cell records, geometry), FP64 arithmetic with the fast
     // divide/sqrt policy.
     const float3 Sn = fi.g;
-    const MixedIn WL = mixed_in(ld(d.Wva, lr.x), ld(d.Wsd, lr.x));
-    MixedIn WR;
-    if (lr.y >= 0) {
-      WR = mixed_in(ld(d.Wva, lr.y), ld(d.Wsd, lr.y));
-    } else {
-      State<double> g;
-      g.d = d.ghostW[0], g.v[0] = d.ghostW[1], g.v[1] = d.ghostW[2], g.v[2] = d.ghostW[3], g.p = d.ghostW[4];
-      float4 va, sd;
-      mixed_record<EOS>(d.gamma, d.Gg, g, va, sd);
-      WR = mixed_in(va, sd);
-    }
-    return mixed_face_flux(WL, WR, Sn.x, Sn.y, Sn.z, F);
+    const MixedIn WL = mixed_in(ld(d.Wva, lr.x), ld(d.Wsg, lr.x));
+    const MixedIn WR = mixed_in(ld(d.Wva, lr.y), ld(d.Wsg, lr.y));  // n_cells on code -3: the ghost
+    return mixed_face_flux<EOS>(d.gamma, WL, WR, Sn.x, Sn.y, Sn.z, F);
   }
 }
 
@@ -436,7 +424,7 @@ __device__ __forceinline__ void finish_cell(const Dev &d, int c, bool live, Stat
       st(d.W4o, c, make_double4(W.d, W.v[0], W.v[1], W.v[2]));
       st(d.Wpo, c, W.p);
     } else {
-      store_W32<EOS>(d, d.Wvao, d.Wsdo, c, W);
+      store_W32<EOS>(d, d.Wvao, d.Wsgo, c, W);
     }
   }
   if (__ballot(live && (code != 0 || fixed || fell_back)) != 0) {  // rare: skip eight ballots otherwise
@@ -638,7 +626,25 @@ __global__ __launch_bounds__(kBlock) void k_seed_mixed(Dev d) {
   if (c >= d.n_cells) return;
   State<double> W;
   load_W(d, c, W);
-  store_W32<EOS>(d, d.Wva, d.Wsd, c, W);
+  store_W32<EOS>(d, d.Wva, d.Wsg, c, W);
+}
+
+// The ghost state of boundary code -3 as entry n_cells of the state arrays,
+// input and output buffers alike (one thread).
+template <int EOS>
+__global__ void k_ghost_cell(Dev d, bool mixed) {
+  const int c = d.n_cells;
+  State<double> W;
+  W.d = d.ghostW[0], W.v[0] = d.ghostW[1], W.v[1] = d.ghostW[2], W.v[2] = d.ghostW[3], W.p = d.ghostW[4];
+  if (mixed) {
+    store_W32<EOS>(d, d.Wva, d.Wsg, c, W);
+    store_W32<EOS>(d, d.Wvao, d.Wsgo, c, W);
+    return;
+  }
+  const double4 u = make_double4(d.ghostU[0], d.ghostU[1], d.ghostU[2], d.ghostU[3]);
+  const double4 w = make_double4(W.d, W.v[0], W.v[1], W.v[2]);
+  d.U4[c] = u, d.UE[c] = d.ghostU[4], d.W4[c] = w, d.Wp[c] = W.p;
+  d.U4o[c] = u, d.UEo[c] = d.ghostU[4], d.W4o[c] = w, d.Wpo[c] = W.p;
 }
 
 // Mixed path: FP64 primitives from the FP64 conserved state (for download).
@@ -684,7 +690,7 @@ __global__ __launch_bounds__(kBlock) void k_add_rad(Dev d, double tau) {
       d.W4[c] = make_double4(W.d, W.v[0], W.v[1], W.v[2]);
       d.Wp[c] = W.p;
     } else {
-      store_W32<EOS>(d, d.Wva, d.Wsd, c, W);
+      store_W32<EOS>(d, d.Wva, d.Wsg, c, W);
     }
   }
   if (__ballot(fixed) != 0) {
@@ -877,6 +883,13 @@ hipError_t launch_seed_mixed(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_seed_mixed<kIdeal>, dim3(blocks_for(d.n_cells)), dim3(kBlock), 0, s, d);
   return hipGetLastError();
 }
+hipError_t launch_ghost_cell(const Dev &d, bool mixed, hipStream_t s) {
+  if (d.eos == kTaub)
+    hipLaunchKernelGGL(k_ghost_cell<kTaub>, dim3(1), dim3(1), 0, s, d, mixed);
+  else
+    hipLaunchKernelGGL(k_ghost_cell<kIdeal>, dim3(1), dim3(1), 0, s, d, mixed);
+  return hipGetLastError();
+}
 hipError_t launch_recover_W(const Dev &d, hipStream_t s) {
   if (d.n_cells == 0) return hipSuccess;
   if (d.eos == kTaub)
@@ -931,12 +944,13 @@ __global__ void k_eval_flux(double gamma, int64_t n, const double *UL, const dou
     // flux rounded to FP32.
     // The cell records the mixed cell kernel would store for these states.
     State<double> ff;
-    float4 la, ls, ra, rs;
-    mixed_record<EOS>(gamma, Gg, wl, la, ls);
-    mixed_record<EOS>(gamma, Gg, wr, ra, rs);
+    float4 la, ra;
+    double2 ls, rs;
+    mixed_record<EOS>(Gg, wl, la, ls);
+    mixed_record<EOS>(Gg, wr, ra, rs);
     const double *g = nS4 + 4 * i;
-    speed[i] = mixed_face_flux(mixed_in(la, ls), mixed_in(ra, rs), (float)(g[0] * g[3]), (float)(g[1] * g[3]),
-                               (float)(g[2] * g[3]), ff);
+    speed[i] = mixed_face_flux<EOS>(gamma, mixed_in(la, ls), mixed_in(ra, rs), (float)(g[0] * g[3]),
+                                    (float)(g[1] * g[3]), (float)(g[2] * g[3]), ff);
     const rhllc_ext::Packed pk = rhllc_ext::pack(ff.d, ff.v[0], ff.v[1], ff.v[2], ff.p);
     rhllc_ext::unpack(pk.a, pk.b, F + 5 * i);
   }

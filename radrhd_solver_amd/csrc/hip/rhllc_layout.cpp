@@ -66,9 +66,11 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
     std::memcpy(&S, r + 32, 8);
     const int32_t l = cnew[fl[f]];
     nb[i] = fr[f] >= 0 ? cnew[fr[f]] : -1;
-    // Copy-type boundaries (Cone geometry: every code but -3) read the cell itself.
+    // Copy-type boundaries (Cone geometry: every code but -3) read the cell
+    // itself; code -3 reads the fixed ghost state, held as one more cell
+    // (index N) after the mesh's cells.
     L.lr[2 * (size_t)i] = l;
-    L.lr[2 * (size_t)i + 1] = fr[f] >= 0 ? nb[i] : (fr[f] == RHLLC_BOUND_OUT_SOURCE ? -1 : l);
+    L.lr[2 * (size_t)i + 1] = fr[f] >= 0 ? nb[i] : (fr[f] == RHLLC_BOUND_OUT_SOURCE ? N : l);
     if (opt.mixed) {
       float *q = &L.Sn32[3 * (size_t)i];
       q[0] = (float)(n[0] * S), q[1] = (float)(n[1] * S), q[2] = (float)(n[2] * S);

@@ -22,7 +22,7 @@ struct Options {
 
 struct Layout {
   std::vector<int32_t> cell_new_to_old, face_new_to_old;
-  rhllc_mesh::RawVec<int32_t> lr;      // per face (left, right): right = cell, -1 ghost (-3), left (copy-type codes)
+  rhllc_mesh::RawVec<int32_t> lr;      // per face (left, right): right = cell, N ghost (-3), left (copy-type codes)
   rhllc_mesh::RawVec<double> nS;       // per face (nx, ny, nz, S), FP64 path
   rhllc_mesh::RawVec<float> Sn32;      // per face S (nx, ny, nz) in FP32, mixed path
   rhllc_mesh::RawVec<double> V;        // per cell, FP64 path

@@ -485,65 +485,54 @@ __device__ __forceinline__ R hllc_flux(R G1, R Gg, const State<R> &cUL, const St
 // every tangential flux component is its tangential momentum times one factor
 // c_t, so the global momentum flux is c_t m + (F_n - c_t m_n) n, with F_n the
 // normal component.  Only v.n and m.n of each side are needed.
-// What the face kernel reads of a cell on the mixed path, stored by the cell
-// kernel as two float4: (vx, vy, vz, p) in FP32, and (sigma, D, cs2, e) with
-// sigma = s - D, s = rho h W^2 (m = s v, E = s - p), D = rho W,
-// cs2 = gamma p/(rho h).  sigma and D are formed in FP64 from the FP32-rounded
-// primitives and stored as FP32 plus 8-bit extensions packed in e
-// (flux_ext.h), so they match the stored v and p to 2^-31: at Lorentz factors
-// ~10 an s rounded on its own (inconsistent with v, p by 6e-8) raised the
-// transverse-momentum L1 of a shear layer from 1e-6 to 2e-5.  The record
-// holds sigma rather than s because E - D = sigma - p is the thermal and
-// kinetic energy: in a gas at T ~ 1e4 K (p / rho c^2 ~ 1e-9) it is below the
-// 2^-31 resolution of s, and an s-based record lost the pressure (momentum L1
-// 3e-3 against FP64 after 20 steps).  Forming these once per cell, instead of
-// once per face side, removes the p2c and sound-speed divides and roots from
-// the face kernel.  rho is not stored: rho = D sqrt(1 - v^2).
+// What the face work reads of a cell on the mixed path, stored by the cell
+// work as two 16-byte halves: (vx, vy, vz, p) in FP32, and (sigma, D) in FP64
+// with sigma = s - D, s = rho h W^2 (m = s v, E = s - p), D = rho W.  sigma
+// and D are formed in FP64 from the FP32-rounded primitives, so they match
+// the stored v and p: at Lorentz factors ~10 an s rounded on its own
+// (inconsistent with v, p by 6e-8) raised the transverse-momentum L1 of a
+// shear layer from 1e-6 to 2e-5.  The record holds sigma rather than s
+// because E - D = sigma - p is the thermal and kinetic energy: in a gas at
+// T ~ 1e4 K (p / rho c^2 ~ 1e-9) it is far below the resolution of an FP32 s.
+// Forming these once per cell, instead of once per face side, removes the
+// p2c divides and roots from the face work.  The sound speed is not stored:
+// a face side forms it from rho h = s (1 - v^2) (ideal gas) or from
+// rho = D sqrt(1 - v^2) (Taub-Mathews).
 struct MixedIn {
-  double v[3], p, s, D, cs2;
+  double v[3], p, s, D;
   double sigma;  // s - D
 };
 
 // The record of an FP64 primitive state (|v| < 1).
 template <int EOS = kIdeal>
-__device__ __forceinline__ void mixed_record(double gamma, double Gg, const State<double> &W, float4 &va,
-                                             float4 &sd) {
+__device__ __forceinline__ void mixed_record(double Gg, const State<double> &W, float4 &va, double2 &sg) {
 #pragma clang fp contract(fast)
   using A = Fast;
   va = make_float4((float)W.v[0], (float)W.v[1], (float)W.v[2], (float)W.p);
   const double rho = (float)W.d, p = va.w;
   const double v[3] = {va.x, va.y, va.z};
   const double vv = dot3(v, v);
-  const double lor = A::rsqrt(1.0 - vv);  // one reciprocal root: 4.2e-15, far below the record's 2^-31
+  const double lor = A::rsqrt(1.0 - vv);  // one reciprocal root: 4.2e-15
   const double lor2 = lor * lor;
-  double rhm1, cs2;  // rho (h - 1), sound speed^2
+  double rhm1;  // rho (h - 1)
   if (EOS == kIdeal) {
     rhm1 = Gg * p;
-    cs2 = A::div(gamma * p, rho + rhm1);
   } else {
     const double th = A::div(p, rho);
     const double r = A::sqrt_(2.25 * th * th + 1.0);
-    const double h = 2.5 * th + r;
     rhm1 = 2.5 * p + rho * A::div(2.25 * th * th, r + 1.0);  // rho (h - 1) without the cancellation
-    cs2 = A::div(th * (5.0 * h - 8.0 * th), 3.0 * h * (h - th));
   }
   // sigma = s - D = rho W (h W - 1) = W (rho (h - 1) W + rho (W - 1)), W - 1 = W^2 v^2 / (W + 1)
-  const double sigma = lor * (rhm1 * lor + rho * A::div(lor2 * vv, lor + 1.0));
-  uint32_t es, eD;
-  sd.x = rhllc_ext::split<8>(sigma, es);
-  sd.y = rhllc_ext::split<8>(rho * lor, eD);
-  sd.z = (float)cs2;
-  sd.w = rhllc_ext::bits_f32(es | (eD << 8));
+  sg.x = lor * (rhm1 * lor + rho * A::div(lor2 * vv, lor + 1.0));
+  sg.y = rho * lor;
 }
 
-__device__ __forceinline__ MixedIn mixed_in(const float4 &va, const float4 &sd) {
+__device__ __forceinline__ MixedIn mixed_in(const float4 &va, const double2 &sg) {
   MixedIn o;
-  const uint32_t e = rhllc_ext::f32_bits(sd.w);
   o.v[0] = va.x, o.v[1] = va.y, o.v[2] = va.z, o.p = va.w;
-  o.sigma = rhllc_ext::join<8>(sd.x, e);
-  o.D = rhllc_ext::join<8>(sd.y, e >> 8);
+  o.sigma = sg.x;
+  o.D = sg.y;
   o.s = o.D + o.sigma;
-  o.cs2 = sd.z;
   return o;
 }
 
@@ -555,7 +544,8 @@ struct MixedSide {
   double cmin, cmax;        // signal speeds
 };
 
-__device__ __forceinline__ void mixed_side(const MixedIn &w, const double *n, MixedSide &o) {
+template <int EOS>
+__device__ __forceinline__ void mixed_side(double gamma, const MixedIn &w, const double *n, MixedSide &o) {
 #pragma clang fp contract(fast)
   using A = FaceFast;
   o.p = w.p;
@@ -574,7 +564,14 @@ __device__ __forceinline__ void mixed_side(const MixedIn &w, const double *n, Mi
   o.E = w.D + (w.sigma - w.p);  // E - D = sigma - p, formed from the two small terms
   o.vn = dot3(o.v, n);
   o.mn = o.s * o.vn;
-  const double cs2 = w.cs2;
+  double cs2;  // sound speed^2, to the hardware reciprocal's 5e-8 (it enters only the signal speeds)
+  if (EOS == kIdeal) {
+    cs2 = gamma * o.p * A::rcp_rough(o.s * (1.0 - V2));  // gamma p / (rho h)
+  } else {
+    const double th = o.p * A::rcp_rough(o.D * A::sqrt_rough(1.0 - V2));  // p / rho
+    const double h = 2.5 * th + A::sqrt_rough(2.25 * th * th + 1.0);
+    cs2 = th * (5.0 * h - 8.0 * th) * A::rcp_rough(3.0 * h * (h - th));
+  }
   const double vn2 = o.vn * o.vn;
   const double sroot = A::sqrt_rough(cs2 * (1.0 - vn2 - (V2 - vn2) * cs2) * (1.0 - V2));
   const double r = A::rcp_rough(1.0 - V2 * cs2);
@@ -590,8 +587,9 @@ __device__ __forceinline__ void mixed_side(const MixedIn &w, const double *n, Mi
 // (lL < 0 < lR), so the face needs no branch.
 // F receives S * flux with F.p the flux of E - D (not of E); returns
 // max(lL, lR) as hllc_flux does.
-__device__ __forceinline__ double mixed_face_flux(const MixedIn &WL, const MixedIn &WR, float sx, float sy,
-                                                  float sz, State<double> &F) {
+template <int EOS>
+__device__ __forceinline__ double mixed_face_flux(double gamma, const MixedIn &WL, const MixedIn &WR, float sx,
+                                                  float sy, float sz, State<double> &F) {
 #pragma clang fp contract(fast)
   using A = FaceFast;
   double n[3] = {sx, sy, sz};
@@ -600,8 +598,8 @@ __device__ __forceinline__ double mixed_face_flux(const MixedIn &WL, const Mixed
   const double S = q * rn;
   n[0] *= rn, n[1] *= rn, n[2] *= rn;
   MixedSide L, R;
-  mixed_side(WL, n, L);
-  mixed_side(WR, n, R);
+  mixed_side<EOS>(gamma, WL, n, L);
+  mixed_side<EOS>(gamma, WR, n, R);
   const double lL = smin(L.cmin, R.cmin);
   const double lR = smax(L.cmax, R.cmax);
   // contact solve (rhllc_flux_stab.cpp:362-421)
