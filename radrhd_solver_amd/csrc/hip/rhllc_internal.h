@@ -48,7 +48,7 @@ struct Dev {
   // cells
   const int4 *cell_faces;  // face index | 0x80000000 when the cell is id_r (sign bit 0)
   const double *cell_V;    // FP64 path
-  const float4 *cell_Vd;   // mixed path: (V, closure defect sum_j sign_j face_Sn32_j)
+  const float4 *cell_Vd;   // mixed path: (1/V, closure defect sum_j sign_j face_Sn32_j)
   // State at the start of the step (read by the face work and by each cell's
   // own update) and where the cell update writes it.  The two-kernel step
   // updates in place (out == in); the tiled step gathers neighbours' state

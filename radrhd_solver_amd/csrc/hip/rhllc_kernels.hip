@@ -323,7 +323,7 @@ __device__ __forceinline__ void update_U(const Dev &d, int c, double tau, const 
     s.v[1] -= U.v[1] * vdl + p * dl[1];
     s.v[2] -= U.v[2] * vdl + p * dl[2];
     s.p -= mdl;
-    k = Fast::div(tau, (double)vd.x);
+    k = tau * (double)vd.x;  // 1/V rounded to FP32 once, on the host
   } else {
     k = tau / ld(d.cell_V, c);
   }

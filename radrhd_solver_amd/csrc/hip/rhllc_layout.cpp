@@ -108,7 +108,7 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
         dl[0] += sg * (double)q[0], dl[1] += sg * (double)q[1], dl[2] += sg * (double)q[2];
       }
       float *o = &L.Vd[4 * (size_t)i];
-      o[0] = (float)V, o[1] = (float)dl[0], o[2] = (float)dl[1], o[3] = (float)dl[2];
+      o[0] = (float)(1.0 / V), o[1] = (float)dl[0], o[2] = (float)dl[1], o[3] = (float)dl[2];
     }
   }
   if (!opt.tiled) {

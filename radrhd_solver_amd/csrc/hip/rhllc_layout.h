@@ -26,7 +26,7 @@ struct Layout {
   rhllc_mesh::RawVec<double> nS;       // per face (nx, ny, nz, S), FP64 path
   rhllc_mesh::RawVec<float> Sn32;      // per face S (nx, ny, nz) in FP32, mixed path
   rhllc_mesh::RawVec<double> V;        // per cell, FP64 path
-  rhllc_mesh::RawVec<float> Vd;        // per cell (V, delta) in FP32, mixed path
+  rhllc_mesh::RawVec<float> Vd;        // per cell (1/V, delta) in FP32, mixed path
   rhllc_mesh::RawVec<int32_t> cf;      // per cell 4 face indices | bit 31 when the cell is id_r (two-kernel step)
   // tiled step
   int32_t n_tiles = 0, max_slots = 0;
