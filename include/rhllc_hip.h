@@ -75,6 +75,19 @@ typedef struct rhllc_params {
   int32_t eos;            /* RHLLC_EOS_*; 0 (zero-initialised params) is the ideal gas.
                              RHLLC_GUESS_MLP needs RHLLC_EOS_IDEAL */
   int32_t schedule;       /* RHLLC_SCHED_*; 0 (zero-initialised params) is the tiled step */
+  /* The boundary branches of GEOMETRY_TYPE builds other than the default
+   * Cone (rhllc_utils.cpp:58-110, include/solvers_config.h:19-29); all zero
+   * is Cone: */
+  int32_t lock_wall;        /* 1: boundary code -2 (e_bound_lock) mirrors the normal velocity,
+                               as every type but Cone does; 0: it copies the cell */
+  int32_t has_inner_source; /* 1: code -4 (e_bound_inner_source) takes the fixed ghost state
+                               below (Sphere, Cone_JET); 0: it copies the cell */
+  double inner_source_W[5], inner_source_U[5];
+  int32_t reject_outer_surface; /* 1: a mesh with code -5 faces is RHLLC_ERR_ARG (every type but
+                                   Cone aborts on them, D_LD at rhllc_utils.cpp:121-129) */
+  int32_t out_source_copy;      /* 1 (with has_out_source 0): code -3 copies the cell, as every
+                                   type but Cone and Cylinder does; 0: a mesh with code -3 faces
+                                   needs the out_source state */
 } rhllc_params;
 
 /* Kernel schedule of the step; the results are the same bits either way on

@@ -38,6 +38,10 @@ class Physics(C.Structure):
         ("out_source_W", C.c_double * 5),
         ("out_source_U", C.c_double * 5),
         ("eos", C.c_int),
+        ("lock_wall", C.c_int),
+        ("has_inner_source", C.c_int),
+        ("inner_source_W", C.c_double * 5),
+        ("inner_source_U", C.c_double * 5),
     ]
 
 
@@ -101,6 +105,7 @@ def _declare(path):
         L.orc_c2p_n.argtypes = [vp, i64, vp, vp, vp, vp]
         L.orc_pressure_fix_n.argtypes = [vp, i64, vp, vp, vp]
         L.orc_rotation.argtypes = [vp, vp]
+        L.orc_wall_mirror.argtypes = [vp, vp]
         L.orc_flux_n.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
         L.orc_load.argtypes = [i64, vp, vp, i64, vp, vp, vp]
         L.orc_step.argtypes = [vp, i64, vp, i64, vp, dbl, vp]
@@ -201,6 +206,14 @@ def rotation(n):
     T = np.zeros(9)
     lib().orc_rotation(n.ctypes.data, T.ctypes.data)
     return T.reshape(3, 3)
+
+
+def wall_mirror(n, v):
+    """The e_bound_lock reflection of v about the face normal n (rhllc_utils.cpp:95-110)."""
+    n = np.ascontiguousarray(n, dtype=np.float64).reshape(3)
+    v = np.array(v, dtype=np.float64).reshape(3)
+    lib().orc_wall_mirror(n.ctypes.data, v.ctypes.data)
+    return v
 
 
 def flux(UL, UR, WL, WR, nS4, gamma=5.0 / 3.0, eos=EOS_IDEAL):

@@ -375,10 +375,27 @@ double orc_flux(const orc_physics_t *ph, const orc_flux_t *cUL, const orc_flux_t
 static inline int32_t face_id_r(const orc_geo_face_t *g) { return ((int32_t)(g->w1 << 6)) >> 6; }
 static inline int32_t face_id_l(const orc_geo_face_t *g) { return (int32_t)(g->w0 & 0x1FFFFFFu); }
 
-/* rhllc_utils.cpp:45-133 with GEOMETRY_TYPE Cone (the reference default,
- * include/solvers_config.h:29): every boundary code except e_bound_out_source
- * (-3) copies the cell; -3 takes a fixed ghost state.  Unknown codes (< -5)
- * copy the cell here (the reference aborts). */
+/* The wall branch of rhllc_utils.cpp:95-110: the normal velocity mirrored,
+ * v - 2 (v.n) n, with v.n summed as (v0 n0 + v1 n1) + v2 n2.  The reference
+ * writes it as T^T diag(-1, 1, 1) T v with T = GetRotationMatrix(n), forming
+ * T^T by `T = T.transpose()` on a fixed-size matrix, which Eigen's aliasing
+ * check rejects in a build without NDEBUG and which, with NDEBUG, overwrites
+ * T's lower triangle with its upper one before the second product.  This is
+ * the reflection that code intends (equal to it up to rounding; parity
+ * unpinned against the reference, pinned by the reflection's closed form). */
+void orc_wall_mirror(const double n[3], double v[3]) {
+  const double vn = (v[0] * n[0] + v[1] * n[1]) + v[2] * n[2];
+  const double t = vn + vn;
+  v[0] = v[0] - t * n[0], v[1] = v[1] - t * n[1], v[2] = v[2] - t * n[2];
+}
+
+/* rhllc_utils.cpp:45-133.  With GEOMETRY_TYPE Cone (the reference default,
+ * include/solvers_config.h:29) every boundary code except e_bound_out_source
+ * (-3) copies the cell, and -3 takes a fixed ghost state.  The other types
+ * differ in three branches, each an option of ph: -2 mirrors the normal
+ * velocity (lock_wall), -4 takes a fixed state (has_inner_source), -3 a fixed
+ * state or the copy (has_out_source).  Unknown codes (< -5) copy the cell here
+ * (the reference aborts). */
 void orc_bound(const orc_physics_t *ph, const orc_face_t *f, const orc_elem_t *cells, orc_flux_t *Wg,
                orc_flux_t *Ug) {
   const int32_t id_l = face_id_l(&f->geo);
@@ -394,6 +411,12 @@ void orc_bound(const orc_physics_t *ph, const orc_face_t *f, const orc_elem_t *c
   if (id_r == -3 && ph->has_out_source) {
     *Ug = ph->out_source_U;
     *Wg = ph->out_source_W;
+  } else if (id_r == -4 && ph->has_inner_source) {
+    *Ug = ph->inner_source_U;
+    *Wg = ph->inner_source_W;
+  } else if (id_r == -2 && ph->lock_wall) {
+    orc_wall_mirror(f->geo.n, Ug->v);
+    orc_wall_mirror(f->geo.n, Wg->v);
   }
 }
 

@@ -59,6 +59,10 @@ typedef struct {
   int has_out_source;   /* e_bound_out_source ghost state below is valid */
   orc_flux_t out_source_W, out_source_U;
   int eos;              /* ORC_EOS_IDEAL or ORC_EOS_TAUB (include/hd_config.h:16-20) */
+  /* The boundary branches of the other GEOMETRY_TYPE builds (rhllc_utils.cpp:58-110): */
+  int lock_wall;        /* e_bound_lock (-2) mirrors the normal velocity (every type but Cone) */
+  int has_inner_source; /* e_bound_inner_source (-4) takes the fixed state below (Sphere, Cone_JET) */
+  orc_flux_t inner_source_W, inner_source_U;
 } orc_physics_t;
 
 #define ORC_EOS_IDEAL 0
@@ -76,6 +80,7 @@ void orc_p2c(const orc_physics_t *ph, orc_flux_t *W, orc_flux_t *U);
 int orc_c2p(const orc_physics_t *ph, const orc_flux_t *U, orc_flux_t *W, int *iters);
 int orc_pressure_fix(const orc_physics_t *ph, orc_flux_t *U, orc_flux_t *W);
 void orc_rotation(const double n[3], double T[9]); /* row-major T[3*r+c] */
+void orc_wall_mirror(const double n[3], double v[3]); /* the e_bound_lock reflection */
 double orc_flux(const orc_physics_t *ph, const orc_flux_t *UL, const orc_flux_t *UR, const orc_flux_t *WL,
                 const orc_flux_t *WR, const double n[3], double S, orc_flux_t *F);
 void orc_bound(const orc_physics_t *ph, const orc_face_t *f, const orc_elem_t *cells, orc_flux_t *W_ghost,

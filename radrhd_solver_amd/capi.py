@@ -46,6 +46,12 @@ class Params(C.Structure):
         ("out_source_U", C.c_double * 5),
         ("eos", C.c_int32),
         ("schedule", C.c_int32),
+        ("lock_wall", C.c_int32),
+        ("has_inner_source", C.c_int32),
+        ("inner_source_W", C.c_double * 5),
+        ("inner_source_U", C.c_double * 5),
+        ("reject_outer_surface", C.c_int32),
+        ("out_source_copy", C.c_int32),
     ]
 
 
@@ -156,7 +162,8 @@ class Solver:
 
     def __init__(self, mesh, gamma=5.0 / 3.0, cfl=0.4, h_min=None, precision=PREC_FP64,
                  guess=GUESS_REFERENCE, reorder=True, device=0, collect_stats=False, out_source=None,
-                 eos=EOS_IDEAL, schedule=SCHED_TILED):
+                 eos=EOS_IDEAL, schedule=SCHED_TILED, inner_source=None, lock_wall=False,
+                 reject_outer_surface=False, out_source_copy=False):
         L = _lib()
         p = Params()
         p.gamma, p.cfl = gamma, cfl
@@ -170,6 +177,14 @@ class Solver:
             for k in range(5):
                 p.out_source_W[k] = out_source[0][k]
                 p.out_source_U[k] = out_source[1][k]
+        if inner_source is not None:
+            p.has_inner_source = 1
+            for k in range(5):
+                p.inner_source_W[k] = inner_source[0][k]
+                p.inner_source_U[k] = inner_source[1][k]
+        p.lock_wall = int(bool(lock_wall))
+        p.reject_outer_surface = int(bool(reject_outer_surface))
+        p.out_source_copy = int(bool(out_source_copy))
         self.params = p
         self.n_cells, self.n_faces = mesh.n_cells, mesh.n_faces
         h = C.c_void_p()

@@ -228,6 +228,10 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
   opt.reorder = p.reorder != 0;
   opt.tiled = tiled;
   opt.has_out_source = p.has_out_source != 0;
+  opt.has_inner_source = p.has_inner_source != 0;
+  opt.lock_wall = p.lock_wall != 0;
+  opt.reject_outer_surface = p.reject_outer_surface != 0;
+  opt.out_source_copy = p.out_source_copy != 0;
   opt.tile_cells = kTileCells;
   rhllc_layout::Layout L;
   {
@@ -256,7 +260,10 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
   d.gamma = p.eos == RHLLC_EOS_IDEAL ? p.gamma : 0.0;
   d.Gg = p.eos == RHLLC_EOS_IDEAL ? p.gamma / (p.gamma - 1.0) : 0.0;
   d.cfl_hmin = p.cfl * p.h_min;
-  for (int k = 0; k < 5; ++k) d.ghostW[k] = p.out_source_W[k], d.ghostU[k] = p.out_source_U[k];
+  for (int k = 0; k < 5; ++k) {
+    d.ghostW[k] = p.out_source_W[k], d.ghostU[k] = p.out_source_U[k];
+    d.ghost2W[k] = p.inner_source_W[k], d.ghost2U[k] = p.inner_source_U[k];
+  }
 
   // Uploads count elements of a host array into a new device array (at least
   // one element is allocated; an empty array copies nothing).
@@ -321,20 +328,21 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
     }
   }
   // State buffers: double-buffered where the tiled step gathers them.
-  // One entry more than the cells: entry N is the ghost state of code -3.
+  // Two entries more than the cells: N and N + 1 hold the ghost states of
+  // codes -3 and -4.
   const bool twoU = tiled && !mixed, twoRec = tiled && mixed;
   for (int k = 0; k < 2; ++k) {
     if (k == 0 || twoU) {
-      HIP_TRY(h->alloc(&h->U4b[k], N + 1));
-      HIP_TRY(h->alloc(&h->UEb[k], N + 1));
-      HIP_TRY(h->alloc(&h->W4b[k], N + 1));
-      HIP_TRY(h->alloc(&h->Wpb[k], N + 1));
+      HIP_TRY(h->alloc(&h->U4b[k], N + 2));
+      HIP_TRY(h->alloc(&h->UEb[k], N + 2));
+      HIP_TRY(h->alloc(&h->W4b[k], N + 2));
+      HIP_TRY(h->alloc(&h->Wpb[k], N + 2));
     } else {
       h->U4b[k] = h->U4b[0], h->UEb[k] = h->UEb[0], h->W4b[k] = h->W4b[0], h->Wpb[k] = h->Wpb[0];
     }
     if (mixed && (k == 0 || twoRec)) {
-      HIP_TRY(h->alloc(&h->Wvab[k], N + 1));
-      HIP_TRY(h->alloc(&h->Wsgb[k], N + 1));
+      HIP_TRY(h->alloc(&h->Wvab[k], N + 2));
+      HIP_TRY(h->alloc(&h->Wsgb[k], N + 2));
     } else {
       h->Wvab[k] = h->Wvab[0], h->Wsgb[k] = h->Wsgb[0];
     }

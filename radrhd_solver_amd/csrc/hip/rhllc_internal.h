@@ -38,7 +38,7 @@ struct Dev {
   int32_t eos;  // Eos (rhllc_physics.h): kIdeal or kTaub
   double gamma, Gg, cfl_hmin;
   // faces
-  const int2 *face_lr;      // (left cell, right cell); right = n_cells: the fixed ghost state
+  const int2 *face_lr;      // (left, right): right n_cells / n_cells + 1 = ghost states, ~left = wall
   const double4 *face_nS;   // (nx, ny, nz, S), FP64 path
   const float3 *face_Sn32;  // S * (nx, ny, nz) in FP32, mixed path
   double4 *F4;              // flux (d, mx, my, mz) * S, FP64 path
@@ -92,10 +92,11 @@ struct Dev {
   const double *tab;
   int32_t tab_nx, tab_ny;
   double tab_min_x, tab_max_x, tab_step_x, tab_min_y, tab_max_y, tab_step_y;
-  // Fixed ghost state for boundary code -3.  The state arrays hold it as one
-  // more cell, index n_cells (written by launch_ghost_cell), which no step
-  // updates; faces of code -3 gather it as their right cell.
-  double ghostW[5], ghostU[5];
+  // Fixed ghost states for boundary codes -3 and -4.  The state arrays hold
+  // them as two more cells, indices n_cells and n_cells + 1 (written by
+  // launch_ghost_cell), which no step updates; faces of those codes gather
+  // them as their right cell.
+  double ghostW[5], ghostU[5], ghost2W[5], ghost2U[5];
   // surrogate c2p guess (RHLLC_GUESS_MLP): FP32 model and bf16 W2
   const float *mlpP;
   const short *mlpW2b;
@@ -126,7 +127,7 @@ hipError_t launch_dt(const Dev &d, hipStream_t s);
 hipError_t launch_clear_scalars(const Dev &d, hipStream_t s);
 // Mixed path: W (FP64) recovered from U by c2p (start from the FP32 pressure).
 hipError_t launch_seed_mixed(const Dev &d, hipStream_t s);
-// Writes the ghost state into entry n_cells of the input and output buffers
+// Writes the ghost states into entries n_cells and n_cells + 1 of the input and output buffers
 // of d (U and W on the FP64 path, the cell record on the mixed path).
 hipError_t launch_ghost_cell(const Dev &d, bool mixed, hipStream_t s);
 hipError_t launch_recover_W(const Dev &d, hipStream_t s);

@@ -141,19 +141,26 @@ __device__ __forceinline__ double face_flux(const Dev &d, const FaceIn<MIXED> &f
   const int2 lr = fi.lr;
   if constexpr (!MIXED) {
     const double4 nS = fi.g;
+    // lr.y: the neighbour, a ghost state (n_cells, n_cells + 1) or ~lr.x on a wall
+    const int r = lr.y >= 0 ? lr.y : ~lr.y;
     State<double> UL, WL, UR, WR;
     load_U(d, lr.x, UL);
     load_W(d, lr.x, WL);
-    load_U(d, lr.y, UR);  // lr.y = n_cells on a face of code -3: the ghost state
-    load_W(d, lr.y, WR);
+    load_U(d, r, UR);
+    load_W(d, r, WR);
+    if (__builtin_expect(lr.y < 0, 0)) {
+      wall_mirror(nS.x, nS.y, nS.z, UR.v);
+      wall_mirror(nS.x, nS.y, nS.z, WR.v);
+    }
     return hllc_flux<double, Ieee, EOS>(d.gamma, d.Gg, UL, UR, WL, WR, nS.x, nS.y, nS.z, nS.w, F);
   } else {
     // FP32 storage (cell records, geometry), FP64 arithmetic with the fast
     // divide/sqrt policy.
     const float3 Sn = fi.g;
+    const int r = lr.y >= 0 ? lr.y : ~lr.y;  // the neighbour, a ghost state, or the cell on a wall
     const MixedIn WL = mixed_in(ld(d.Wva, lr.x), ld(d.Wsg, lr.x));
-    const MixedIn WR = mixed_in(ld(d.Wva, lr.y), ld(d.Wsg, lr.y));  // n_cells on code -3: the ghost
-    return mixed_face_flux<EOS>(d.gamma, WL, WR, Sn.x, Sn.y, Sn.z, F);
+    const MixedIn WR = mixed_in(ld(d.Wva, r), ld(d.Wsg, r));
+    return mixed_face_flux<EOS>(d.gamma, WL, WR, Sn.x, Sn.y, Sn.z, F, lr.y < 0);
   }
 }
 
@@ -629,22 +636,25 @@ __global__ __launch_bounds__(kBlock) void k_seed_mixed(Dev d) {
   store_W32<EOS>(d, d.Wva, d.Wsg, c, W);
 }
 
-// The ghost state of boundary code -3 as entry n_cells of the state arrays,
-// input and output buffers alike (one thread).
+// The ghost states of boundary codes -3 and -4 as entries n_cells and
+// n_cells + 1 of the state arrays, input and output buffers alike (two threads).
 template <int EOS>
 __global__ void k_ghost_cell(Dev d, bool mixed) {
-  const int c = d.n_cells;
+  const int k = threadIdx.x;
+  if (k > 1) return;
+  const int c = d.n_cells + k;
+  const double *gw = k == 0 ? d.ghostW : d.ghost2W, *gu = k == 0 ? d.ghostU : d.ghost2U;
   State<double> W;
-  W.d = d.ghostW[0], W.v[0] = d.ghostW[1], W.v[1] = d.ghostW[2], W.v[2] = d.ghostW[3], W.p = d.ghostW[4];
+  W.d = gw[0], W.v[0] = gw[1], W.v[1] = gw[2], W.v[2] = gw[3], W.p = gw[4];
   if (mixed) {
     store_W32<EOS>(d, d.Wva, d.Wsg, c, W);
     store_W32<EOS>(d, d.Wvao, d.Wsgo, c, W);
     return;
   }
-  const double4 u = make_double4(d.ghostU[0], d.ghostU[1], d.ghostU[2], d.ghostU[3]);
+  const double4 u = make_double4(gu[0], gu[1], gu[2], gu[3]);
   const double4 w = make_double4(W.d, W.v[0], W.v[1], W.v[2]);
-  d.U4[c] = u, d.UE[c] = d.ghostU[4], d.W4[c] = w, d.Wp[c] = W.p;
-  d.U4o[c] = u, d.UEo[c] = d.ghostU[4], d.W4o[c] = w, d.Wpo[c] = W.p;
+  d.U4[c] = u, d.UE[c] = gu[4], d.W4[c] = w, d.Wp[c] = W.p;
+  d.U4o[c] = u, d.UEo[c] = gu[4], d.W4o[c] = w, d.Wpo[c] = W.p;
 }
 
 // Mixed path: FP64 primitives from the FP64 conserved state (for download).
@@ -885,9 +895,9 @@ hipError_t launch_seed_mixed(const Dev &d, hipStream_t s) {
 }
 hipError_t launch_ghost_cell(const Dev &d, bool mixed, hipStream_t s) {
   if (d.eos == kTaub)
-    hipLaunchKernelGGL(k_ghost_cell<kTaub>, dim3(1), dim3(1), 0, s, d, mixed);
+    hipLaunchKernelGGL(k_ghost_cell<kTaub>, dim3(1), dim3(64), 0, s, d, mixed);
   else
-    hipLaunchKernelGGL(k_ghost_cell<kIdeal>, dim3(1), dim3(1), 0, s, d, mixed);
+    hipLaunchKernelGGL(k_ghost_cell<kIdeal>, dim3(1), dim3(64), 0, s, d, mixed);
   return hipGetLastError();
 }
 hipError_t launch_recover_W(const Dev &d, hipStream_t s) {

@@ -22,8 +22,16 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
 
   // Validate so every device gather stays in bounds.
   RawVec<int32_t> fl(F), fr(F);
-  if (!rhllc_mesh::validate_geo(N, geo_cells, F, geo_faces, opt.has_out_source, err, fl.data(), fr.data()))
+  if (!rhllc_mesh::validate_geo(N, geo_cells, F, geo_faces, opt.has_out_source || opt.out_source_copy, err,
+                                 fl.data(), fr.data()))
     return false;
+  if (opt.reject_outer_surface)
+    for (int32_t f = 0; f < F; ++f)
+      if (fr[f] == RHLLC_BOUND_OUTER_SURFACE) {
+        err = "face " + std::to_string(f) +
+              ": e_bound_outer_surface (-5) faces abort the reference's step outside GEOMETRY_TYPE Cone";
+        return false;
+      }
 
   // Numbering on the device.  Faces are grouped by the earlier new index of
   // their two cells (a boundary face goes with its id_l): the Hilbert order
@@ -66,11 +74,20 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
     std::memcpy(&S, r + 32, 8);
     const int32_t l = cnew[fl[f]];
     nb[i] = fr[f] >= 0 ? cnew[fr[f]] : -1;
-    // Copy-type boundaries (Cone geometry: every code but -3) read the cell
-    // itself; code -3 reads the fixed ghost state, held as one more cell
-    // (index N) after the mesh's cells.
+    // Copy-type boundaries read the cell itself; codes -3 and -4 read a fixed
+    // ghost state, held as cells N and N + 1 after the mesh's cells; a wall
+    // (-2 with lock_wall) reads the cell itself, mirrored (~l < 0).
     L.lr[2 * (size_t)i] = l;
-    L.lr[2 * (size_t)i + 1] = fr[f] >= 0 ? nb[i] : (fr[f] == RHLLC_BOUND_OUT_SOURCE ? N : l);
+    int32_t right = l;
+    if (fr[f] >= 0)
+      right = nb[i];
+    else if (fr[f] == RHLLC_BOUND_OUT_SOURCE && opt.has_out_source)
+      right = N;
+    else if (fr[f] == RHLLC_BOUND_INNER_SOURCE && opt.has_inner_source)
+      right = N + 1;
+    else if (fr[f] == RHLLC_BOUND_LOCK && opt.lock_wall)
+      right = ~l;
+    L.lr[2 * (size_t)i + 1] = right;
     if (opt.mixed) {
       float *q = &L.Sn32[3 * (size_t)i];
       q[0] = (float)(n[0] * S), q[1] = (float)(n[1] * S), q[2] = (float)(n[2] * S);

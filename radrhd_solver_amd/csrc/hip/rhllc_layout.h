@@ -17,12 +17,19 @@ struct Options {
   bool reorder = true;          // Hilbert relabelling of cells (faces are grouped either way when tiled)
   bool tiled = true;            // build the tiles of the tiled step (else the face index per cell slot)
   bool has_out_source = false;  // boundary code -3 faces are allowed (fixed ghost state)
+  bool has_inner_source = false;  // code -4 takes the second fixed ghost state (else copies)
+  bool lock_wall = false;         // code -2 mirrors the normal velocity (else copies)
+  bool reject_outer_surface = false;  // code -5 faces are an error
+  bool out_source_copy = false;       // code -3 copies the cell when there is no out_source state
   int tile_cells = 256;
 };
 
 struct Layout {
   std::vector<int32_t> cell_new_to_old, face_new_to_old;
-  rhllc_mesh::RawVec<int32_t> lr;      // per face (left, right): right = cell, N ghost (-3), left (copy-type codes)
+  // per face (left, right): right = the neighbour; N and N + 1 the ghost
+  // states of codes -3 and -4; ~left (negative) a wall of code -2; left for
+  // the copy-type codes
+  rhllc_mesh::RawVec<int32_t> lr;
   rhllc_mesh::RawVec<double> nS;       // per face (nx, ny, nz, S), FP64 path
   rhllc_mesh::RawVec<float> Sn32;      // per face S (nx, ny, nz) in FP32, mixed path
   rhllc_mesh::RawVec<double> V;        // per cell, FP64 path

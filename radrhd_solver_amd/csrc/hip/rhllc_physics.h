@@ -177,6 +177,20 @@ __device__ __forceinline__ void rot_back(const Rot<R> &T, const R *v, R *r) {
   r[0] = r0, r[1] = r1, r[2] = r2;
 }
 
+// The wall of boundary code -2 outside GEOMETRY_TYPE Cone (rhllc_utils.cpp:
+// 95-110): the normal velocity mirrored, v - 2 (v.n) n, with v.n summed as
+// (v0 n0 + v1 n1) + v2 n2.  The reference writes it as T^T diag(-1, 1, 1) T v
+// with T = GetRotationMatrix(n), forming T^T by `T = T.transpose()`; Eigen
+// rejects that assignment without NDEBUG (aliasing), and with NDEBUG it
+// overwrites T's lower triangle with its upper one before the second product.
+// This is the reflection that code intends (equal to it up to rounding).
+template <typename R>
+__device__ __forceinline__ void wall_mirror(R n0, R n1, R n2, R *v) {
+  const R vn = (v[0] * n0 + v[1] * n1) + v[2] * n2;
+  const R t = vn + vn;
+  v[0] = v[0] - t * n0, v[1] = v[1] - t * n1, v[2] = v[2] - t * n2;
+}
+
 // Conserved from primitive; clamps |v| >= 1 to 0.9999 and rewrites W.v as the reference does.
 template <typename R, typename A = Ieee, int EOS = kIdeal>
 __device__ __forceinline__ void p2c(R Gg, State<R> &W, State<R> &U) {
@@ -587,9 +601,11 @@ __device__ __forceinline__ void mixed_side(double gamma, const MixedIn &w, const
 // (lL < 0 < lR), so the face needs no branch.
 // F receives S * flux with F.p the flux of E - D (not of E); returns
 // max(lL, lR) as hllc_flux does.
+// wall: the right state is the left one with the normal velocity mirrored
+// (boundary code -2 outside GEOMETRY_TYPE Cone), v - 2 (v.n) n.
 template <int EOS>
-__device__ __forceinline__ double mixed_face_flux(double gamma, const MixedIn &WL, const MixedIn &WR, float sx,
-                                                  float sy, float sz, State<double> &F) {
+__device__ __forceinline__ double mixed_face_flux(double gamma, const MixedIn &WL, const MixedIn &WR_in, float sx,
+                                                  float sy, float sz, State<double> &F, bool wall = false) {
 #pragma clang fp contract(fast)
   using A = FaceFast;
   double n[3] = {sx, sy, sz};
@@ -598,6 +614,8 @@ __device__ __forceinline__ double mixed_face_flux(double gamma, const MixedIn &W
   const double S = q * rn;
   n[0] *= rn, n[1] *= rn, n[2] *= rn;
   MixedSide L, R;
+  MixedIn WR = WR_in;
+  if (__builtin_expect(wall, 0)) wall_mirror(n[0], n[1], n[2], WR.v);
   mixed_side<EOS>(gamma, WL, n, L);
   mixed_side<EOS>(gamma, WR, n, R);
   const double lL = smin(L.cmin, R.cmin);

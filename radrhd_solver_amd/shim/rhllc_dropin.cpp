@@ -40,6 +40,17 @@ double GetPressure(double rho, double tem) {
   return (g_idealGasConst_inv / kPressure) * tem * rho;
 }
 
+// rhllc::GetConvValue (rhllc_flux.cpp:7-16) of a fixed ghost state W.
+void ghost_conv(double gamma, const double W[5], double U[5]) {
+  const double Gg = gamma / (gamma - 1.0);
+  const double Gamma = 1. / std::sqrt(1 - (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]));
+  const double h = 1 + Gg * W[4] / W[0];
+  const double dhGG = W[0] * h * Gamma * Gamma;
+  U[0] = Gamma * W[0];
+  U[1] = dhGG * W[1], U[2] = dhGG * W[2], U[3] = dhGG * W[3];
+  U[4] = dhGG - W[4];
+}
+
 // Ghost state of e_bound_out_source faces under GEOMETRY_TYPE Cone
 // (rhllc_utils.cpp:80-84): W = (rho0, 1e3 / kVelocity, 0, 0, GetPressure(rho0, 10 kEv)),
 // U from W by rhllc::GetConvValue (rhllc_flux.cpp:7-16), which is not the
@@ -48,13 +59,45 @@ void cone_out_source(double gamma, double W[5], double U[5]) {
   W[0] = kM_hydrogen * 1e14 / kDensity;
   W[1] = 1e3 / kVelocity, W[2] = 0.0, W[3] = 0.0;
   W[4] = GetPressure(W[0], 10 * kEv);
-  const double Gg = gamma / (gamma - 1.0);
-  const double Gamma = 1. / std::sqrt(1 - (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]));
-  const double h = 1 + Gg * W[4] / W[0];
-  const double dhGG = W[0] * h * Gamma * Gamma;
-  U[0] = Gamma * W[0];
-  U[1] = dhGG * W[1], U[2] = dhGG * W[2], U[3] = dhGG * W[3];
-  U[4] = dhGG - W[4];
+  ghost_conv(gamma, W, U);
+}
+
+// The boundary branches of BoundConditions for the configured GEOMETRY_TYPE
+// (rhllc_utils.cpp:45-133; the reference fixes it at compile time,
+// include/solvers_config.h:29, default Cone).  Cone: -3 is the fixed inflow
+// state, every other code copies the cell.  Every other type: -2 is a wall
+// and -5 aborts; Sphere and Cone_JET give -4 a fixed state, Cylinder gives -3
+// one; the source codes copy the cell otherwise.
+void geometry_bounds(const std::string &g, double gamma, bool has_code3, rhllc_params &p) {
+  static const char *kTypes[] = {"cube", "step", "cone", "cone_jet", "sphere", "cylinder", "test_ellipse",
+                                 "main_ellipse"};
+  bool known = false;
+  for (const char *t : kTypes) known = known || g == t;
+  if (!known) die("hllc_cfg.json: unknown \"geometry\"", g);
+  if (g == "cone") {
+    if (has_code3) {
+      p.has_out_source = 1;
+      cone_out_source(gamma, p.out_source_W, p.out_source_U);
+    }
+    return;
+  }
+  p.lock_wall = 1;
+  p.reject_outer_surface = 1;
+  if (g == "sphere" || g == "cone_jet") {
+    const double Ws[5] = {0.1, 0.0, 0.0, 0.0, 1.0}, Wj[5] = {0.1, 0.99, 0.0, 0.0, 0.01};
+    const double *W = g == "sphere" ? Ws : Wj;
+    p.has_inner_source = 1;
+    for (int k = 0; k < 5; ++k) p.inner_source_W[k] = W[k];
+    ghost_conv(gamma, p.inner_source_W, p.inner_source_U);
+  }
+  if (g == "cylinder") {
+    const double W[5] = {0.1, 0.99, 0.0, 0.0, 0.01};
+    p.has_out_source = 1;
+    for (int k = 0; k < 5; ++k) p.out_source_W[k] = W[k];
+    ghost_conv(gamma, p.out_source_W, p.out_source_U);
+  } else {
+    p.out_source_copy = 1;
+  }
 }
 
 struct DeviceGrid {
@@ -103,11 +146,9 @@ void ensure_device(grid_t &grid) {
   p.c2p_guess = guess_code(_hllc_cfg.c2p_start);
   p.reorder = 1;
   p.device = 0;
-  for (size_t i = 0; i < F && !p.has_out_source; ++i)
-    if (grid.faces[i].geo.id_r == RHLLC_BOUND_OUT_SOURCE) {
-      p.has_out_source = 1;
-      cone_out_source(_hllc_cfg.gamma, p.out_source_W, p.out_source_U);
-    }
+  bool has_code3 = false;
+  for (size_t i = 0; i < F && !has_code3; ++i) has_code3 = grid.faces[i].geo.id_r == RHLLC_BOUND_OUT_SOURCE;
+  geometry_bounds(_hllc_cfg.geometry, _hllc_cfg.gamma, has_code3, p);
   if (rhllc_create(&p, (int64_t)N, gc.data(), (int64_t)F, gf.data(), &g_dev.h) != RHLLC_OK)
     die("rhllc_create", rhllc_last_error());
   if (p.c2p_guess == RHLLC_GUESS_MLP) {
@@ -406,6 +447,7 @@ int Read(const std::string &file, hllc_value_t &st) {
   opt(kv, "precision", st.precision);
   opt(kv, "c2p_start", st.c2p_start);
   opt(kv, "mlp_model", st.mlp_model);
+  opt(kv, "geometry", st.geometry);
   return ok ? e_completion_success : e_completion_fail;
 }
 

@@ -108,6 +108,7 @@ struct hllc_value_t {
   std::string precision = "fp64";
   std::string c2p_start = "reference";
   std::string mlp_model;
+  std::string geometry = "cone";  // the reference's GEOMETRY_TYPE (include/solvers_config.h:19-29)
 };
 
 // lib/json/include/json_struct.h:39-55

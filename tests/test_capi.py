@@ -40,6 +40,8 @@ def _create(mesh, **kw):
     p = capi.Params()
     p.gamma, p.cfl, p.h_min = kw.get("gamma", 5 / 3), kw.get("cfl", 0.4), kw.get("h_min", 1e-3)
     p.precision, p.c2p_guess = kw.get("precision", 0), kw.get("guess", 0)
+    p.reject_outer_surface = kw.get("reject_outer_surface", 0)
+    p.out_source_copy = kw.get("out_source_copy", 0)
     h = C.c_void_p()
     lib = _lib.load_hip_lib()
     rc = lib.rhllc_create(C.byref(p), mesh.n_cells, mesh.cells.ctypes.data, mesh.n_faces, mesh.faces.ctypes.data,
@@ -69,6 +71,20 @@ def test_create_rejects_inconsistent_meshes_before_touching_the_device():
     bad.cells["V"][4] = -1.0
     rc, msg = _create(bad)
     assert rc == 1 and "V <= 0" in msg
+
+
+def test_create_boundary_options():
+    """Code -5 faces are an error with reject_outer_surface (GEOMETRY_TYPE other than Cone);
+    code -3 faces need the out_source state unless out_source_copy is set."""
+    m = M.kuhn_box(2, 2, 2, seed=1)
+    f = int(np.nonzero(M.face_id_r(m.faces) == -1)[0][0])
+    bad = M.Mesh(m.cells.copy(), m.faces.copy())
+    bad.faces["w1"][f] = np.uint32((-5) & 0x3FFFFFF)
+    rc, msg = _create(bad, reject_outer_surface=1)
+    assert rc == 1 and "outer_surface" in msg
+    bad.faces["w1"][f] = np.uint32((-3) & 0x3FFFFFF)
+    rc, msg = _create(bad, out_source_copy=1)
+    assert rc != 1, msg  # passes validation (no device here: the create stops at the device)
 
 
 def test_create_rejects_bad_parameters():

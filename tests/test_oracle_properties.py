@@ -349,3 +349,86 @@ def test_taub_c2p_round_trip():
     assert (codes == 0).all()
     np.testing.assert_allclose(W2[:, 4], W[:, 4], rtol=1e-7)
     np.testing.assert_allclose(W2[:, 0], W[:, 0], rtol=1e-9)
+
+
+def _mark_boundary(m, axis, side, code):
+    """Set the boundary code of the faces on one side of the box (axis 0..2, side -1 / +1)."""
+    r = M.face_id_r(m.faces)
+    sel = (r < 0) & (np.sign(m.faces["n"][:, axis]) == side) & (np.abs(m.faces["n"][:, axis]) > 0.9)
+    m.faces["w1"][sel] = (m.faces["w1"][sel] & np.uint32(0xFC000000)) | np.uint32(code & 0x3FFFFFF)
+    return int(sel.sum())
+
+
+def test_wall_mirror_is_the_reflection():
+    """The e_bound_lock wall: v - 2 (v.n) n, i.e. the normal component negated and the
+    tangential ones kept -- the T^T diag(-1, 1, 1) T v the reference writes, to rounding."""
+    rng = np.random.default_rng(SEED + 20)
+    for _ in range(200):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        v = rng.normal(size=3) * 0.3
+        r = O.wall_mirror(n, v)
+        assert abs(r @ n + v @ n) <= 1e-15
+        assert np.abs((r - (r @ n) * n) - (v - (v @ n) * n)).max() <= 1e-15
+        T = O.rotation(n)
+        want = T.T @ (np.diag([-1.0, 1.0, 1.0]) @ (T @ v))  # rounds worse where n is near +-z
+        assert np.abs(r - want).max() <= 1e-12
+    for n in ((0.0, 0.0, 1.0), (0.0, 0.0, -1.0)):  # GetRotationMatrix's special frames
+        r = O.wall_mirror(np.array(n), np.array([0.1, -0.2, 0.3]))
+        assert np.allclose(r, [0.1, -0.2, -0.3], rtol=0, atol=1e-17)
+
+
+def test_walled_box_keeps_gas_at_rest_and_conserves_mass_and_energy():
+    """A box closed by walls (every boundary face e_bound_lock with the wall branch on):
+    gas at rest stays at rest, and a blast inside moves no mass or energy through the walls
+    (the mirrored Riemann problem has u* = 0)."""
+    m = M.kuhn_box(8, 8, 8, seed=SEED)
+    for axis in range(3):
+        for side in (-1, 1):
+            assert _mark_boundary(m, axis, side, -2) > 0
+    W0 = M.init_state(m, M.INIT_UNIFORM, params=[1.0, 0.0, 0.0, 0.0, 1.0])
+    st = O.State(m, W0)
+    st.ph.lock_wall = 1
+    st.run(0.4, m.h_min(), 20)
+    assert np.abs(st.W[:, 1:4]).max() < 1e-13
+    assert np.abs(st.W[:, 4] - 1.0).max() < 1e-13
+    W0 = M.init_state(m, M.INIT_BLAST, params=[0.5, 0.5, 0.5, 0.25, 1.0, 100.0, 1.0])
+    st = O.State(m, W0)
+    st.ph.lock_wall = 1
+    V = m.cells["V"]
+    st.run(0.4, m.h_min(), 0)
+    tot0 = (st.U * V[:, None]).sum(0)
+    st.run(0.4, m.h_min(), 60)
+    tot1 = (st.U * V[:, None]).sum(0)
+    assert np.abs(st.W[:, 1:4]).max() > 1e-2  # the blast reached the walls
+    scale = (np.abs(st.U) * V[:, None]).sum(0)
+    assert abs(tot1[0] - tot0[0]) / scale[0] < 1e-13 and abs(tot1[4] - tot0[4]) / scale[4] < 1e-13
+    # the same box without the wall branch (GEOMETRY_TYPE Cone: the wall copies the cell) leaks
+    st = O.State(m, W0)
+    st.run(0.4, m.h_min(), 0)
+    st.run(0.4, m.h_min(), 60)
+    tot2 = (st.U * V[:, None]).sum(0)
+    assert abs(tot2[4] - tot0[4]) / scale[4] > 1e-6
+
+
+def test_inner_source_faces_take_the_fixed_state():
+    """e_bound_inner_source (-4) with a fixed state: the face flux is the HLLC flux against it."""
+    m = M.kuhn_box(4, 4, 4, seed=SEED)
+    assert _mark_boundary(m, 0, -1, -4) > 0
+    W0 = M.init_state(m, M.INIT_UNIFORM, params=[1.0, 0.0, 0.0, 0.0, 1.0])
+    Wg = np.array([0.1, 0.5, 0.0, 0.0, 0.01])
+    _, Ug = O.p2c(Wg.reshape(1, 5))
+    st = O.State(m, W0)
+    st.ph.has_inner_source = 1
+    for k in range(5):
+        st.ph.inner_source_W[k], st.ph.inner_source_U[k] = Wg[k], Ug[0, k]
+    st.run(0.4, m.h_min(), 0)
+    st.step(0.4 * m.h_min())
+    F = st.face_flux
+    r = M.face_id_r(m.faces)
+    l = M.face_id_l(m.faces)
+    src = np.nonzero(r == -4)[0]
+    _, U0 = O.p2c(W0[l[src]])
+    Fo, _ = O.flux(U0, np.repeat(Ug, src.size, 0), W0[l[src]], np.repeat(Wg.reshape(1, 5), src.size, 0),
+                   np.column_stack([m.faces["n"][src], m.faces["S"][src]]))
+    assert np.array_equal(F[src], Fo)
