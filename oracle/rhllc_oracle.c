@@ -15,6 +15,9 @@
  *   orc_c2p            GetPhysValueStab   -- pinned: compiled reference + long-double bracketed root
  *   orc_pressure_fix   PhysPressureFix    -- pinned: compiled reference vectors
  *   orc_rotation       GetRotationMatrix  -- pinned: compiled reference + orthonormality/det
+ *   orc_wall_mirror    the e_bound_lock wall of the non-Cone builds -- pinned: closed form
+ *                                            (the reflection), a walled box conserving mass and
+ *                                            energy; not against the reference, whose wall aborts
  *   orc_flux           GetFluxStab        -- pinned: compiled reference vectors (all 4 branches)
  *                                            + consistency F(U,U) and mirror symmetry
  *   orc_step / orc_run Hllc3dStab + loop  -- pinned: compiled reference trajectories (200 steps)
