@@ -238,6 +238,10 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
     std::string err;
     if (!rhllc_layout::build(opt, n_cells, geo_cells, n_faces, geo_faces, L, err)) return fail(RHLLC_ERR_ARG, err);
   }
+  // The kernels address the face arrays with 32-bit byte offsets (32-byte
+  // FP64 entries).
+  if (n_faces + (int64_t)L.tile_extra.size() > ((int64_t)1 << 27))
+    return fail(RHLLC_ERR_ARG, "faces plus the tiles' extra faces exceed 2^27; use RHLLC_SCHED_TWO_KERNEL");
   const int32_t N = (int32_t)n_cells, F = (int32_t)n_faces;
 
   std::unique_ptr<rhllc_solver> h(new rhllc_solver());
@@ -272,46 +276,32 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
     if (e == hipSuccess && count) e = hipMemcpy(*dst, src, sizeof(**dst) * count, hipMemcpyHostToDevice);
     return e;
   };
+  // The face arrays, followed on the tiled step by the extra faces' copies.
+  const size_t nfe = (size_t)F + L.tile_extra.size();
   int2 *lr_d;
-  HIP_TRY(upload(&lr_d, L.lr.data(), F));
+  HIP_TRY(upload(&lr_d, L.lr.data(), nfe));
   d.face_lr = lr_d;
   if (!mixed) {
     double *V_d;
     double4 *nS_d;
     HIP_TRY(upload(&V_d, L.V.data(), N));
-    HIP_TRY(upload(&nS_d, L.nS.data(), F));
+    HIP_TRY(upload(&nS_d, L.nS.data(), nfe));
     d.cell_V = V_d;
     d.face_nS = nS_d;
   } else {
     float4 *vd_d;
     float3 *sn_d;
     HIP_TRY(upload(&vd_d, L.Vd.data(), N));
-    HIP_TRY(upload(&sn_d, L.Sn32.data(), F));
+    HIP_TRY(upload(&sn_d, L.Sn32.data(), nfe));
     d.cell_Vd = vd_d;
     d.face_Sn32 = sn_d;
   }
   if (tiled) {
     int4 *tm;
-    int32_t *te;
     uint2 *cs;
     HIP_TRY(upload(&tm, L.tile_meta.data(), L.n_tiles));
-    HIP_TRY(upload(&te, L.tile_extra.data(), L.tile_extra.size()));
     HIP_TRY(upload(&cs, L.cell_slots.data(), N));
-    const size_t ne = L.tile_extra.size();
-    int2 *elr;
-    HIP_TRY(upload(&elr, L.extra_lr.data(), ne));
-    d.extra_lr = elr;
-    if (mixed) {
-      float3 *esn;
-      HIP_TRY(upload(&esn, L.extra_Sn32.data(), ne));
-      d.extra_Sn32 = esn;
-    } else {
-      double4 *ens;
-      HIP_TRY(upload(&ens, L.extra_nS.data(), ne));
-      d.extra_nS = ens;
-    }
     d.tile_meta = tm;
-    d.tile_extra = te;
     d.cell_slots = cs;
     h->n_tiles = L.n_tiles;
     h->tile_slots = L.max_slots > 0 ? L.max_slots : 1;

@@ -74,11 +74,10 @@ struct Dev {
   // bits).  tile_meta = (first own face, own count, first extra, extra count);
   // cell_slots = four 16-bit tile-local face slots in slot order, bit 15 set
   // when the cell is the face's id_r (the flux is subtracted).
+  // face_lr and face_nS / face_Sn32 hold n_faces + (extra faces of all tiles)
+  // entries: extra face j of tile t is entry n_faces + first extra + j, a copy
+  // of the face's own entry, so a tile reads its extra faces contiguously.
   const int4 *tile_meta;
-  const int32_t *tile_extra;
-  const int2 *extra_lr;        // face_lr of each extra face, in tile_extra's order
-  const double4 *extra_nS;     // face_nS of each extra face (FP64 path)
-  const float3 *extra_Sn32;    // face_Sn32 of each extra face (mixed path)
   const uint2 *cell_slots;
   // radiation source (optional): (Fx, Fy, Fz, Ur) and (kappa_a, kappa_s)
   const float4 *rad_FU;

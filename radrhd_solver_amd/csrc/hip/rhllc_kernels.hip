@@ -121,17 +121,6 @@ __device__ __forceinline__ FaceIn<MIXED> face_in(const Dev &d, int i) {
     o.g = ld(d.face_nS, i);
   return o;
 }
-// The same for extra face k of the tiled step, from the copies in tile order.
-template <bool MIXED>
-__device__ __forceinline__ FaceIn<MIXED> extra_in(const Dev &d, int k) {
-  FaceIn<MIXED> o;
-  o.lr = ld(d.extra_lr, k);
-  if constexpr (MIXED)
-    o.g = ld(d.extra_Sn32, k);
-  else
-    o.g = ld(d.extra_nS, k);
-  return o;
-}
 
 // One face: gather of the two cells' state at the start of the step, HLLC
 // flux into F (S * flux in the global frame; on the mixed path F.p is the
@@ -537,7 +526,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
   double *C = reinterpret_cast<double *>(lds_flux + 2 * nf);
   double speed = 0.0;
   for (int k = threadIdx.x; k < nf; k += kBlock) {
-    const FaceIn<MIXED> fi = k < tm.y ? face_in<MIXED>(d, tm.x + k) : extra_in<MIXED>(d, tm.z + (k - tm.y));
+    // One index into one set of arrays (the extra faces' copies follow the
+    // faces): a select between two sets of base pointers would hold them in
+    // VGPRs through the loop.
+    const FaceIn<MIXED> fi = face_in<MIXED>(d, k < tm.y ? tm.x + k : d.n_faces + tm.z + (k - tm.y));
     State<double> F;
     speed = smax(speed, face_flux<MIXED, EOS>(d, fi, F));
     A[k] = make_double2(F.d, F.v[0]);

@@ -218,19 +218,20 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
       L.cell_slots[2 * (size_t)c + 1] = s[2] | (s[3] << 16);
     }
   }
-  L.extra_lr.resize(2 * (size_t)ne);
+  // The extra faces' copies, entry k at F + k.
+  L.lr.resize(2 * ((size_t)F + ne));
   if (opt.mixed)
-    L.extra_Sn32.resize(3 * (size_t)ne);
+    L.Sn32.resize(3 * ((size_t)F + ne));
   else
-    L.extra_nS.resize(4 * (size_t)ne);
+    L.nS.resize(4 * ((size_t)F + ne));
 #pragma omp parallel for schedule(static)
   for (int64_t k = 0; k < ne; ++k) {
-    const size_t i = (size_t)L.tile_extra[k];
-    L.extra_lr[2 * k] = L.lr[2 * i], L.extra_lr[2 * k + 1] = L.lr[2 * i + 1];
+    const size_t i = (size_t)L.tile_extra[k], j = (size_t)F + k;
+    L.lr[2 * j] = L.lr[2 * i], L.lr[2 * j + 1] = L.lr[2 * i + 1];
     if (opt.mixed)
-      for (int a = 0; a < 3; ++a) L.extra_Sn32[3 * k + a] = L.Sn32[3 * i + a];
+      for (int a = 0; a < 3; ++a) L.Sn32[3 * j + a] = L.Sn32[3 * i + a];
     else
-      for (int a = 0; a < 4; ++a) L.extra_nS[4 * k + a] = L.nS[4 * i + a];
+      for (int a = 0; a < 4; ++a) L.nS[4 * j + a] = L.nS[4 * i + a];
   }
   return true;
 }

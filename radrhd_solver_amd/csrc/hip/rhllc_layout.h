@@ -38,12 +38,11 @@ struct Layout {
   // tiled step
   int32_t n_tiles = 0, max_slots = 0;
   std::vector<int32_t> tile_meta;    // per tile (first own face, own count, first extra, extra count)
-  std::vector<int32_t> tile_extra;   // extra faces of all tiles, tile by tile, by face index
-  // The cells and geometry of each extra face, in tile_extra's order (copies
-  // of lr and nS / Sn32), so a tile reads them contiguously.
-  rhllc_mesh::RawVec<int32_t> extra_lr;
-  rhllc_mesh::RawVec<double> extra_nS;
-  rhllc_mesh::RawVec<float> extra_Sn32;
+  // extra faces of all tiles, tile by tile, by face index.  lr and nS / Sn32
+  // continue past the F faces with a copy for each entry, in this order
+  // (entry k at F + k), so a tile reads its extra faces contiguously and
+  // through the same arrays as its own.
+  std::vector<int32_t> tile_extra;
   rhllc_mesh::RawVec<uint32_t> cell_slots;  // per cell two words: four 16-bit slots, bit 15 = id_r
 };
 

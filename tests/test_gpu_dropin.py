@@ -15,9 +15,10 @@ pytestmark = pytest.mark.gpu
 SEED = 20261202
 
 
-def _oracle_loop(m, W0, cfl, h, T):
+def _oracle_loop(m, W0, cfl, h, T, lock_wall=False):
     """rhllc_main.cpp:34-56: while t < T {step(tau); t += tau; tau = CFL h / speed}."""
     st = O.State(m, W0)
+    st.ph.lock_wall = int(lock_wall)
     st.run(cfl, h, 0)
     t, tau, n = 0.0, cfl * h / 1.0, 0
     while t < T:
@@ -50,6 +51,27 @@ def test_rhllc_solver_executable_matches_oracle(tmp_path):
     assert np.array_equal(rho, st.W[:, 0]) and np.array_equal(p, st.W[:, 4]) and np.array_equal(v, st.W[:, 1:4])
     rho0 = _read(dirs["solve_address"] + "0density.bin", 1)
     assert np.array_equal(rho0, W0[:, 0])
+
+
+def test_rhllc_solver_geometry_cube_walls(tmp_path):
+    """"geometry": "cube" in hllc_cfg.json: e_bound_lock (-2) faces are walls, as the
+    reference's BoundConditions has them for every GEOMETRY_TYPE but Cone."""
+    m = M.kuhn_box(24, 3, 3, length=(1.0, 0.125, 0.125), seed=SEED)
+    r_code = M.face_id_r(m.faces)
+    sel = (r_code < 0) & (np.abs(m.faces["n"][:, 0]) > 0.9)  # the two ends of the tube
+    m.faces["w1"][sel] = (m.faces["w1"][sel] & np.uint32(0xFC000000)) | np.uint32((-2) & 0x3FFFFFF)
+    W0 = M.init_state(m, M.INIT_SHOCK_TUBE)
+    h = m.h_min()
+    T = 0.5  # the waves of the tube have reached both walls and reflected
+    dirs = M.write_case(str(tmp_path), m, W0, T=T, CFL=0.4, save_timer=1e9, eos="ideal", geometry="cube")
+    st, nsteps = _oracle_loop(m, W0, 0.4, h, T, lock_wall=True)
+    r = subprocess.run([SOLVER_EXE, str(tmp_path / "directories_cfg.json")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    base = dirs["solve_address"] + "1"
+    rho, p, v = _read(base + "density.bin", 1), _read(base + "pressure.bin", 1), _read(base + "velocity.bin", 3)
+    assert np.array_equal(rho, st.W[:, 0]) and np.array_equal(p, st.W[:, 4]) and np.array_equal(v, st.W[:, 1:4])
+    cone, _ = _oracle_loop(m, W0, 0.4, h, T, lock_wall=False)
+    assert np.abs(cone.W[:, 0] - rho).max() > 1e-3  # the wall made a difference
 
 
 def test_rhllc_solver_mpi_one_rank_writes_flux_bin(tmp_path):
