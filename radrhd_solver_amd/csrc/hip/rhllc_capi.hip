@@ -289,12 +289,12 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
     d.cell_V = V_d;
     d.face_nS = nS_d;
   } else {
-    float4 *vd_d;
-    float3 *sn_d;
-    HIP_TRY(upload(&vd_d, L.Vd.data(), N));
-    HIP_TRY(upload(&sn_d, L.Sn32.data(), nfe));
-    d.cell_Vd = vd_d;
-    d.face_Sn32 = sn_d;
+    double *vi_d;
+    double3 *sn_d;
+    HIP_TRY(upload(&vi_d, L.Vinv.data(), N));
+    HIP_TRY(upload(&sn_d, L.Sn.data(), nfe));
+    d.cell_Vinv = vi_d;
+    d.face_Sn = sn_d;
   }
   if (tiled) {
     int4 *tm;

@@ -40,7 +40,7 @@ struct Dev {
   // faces
   const int2 *face_lr;      // (left, right): right n_cells / n_cells + 1 = ghost states, ~left = wall
   const double4 *face_nS;   // (nx, ny, nz, S), FP64 path
-  const float3 *face_Sn32;  // S * (nx, ny, nz) in FP32, mixed path
+  const double3 *face_Sn;   // S * (nx, ny, nz), mixed path
   double4 *F4;              // flux (d, mx, my, mz) * S, FP64 path
   double *FE;               // flux E * S
   float4 *F4f;              // mixed path: flux * S, FP32 parts (flux_ext.h)
@@ -48,7 +48,7 @@ struct Dev {
   // cells
   const int4 *cell_faces;  // face index | 0x80000000 when the cell is id_r (sign bit 0)
   const double *cell_V;    // FP64 path
-  const float4 *cell_Vd;   // mixed path: (1/V, closure defect sum_j sign_j face_Sn32_j)
+  const double *cell_Vinv;  // mixed path: 1/V
   // State at the start of the step (read by the face work and by each cell's
   // own update) and where the cell update writes it.  The two-kernel step
   // updates in place (out == in); the tiled step gathers neighbours' state
@@ -74,7 +74,7 @@ struct Dev {
   // bits).  tile_meta = (first own face, own count, first extra, extra count);
   // cell_slots = four 16-bit tile-local face slots in slot order, bit 15 set
   // when the cell is the face's id_r (the flux is subtracted).
-  // face_lr and face_nS / face_Sn32 hold n_faces + (extra faces of all tiles)
+  // face_lr and face_nS / face_Sn hold n_faces + (extra faces of all tiles)
   // entries: extra face j of tile t is entry n_faces + first extra + j, a copy
   // of the face's own entry, so a tile reads its extra faces contiguously.
   const int4 *tile_meta;

@@ -105,18 +105,18 @@ __device__ __forceinline__ void fold_speed(const Dev &d, double speed) {
 }
 
 // What a face reads before its cells: the two cell indices and the geometry
-// (FP64 path: n and S; mixed path: the FP32 vector S n).
+// (FP64 path: n and S; mixed path: the vector S n).
 template <bool MIXED>
 struct FaceIn {
   int2 lr;
-  typename std::conditional<MIXED, float3, double4>::type g;
+  typename std::conditional<MIXED, double3, double4>::type g;
 };
 template <bool MIXED>
 __device__ __forceinline__ FaceIn<MIXED> face_in(const Dev &d, int i) {
   FaceIn<MIXED> o;
   o.lr = ld(d.face_lr, i);
   if constexpr (MIXED)
-    o.g = ld(d.face_Sn32, i);
+    o.g = ld(d.face_Sn, i);
   else
     o.g = ld(d.face_nS, i);
   return o;
@@ -143,9 +143,9 @@ __device__ __forceinline__ double face_flux(const Dev &d, const FaceIn<MIXED> &f
     }
     return hllc_flux<double, Ieee, EOS>(d.gamma, d.Gg, UL, UR, WL, WR, nS.x, nS.y, nS.z, nS.w, F);
   } else {
-    // FP32 storage (cell records, geometry), FP64 arithmetic with the fast
+    // FP32 cell records, FP64 geometry and arithmetic with the fast
     // divide/sqrt policy.
-    const float3 Sn = fi.g;
+    const double3 Sn = fi.g;
     const int r = lr.y >= 0 ? lr.y : ~lr.y;  // the neighbour, a ghost state, or the cell on a wall
     const MixedIn WL = mixed_in(ld(d.Wva, lr.x), ld(d.Wsg, lr.x));
     const MixedIn WR = mixed_in(ld(d.Wva, r), ld(d.Wsg, r));
@@ -280,16 +280,9 @@ struct TileFlux {
 // Gather of the four face fluxes in slot order with their signs
 // (rhllc_3d_stab.cpp:30-45) and U -= tau/V * sum.  Also returns the change of
 // tau = E - D (for the surrogate gate) and, on the mixed path, the cell's
-// FP32 pressure (the c2p start).
-//
-// Mixed path: the FP32 face vectors S n of a cell do not sum to zero, so a
-// uniform state would see the net flux F(U) . delta, delta = sum_j sign_j S n_j
-// (~6e-8 S), where the FP64 geometry closes to rounding.  In a gas near rest
-// that flux, p delta, outweighed the physical momentum change (relative L1 3e-5
-// against FP64).  Subtracting the cell's own flux through delta,
-// (D v.delta, m v.delta + p delta, m.delta) with v.delta = m.delta / (E + p),
-// restores that balance; with exact geometry delta = 0 and the update is
-// unchanged.
+// FP32 pressure (the c2p start).  Both paths add each face's flux to its two
+// cells with opposite signs, so the totals change only by the boundary fluxes
+// (to rounding).
 template <bool MIXED, typename Src>
 __device__ __forceinline__ void update_U(const Dev &d, int c, double tau, const Src &src, State<double> &U,
                                          double &dtau, float &p32) {
@@ -309,17 +302,8 @@ __device__ __forceinline__ void update_U(const Dev &d, int c, double tau, const 
   load_U(d, c, U);
   double k;
   if (MIXED) {
-    const float4 vd = ld(d.cell_Vd, c);
     p32 = ld(d.Wva, c).w;
-    const double dl[3] = {vd.y, vd.z, vd.w}, p = p32;
-    const double mdl = dot3(U.v, dl);
-    const double vdl = mdl * Fast::rcp(U.p + p);
-    s.d -= U.d * vdl;
-    s.v[0] -= U.v[0] * vdl + p * dl[0];
-    s.v[1] -= U.v[1] * vdl + p * dl[1];
-    s.v[2] -= U.v[2] * vdl + p * dl[2];
-    s.p -= mdl;
-    k = tau * (double)vd.x;  // 1/V rounded to FP32 once, on the host
+    k = tau * ld(d.cell_Vinv, c);  // V k = tau to rounding: the totals of V U telescope
   } else {
     k = tau / ld(d.cell_V, c);
   }
@@ -942,8 +926,8 @@ __global__ void k_eval_flux(double gamma, int64_t n, const double *UL, const dou
                                             nS4[4 * i + 3], f);
     st5(F + 5 * i, f);
   } else {
-    // The mixed face path: FP32 primitives and geometry, FP64 fast arithmetic,
-    // flux rounded to FP32.
+    // The mixed face path: FP32 primitives, FP64 geometry and fast arithmetic,
+    // flux in the two-kernel step's packed record.
     // The cell records the mixed cell kernel would store for these states.
     State<double> ff;
     float4 la, ra;
@@ -951,8 +935,8 @@ __global__ void k_eval_flux(double gamma, int64_t n, const double *UL, const dou
     mixed_record<EOS>(Gg, wl, la, ls);
     mixed_record<EOS>(Gg, wr, ra, rs);
     const double *g = nS4 + 4 * i;
-    speed[i] = mixed_face_flux<EOS>(gamma, mixed_in(la, ls), mixed_in(ra, rs), (float)(g[0] * g[3]),
-                                    (float)(g[1] * g[3]), (float)(g[2] * g[3]), ff);
+    speed[i] = mixed_face_flux<EOS>(gamma, mixed_in(la, ls), mixed_in(ra, rs), g[0] * g[3], g[1] * g[3],
+                                    g[2] * g[3], ff);
     const rhllc_ext::Packed pk = rhllc_ext::pack(ff.d, ff.v[0], ff.v[1], ff.v[2], ff.p);
     rhllc_ext::unpack(pk.a, pk.b, F + 5 * i);
   }

@@ -61,7 +61,7 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
   // Faces: cells, geometry.
   L.lr.resize(2 * (size_t)F);
   if (opt.mixed)
-    L.Sn32.resize(3 * (size_t)F);
+    L.Sn.resize(3 * (size_t)F);
   else
     L.nS.resize(4 * (size_t)F);
   RawVec<int32_t> nb(F);  // the neighbour across the face (device numbering), -1 on the boundary
@@ -89,8 +89,8 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
       right = ~l;
     L.lr[2 * (size_t)i + 1] = right;
     if (opt.mixed) {
-      float *q = &L.Sn32[3 * (size_t)i];
-      q[0] = (float)(n[0] * S), q[1] = (float)(n[1] * S), q[2] = (float)(n[2] * S);
+      double *q = &L.Sn[3 * (size_t)i];
+      q[0] = n[0] * S, q[1] = n[1] * S, q[2] = n[2] * S;
     } else {
       double *q = &L.nS[4 * (size_t)i];
       q[0] = n[0], q[1] = n[1], q[2] = n[2], q[3] = S;
@@ -100,7 +100,7 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
   // Cells: face slots (device face index, bit 31 when the cell is id_r) and V.
   RawVec<int32_t> cf(4 * (size_t)N);
   if (opt.mixed)
-    L.Vd.resize(4 * (size_t)N);
+    L.Vinv.resize(N);
   else
     L.V.resize(N);
 #pragma omp parallel for schedule(static)
@@ -111,22 +111,10 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
     for (int j = 0; j < 4; ++j)
       cf[4 * (size_t)i + j] = fnew[ids[j]] | (((r[48] >> j) & 1) ? 0 : (int32_t)0x80000000u);
     const double V = cell_V(r);
-    if (!opt.mixed) {
+    if (!opt.mixed)
       L.V[i] = V;
-    } else {
-      // delta = sum over the cell's faces of +-S n as the face kernel sees it
-      // (FP32), summed in FP64; the cell update subtracts the cell's own flux
-      // through it.
-      double dl[3] = {0.0, 0.0, 0.0};
-      for (int j = 0; j < 4; ++j) {
-        const int32_t e = cf[4 * (size_t)i + j];
-        const float *q = &L.Sn32[3 * (size_t)(e & 0x7FFFFFFF)];
-        const double sg = e >= 0 ? 1.0 : -1.0;
-        dl[0] += sg * (double)q[0], dl[1] += sg * (double)q[1], dl[2] += sg * (double)q[2];
-      }
-      float *o = &L.Vd[4 * (size_t)i];
-      o[0] = (float)(1.0 / V), o[1] = (float)dl[0], o[2] = (float)dl[1], o[3] = (float)dl[2];
-    }
+    else
+      L.Vinv[i] = 1.0 / V;
   }
   if (!opt.tiled) {
     L.cf.assign(cf.begin(), cf.end());
@@ -221,7 +209,7 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
   // The extra faces' copies, entry k at F + k.
   L.lr.resize(2 * ((size_t)F + ne));
   if (opt.mixed)
-    L.Sn32.resize(3 * ((size_t)F + ne));
+    L.Sn.resize(3 * ((size_t)F + ne));
   else
     L.nS.resize(4 * ((size_t)F + ne));
 #pragma omp parallel for schedule(static)
@@ -229,7 +217,7 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
     const size_t i = (size_t)L.tile_extra[k], j = (size_t)F + k;
     L.lr[2 * j] = L.lr[2 * i], L.lr[2 * j + 1] = L.lr[2 * i + 1];
     if (opt.mixed)
-      for (int a = 0; a < 3; ++a) L.Sn32[3 * j + a] = L.Sn32[3 * i + a];
+      for (int a = 0; a < 3; ++a) L.Sn[3 * j + a] = L.Sn[3 * i + a];
     else
       for (int a = 0; a < 4; ++a) L.nS[4 * j + a] = L.nS[4 * i + a];
   }

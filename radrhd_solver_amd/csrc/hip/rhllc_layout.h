@@ -13,7 +13,7 @@
 namespace rhllc_layout {
 
 struct Options {
-  bool mixed = false;           // FP32 face vectors and (V, closure defect) instead of FP64 (n, S) and V
+  bool mixed = false;           // face vectors S n and 1/V instead of (n, S) and V
   bool reorder = true;          // Hilbert relabelling of cells (faces are grouped either way when tiled)
   bool tiled = true;            // build the tiles of the tiled step (else the face index per cell slot)
   bool has_out_source = false;  // boundary code -3 faces are allowed (fixed ghost state)
@@ -31,14 +31,14 @@ struct Layout {
   // the copy-type codes
   rhllc_mesh::RawVec<int32_t> lr;
   rhllc_mesh::RawVec<double> nS;       // per face (nx, ny, nz, S), FP64 path
-  rhllc_mesh::RawVec<float> Sn32;      // per face S (nx, ny, nz) in FP32, mixed path
+  rhllc_mesh::RawVec<double> Sn;       // per face S (nx, ny, nz) in FP64, mixed path
   rhllc_mesh::RawVec<double> V;        // per cell, FP64 path
-  rhllc_mesh::RawVec<float> Vd;        // per cell (1/V, delta) in FP32, mixed path
+  rhllc_mesh::RawVec<double> Vinv;     // per cell 1/V, mixed path
   rhllc_mesh::RawVec<int32_t> cf;      // per cell 4 face indices | bit 31 when the cell is id_r (two-kernel step)
   // tiled step
   int32_t n_tiles = 0, max_slots = 0;
   std::vector<int32_t> tile_meta;    // per tile (first own face, own count, first extra, extra count)
-  // extra faces of all tiles, tile by tile, by face index.  lr and nS / Sn32
+  // extra faces of all tiles, tile by tile, by face index.  lr and nS / Sn
   // continue past the F faces with a copy for each entry, in this order
   // (entry k at F + k), so a tile reads its extra faces contiguously and
   // through the same arrays as its own.

@@ -492,7 +492,7 @@ __device__ __forceinline__ R hllc_flux(R G1, R Gg, const State<R> &cUL, const St
 }
 
 // ---------------------------------------------------------------- mixed path
-// Face flux of the mixed path: FP32 primitives and geometry in, FP64
+// Face flux of the mixed path: FP32 primitives and FP64 geometry in, FP64
 // arithmetic with the Fast policy and FMA contraction.  Same HLLC as
 // hllc_flux (same signal speeds, same quadratic for the contact speed, same
 // side selection), written without the rotation matrix: in the face frame
@@ -594,9 +594,10 @@ __device__ __forceinline__ void mixed_side(double gamma, const MixedIn &w, const
   o.cmin = (a - sroot) * r;
 }
 
-// The face geometry is the FP32 vector S n; the unit normal and the area are
-// recovered in FP64 (n exactly unit to rounding, which the frame-free form
-// relies on: m = m_n n + m_t holds only for |n| = 1).  The contact solve is
+// The face geometry is the vector S n in FP64, so a cell's four vectors close
+// to rounding as the FP64 path's do; the unit normal and the area are
+// recovered from it (n unit to rounding, which the frame-free form relies on:
+// m = m_n n + m_t holds only for |n| = 1).  The contact solve is
 // evaluated for every face and its result taken only in the subsonic case
 // (lL < 0 < lR), so the face needs no branch.
 // F receives S * flux with F.p the flux of E - D (not of E); returns
@@ -604,8 +605,8 @@ __device__ __forceinline__ void mixed_side(double gamma, const MixedIn &w, const
 // wall: the right state is the left one with the normal velocity mirrored
 // (boundary code -2 outside GEOMETRY_TYPE Cone), v - 2 (v.n) n.
 template <int EOS>
-__device__ __forceinline__ double mixed_face_flux(double gamma, const MixedIn &WL, const MixedIn &WR_in, float sx,
-                                                  float sy, float sz, State<double> &F, bool wall = false) {
+__device__ __forceinline__ double mixed_face_flux(double gamma, const MixedIn &WL, const MixedIn &WR_in, double sx,
+                                                  double sy, double sz, State<double> &F, bool wall = false) {
 #pragma clang fp contract(fast)
   using A = FaceFast;
   double n[3] = {sx, sy, sz};

@@ -125,5 +125,4 @@ def test_walled_box_conserves_mass_and_energy(precision):
     scale = (np.abs(U) * V[:, None]).sum(0)
     drift = np.abs(tot1 - tot0) / scale
     print("precision", precision, "relative drift of the totals", drift, flush=True)
-    bar = 1e-13 if precision == capi.PREC_FP64 else 1e-9
-    assert drift[0] <= bar and drift[4] <= bar
+    assert drift[0] <= 1e-13 and drift[4] <= 1e-13

@@ -11,13 +11,10 @@ matched step for step at smaller sizes in test_gpu_step.py):
   graph-captured, 100 steps.
 
 Properties: mass and energy conservation (on the periodic box for every field;
-on the blast's box while its outer cells are still untouched).  The FP64 step
-sums each face flux into its two cells with opposite signs, so the totals move
-only by rounding (bar 1e-13 relative).  The mixed step also subtracts each
-cell's own flux through the closure defect of its FP32 face vectors, a per-cell
-term that keeps a uniform state at rest but does not telescope, so its totals
-move at the FP32 geometry's rounding times the flow's variation (bar 1e-9).
-With the radiation source (config 4) the momenta and energy take the source,
+on the blast's box while its outer cells are still untouched).  Both steps
+add each face flux to its two cells with opposite signs and scale by tau/V
+with V the FP64 volume, so the totals move only by rounding (bar 1e-13
+relative).  With the radiation source (config 4) the momenta and energy take the source,
 so only the mass balances: one step changes the total of V*D by tau times the
 D flux through the boundary faces (their outer state is the cell's own).
 Also: the mixed step against the FP64 step (per-field relative L1 <= 1e-5), no
@@ -92,10 +89,10 @@ def test_config1_blast_6M():
     outer = np.abs(x - 0.5).max(axis=1) > 0.45
     assert np.abs(W64[outer, 1:4]).max() <= 1e-10
     assert np.abs(W64[outer, 4] / W0[outer, 4] - 1).max() <= 1e-10
-    for label, U, bar in (("fp64", U64, 1e-13), ("mixed", U32, 1e-9)):
+    for label, U in (("fp64", U64), ("mixed", U32)):
         drift = _drift(U0, U, V)
         print(label, "relative drift of (D, mx, my, mz, E) totals", drift, flush=True)
-        assert drift[0] <= bar and drift[4] <= bar
+        assert drift[0] <= 1e-13 and drift[4] <= 1e-13
     l1 = _rel_l1(U32, U64, V)
     print("mixed vs fp64 rel L1", l1, flush=True)
     assert max(l1) <= 1e-5
@@ -110,10 +107,10 @@ def test_config2_shear_12M():
     _clean(st64, "fp64")
     _, _, U32, st32 = _run(m, W0, 4.0 / 3.0, 200, capi.PREC_MIXED, capi.GUESS_PREVIOUS)
     _clean(st32, "mixed")
-    for label, U, bar in (("fp64", U64, 1e-13), ("mixed", U32, 1e-9)):
+    for label, U in (("fp64", U64), ("mixed", U32)):
         drift = _drift(U0, U, V)
         print(label, "relative drift of (D, mx, my, mz, E) totals", drift, flush=True)
-        assert drift.max() <= bar  # every face is interior: each flux enters two cells with opposite signs
+        assert drift.max() <= 1e-13  # every face is interior: each flux enters two cells with opposite signs
     l1 = _rel_l1(U32, U64, V)
     print("mixed vs fp64 rel L1", l1, flush=True)
     assert max(l1) <= 1e-5
@@ -225,7 +222,7 @@ def test_config4_full_form_24M():
             out[label] = s.download()[1]
             bal, dm = _mass_balance(s, m, V, tau)
             print(label, "mass change of one step", dm, "imbalance vs boundary flux", bal, flush=True)
-            assert bal <= (1e-13 if prec == capi.PREC_FP64 else 1e-9)
+            assert bal <= 1e-13
         finally:
             s.close()
     l1 = _rel_l1(out["mixed+mlp"], out["fp64"], V)
