@@ -290,7 +290,7 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
     d.face_nS = nS_d;
   } else {
     double *vi_d;
-    double3 *sn_d;
+    float4 *sn_d;
     HIP_TRY(upload(&vi_d, L.Vinv.data(), N));
     HIP_TRY(upload(&sn_d, L.Sn.data(), nfe));
     d.cell_Vinv = vi_d;

@@ -104,12 +104,26 @@ __device__ __forceinline__ void fold_speed(const Dev &d, double speed) {
   }
 }
 
+// The mixed path's face vector S n from its 16-byte form: the FP32 vector
+// plus k Q per component, k the signed 10-bit fields of the fourth word and
+// Q = 2^(E - 33), 2^E the power of two at or below the largest |component|.
+// k Q is exact and so is the sum (at most 34 significant bits), so this is the
+// vector the host encoded, to 2^-33 of its largest component.
+__device__ __forceinline__ void unpack_sn(const float4 &g, double s[3]) {
+  const float m = fmaxf(fmaxf(fabsf(g.x), fabsf(g.y)), fabsf(g.z));
+  const double Q = (double)__int_as_float(__float_as_int(m) & 0x7F800000) * 0x1p-33;
+  const int w = __float_as_int(g.w);
+  s[0] = fma((double)((w << 22) >> 22), Q, (double)g.x);
+  s[1] = fma((double)((w << 12) >> 22), Q, (double)g.y);
+  s[2] = fma((double)((w << 2) >> 22), Q, (double)g.z);
+}
+
 // What a face reads before its cells: the two cell indices and the geometry
-// (FP64 path: n and S; mixed path: the vector S n).
+// (FP64 path: n and S; mixed path: the vector S n in its 16-byte form).
 template <bool MIXED>
 struct FaceIn {
   int2 lr;
-  typename std::conditional<MIXED, double3, double4>::type g;
+  typename std::conditional<MIXED, float4, double4>::type g;
 };
 template <bool MIXED>
 __device__ __forceinline__ FaceIn<MIXED> face_in(const Dev &d, int i) {
@@ -145,11 +159,12 @@ __device__ __forceinline__ double face_flux(const Dev &d, const FaceIn<MIXED> &f
   } else {
     // FP32 cell records, FP64 geometry and arithmetic with the fast
     // divide/sqrt policy.
-    const double3 Sn = fi.g;
+    double Sn[3];
+    unpack_sn(fi.g, Sn);
     const int r = lr.y >= 0 ? lr.y : ~lr.y;  // the neighbour, a ghost state, or the cell on a wall
     const MixedIn WL = mixed_in(ld(d.Wva, lr.x), ld(d.Wsg, lr.x));
     const MixedIn WR = mixed_in(ld(d.Wva, r), ld(d.Wsg, r));
-    return mixed_face_flux<EOS>(d.gamma, WL, WR, Sn.x, Sn.y, Sn.z, F, lr.y < 0);
+    return mixed_face_flux<EOS>(d.gamma, WL, WR, Sn[0], Sn[1], Sn[2], F, lr.y < 0);
   }
 }
 

@@ -2,6 +2,7 @@
 #include "rhllc_layout.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "../mesh/mesh_common.h"
@@ -11,6 +12,30 @@ namespace rhllc_layout {
 using rhllc_mesh::cell_ids;
 using rhllc_mesh::cell_V;
 using rhllc_mesh::RawVec;
+
+namespace {
+// S n into the 16-byte form described with Layout::Sn.
+void pack_sn(const double e[3], float *o) {
+  uint32_t w = 0;
+  float h[3];
+  for (int k = 0; k < 3; ++k) h[k] = (float)e[k];
+  const float m = std::max(std::max(std::fabs(h[0]), std::fabs(h[1])), std::fabs(h[2]));
+  uint32_t mb;
+  std::memcpy(&mb, &m, 4);
+  mb &= 0x7F800000u;  // 2^E (0 for a zero or subnormal vector: no correction)
+  float pw;
+  std::memcpy(&pw, &mb, 4);
+  const double Q = (double)pw * 0x1p-33;
+  if (Q > 0.0)
+    for (int k = 0; k < 3; ++k) {
+      // |e - h| <= ulp(h) / 2 <= 2^(E - 24) = 512 Q; 512 itself is clamped
+      const long n = std::min(511L, std::max(-511L, std::lround((e[k] - (double)h[k]) / Q)));
+      w |= ((uint32_t)n & 0x3FFu) << (10 * k);
+    }
+  o[0] = h[0], o[1] = h[1], o[2] = h[2];
+  std::memcpy(&o[3], &w, 4);
+}
+}  // namespace
 
 bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n_faces, const void *geo_faces,
            Layout &L, std::string &err) {
@@ -61,7 +86,7 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
   // Faces: cells, geometry.
   L.lr.resize(2 * (size_t)F);
   if (opt.mixed)
-    L.Sn.resize(3 * (size_t)F);
+    L.Sn.resize(4 * (size_t)F);
   else
     L.nS.resize(4 * (size_t)F);
   RawVec<int32_t> nb(F);  // the neighbour across the face (device numbering), -1 on the boundary
@@ -89,8 +114,8 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
       right = ~l;
     L.lr[2 * (size_t)i + 1] = right;
     if (opt.mixed) {
-      double *q = &L.Sn[3 * (size_t)i];
-      q[0] = n[0] * S, q[1] = n[1] * S, q[2] = n[2] * S;
+      const double e[3] = {n[0] * S, n[1] * S, n[2] * S};
+      pack_sn(e, &L.Sn[4 * (size_t)i]);
     } else {
       double *q = &L.nS[4 * (size_t)i];
       q[0] = n[0], q[1] = n[1], q[2] = n[2], q[3] = S;
@@ -209,7 +234,7 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
   // The extra faces' copies, entry k at F + k.
   L.lr.resize(2 * ((size_t)F + ne));
   if (opt.mixed)
-    L.Sn.resize(3 * ((size_t)F + ne));
+    L.Sn.resize(4 * ((size_t)F + ne));
   else
     L.nS.resize(4 * ((size_t)F + ne));
 #pragma omp parallel for schedule(static)
@@ -217,7 +242,7 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
     const size_t i = (size_t)L.tile_extra[k], j = (size_t)F + k;
     L.lr[2 * j] = L.lr[2 * i], L.lr[2 * j + 1] = L.lr[2 * i + 1];
     if (opt.mixed)
-      for (int a = 0; a < 3; ++a) L.Sn[3 * j + a] = L.Sn[3 * i + a];
+      std::memcpy(&L.Sn[4 * j], &L.Sn[4 * i], 16);  // the fourth word is bits, copied as such
     else
       for (int a = 0; a < 4; ++a) L.nS[4 * j + a] = L.nS[4 * i + a];
   }
