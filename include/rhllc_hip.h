@@ -43,9 +43,10 @@ extern "C" {
 /* Arithmetic of the step. */
 #define RHLLC_PREC_FP64 0  /* all FP64, bitwise equal to the reference compiled strict */
 #define RHLLC_PREC_MIXED 1 /* FP32 storage of the primitives v and p (per-cell records, which
-                              also hold s - D and D in FP64) and of the face geometry; the
-                              two-kernel step also stores its face flux in FP32 with small
-                              extensions; FP64 U and FP64 arithmetic */
+                              also hold s - D and D in FP64); face vectors S n in 16 bytes
+                              (FP32 plus 10-bit residuals, exact to 2^-33); the two-kernel
+                              step also stores its face flux in FP32 with small extensions;
+                              FP64 U, 1/V and arithmetic; conserves to rounding */
 
 /* Equation of state: the reference's compile-time EOS (include/hd_config.h:16-20). */
 #define RHLLC_EOS_IDEAL 0 /* h = 1 + gamma/(gamma-1) p/rho, runtime gamma (EOS IDEAL) */
