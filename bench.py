@@ -301,7 +301,7 @@ def main():
         "data": "synthetic: seeded jittered/permuted Kuhn-tet mesh, blast-wave state, seeded radiation field",
         "config": {
             "workload": f"full-form blast wave, {n}^3 hexes x 6 = {N} tets, {F} faces, radiation source, "
-            "mixed precision (FP32 cell records and face geometry, FP64 U, fluxes and arithmetic), surrogate c2p "
+            "mixed precision (FP32 cell records, face vectors FP32 plus residuals, FP64 U, fluxes and arithmetic), surrogate c2p "
             "start, tiled step (faces into LDS, cells from LDS), hipGraph-replayed",
             "n_cells": N,
             "n_faces": F,
