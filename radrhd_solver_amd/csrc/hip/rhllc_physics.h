@@ -52,6 +52,12 @@ template <typename R>
 __device__ __forceinline__ R dot3(const R *a, const R *b) {
   return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
 }
+// The same with fused multiply-adds, for the mixed path: a function's
+// `fp contract(fast)` does not reach into dot3, which the FP64 path needs
+// unfused (three products and two sums, against one product and two FMAs).
+__device__ __forceinline__ double dot3c(const double *a, const double *b) {
+  return fma(a[2], b[2], fma(a[1], b[1], a[0] * b[0]));
+}
 
 // Arithmetic policies.  Ieee: correctly rounded divide and square root (the
 // parity path; every expression keeps the reference's form).  Fast: hardware
@@ -110,18 +116,20 @@ struct FaceFast {
     return fma(r, fma(-x, r, 1.0), r);
   }
   __device__ static __forceinline__ double div(double a, double b) { return a * rcp(b); }
-  __device__ static __forceinline__ double sqrt_(double x) {  // x = 0 returns 0, as Fast does
-    double y = __builtin_amdgcn_rsq(x);
-    y = y * fma(-0.5 * x * y, y, 1.5);
-    const double s = x * y;
-    return x == 0.0 ? x : s;
+  // x = 0 returns 0, as Fast does, without a branch: the root is taken of
+  // x + 1e-300 (equal to x from x = 1e-284 on) and multiplied by x.
+  __device__ static __forceinline__ double sqrt_(double x) {
+    const double xs = x + 1e-300;
+    double y = __builtin_amdgcn_rsq(xs);
+    y = y * fma(-0.5 * xs * y, y, 1.5);
+    return x * y;
   }
   __device__ static __forceinline__ double rsqrt(double x) { return Fast::rsqrt(x); }
   // The hardware approximations alone (within 5.2e-8 and 4.6e-8 relative),
   // for the signal speeds: the HLLC flux depends on them smoothly, and two
   // equal states give the physical flux whatever they are.
-  __device__ static __forceinline__ double sqrt_rough(double x) {  // x = 0 returns 0
-    return x == 0.0 ? x : x * __builtin_amdgcn_rsq(x);
+  __device__ static __forceinline__ double sqrt_rough(double x) {  // x = 0 returns 0, as sqrt_
+    return x * __builtin_amdgcn_rsq(x + 1e-300);
   }
   __device__ static __forceinline__ double rcp_rough(double x) { return __builtin_amdgcn_rcp(x); }
 };
@@ -287,7 +295,7 @@ __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<doub
   const double eps2 = 1.e-12;
   const double D = U.d;
   const double E = U.p;
-  const double m2 = dot3(U.v, U.v);
+  const double m2 = A::kFast ? dot3c(U.v, U.v) : dot3(U.v, U.v);
   iters = 0;
   double m, pmin;
   if (!A::kFast) {
@@ -525,7 +533,7 @@ __device__ __forceinline__ void mixed_record(double Gg, const State<double> &W, 
   va = make_float4((float)W.v[0], (float)W.v[1], (float)W.v[2], (float)W.p);
   const double rho = (float)W.d, p = va.w;
   const double v[3] = {va.x, va.y, va.z};
-  const double vv = dot3(v, v);
+  const double vv = dot3c(v, v);
   const double lor = A::rsqrt(1.0 - vv);  // one reciprocal root: 4.2e-15
   const double lor2 = lor * lor;
   double rhm1;  // rho (h - 1)
@@ -554,17 +562,19 @@ struct MixedSide {
   double p, v[3];           // primitives (v after the clamp)
   double D, E, s;           // conserved D, E and s (m = s v)
   double sigma;             // s - D
-  double vn, mn;            // v.n, m.n
+  double vn, mn;            // v.n, m.n (n the unit normal)
   double cmin, cmax;        // signal speeds
 };
 
+// sv = S n, rS = 1/S.
 template <int EOS>
-__device__ __forceinline__ void mixed_side(double gamma, const MixedIn &w, const double *n, MixedSide &o) {
+__device__ __forceinline__ void mixed_side(double gamma, const MixedIn &w, const double *sv, double rS,
+                                           MixedSide &o) {
 #pragma clang fp contract(fast)
   using A = FaceFast;
   o.p = w.p;
   o.v[0] = w.v[0], o.v[1] = w.v[1], o.v[2] = w.v[2];
-  double V2 = dot3(o.v, o.v);
+  double V2 = dot3c(o.v, o.v);
   if (__builtin_expect(V2 >= 1.0, 0)) {
     // GetConvValueStab's clamp, for a velocity that rounded to |v| >= 1 in
     // FP32 (Lorentz factors above ~3000); s and D keep their FP64 values.
@@ -576,7 +586,7 @@ __device__ __forceinline__ void mixed_side(double gamma, const MixedIn &w, const
   o.D = w.D;
   o.sigma = w.sigma;
   o.E = w.D + (w.sigma - w.p);  // E - D = sigma - p, formed from the two small terms
-  o.vn = dot3(o.v, n);
+  o.vn = dot3c(o.v, sv) * rS;
   o.mn = o.s * o.vn;
   double cs2;  // sound speed^2, to the hardware reciprocal's 5e-8 (it enters only the signal speeds)
   if (EOS == kIdeal) {
@@ -594,10 +604,10 @@ __device__ __forceinline__ void mixed_side(double gamma, const MixedIn &w, const
   o.cmin = (a - sroot) * r;
 }
 
-// The face geometry is the vector S n in FP64, so a cell's four vectors close
-// to rounding as the FP64 path's do; the unit normal and the area are
-// recovered from it (n unit to rounding, which the frame-free form relies on:
-// m = m_n n + m_t holds only for |n| = 1).  The contact solve is
+// The face geometry is the vector S n (sx, sy, sz), exact to 2^-33 of its size,
+// so a cell's four vectors close to that; the area S and v.n = v.(S n)/S are
+// recovered from it (the frame-free form relies on |n| = 1: m = m_n n + m_t),
+// and the flux's normal part is scaled by S n directly.  The contact solve is
 // evaluated for every face and its result taken only in the subsonic case
 // (lL < 0 < lR), so the face needs no branch.
 // F receives S * flux with F.p the flux of E - D (not of E); returns
@@ -609,16 +619,16 @@ __device__ __forceinline__ double mixed_face_flux(double gamma, const MixedIn &W
                                                   double sy, double sz, State<double> &F, bool wall = false) {
 #pragma clang fp contract(fast)
   using A = FaceFast;
-  double n[3] = {sx, sy, sz};
-  const double q = dot3(n, n);
-  const double rn = q > 0.0 ? A::rsqrt(q) : 0.0;  // a zero-area face gets n = 0 and a zero flux
-  const double S = q * rn;
-  n[0] *= rn, n[1] *= rn, n[2] *= rn;
+  const double sv[3] = {sx, sy, sz};
+  const double q = dot3c(sv, sv);
+  // 1/S; a zero-area face (q = 0) takes a finite rS and S = 0, so a zero flux
+  const double rS = A::rsqrt(q + 1e-300);
+  const double S = q * rS;
   MixedSide L, R;
   MixedIn WR = WR_in;
-  if (__builtin_expect(wall, 0)) wall_mirror(n[0], n[1], n[2], WR.v);
-  mixed_side<EOS>(gamma, WL, n, L);
-  mixed_side<EOS>(gamma, WR, n, R);
+  if (__builtin_expect(wall, 0)) wall_mirror(sx * rS, sy * rS, sz * rS, WR.v);
+  mixed_side<EOS>(gamma, WL, sv, rS, L);
+  mixed_side<EOS>(gamma, WR, sv, rS, R);
   const double lL = smin(L.cmin, R.cmin);
   const double lR = smax(L.cmax, R.cmax);
   // contact solve (rhllc_flux_stab.cpp:362-421)
@@ -644,12 +654,13 @@ __device__ __forceinline__ double mixed_face_flux(double gamma, const MixedIn &W
   const double ms = (lam * (E + ps) - mn) * us * dif;  // star normal momentum
   const double ct = sup ? vn : vn + lam * ((lam - vn) * dif - 1.0);
   const double Fn = sup ? mn * vn + p : mn * vn + lam * (ms - mn) + p;
-  const double cm = ct * s;         // c_t m = c_t s v
-  const double cn = Fn - ct * mn;   // normal remainder (the pressure-like part)
-  F.d = D * ct * S;
-  F.v[0] = (cm * v0 + cn * n[0]) * S;
-  F.v[1] = (cm * v1 + cn * n[1]) * S;
-  F.v[2] = (cm * v2 + cn * n[2]) * S;
+  const double ctS = ct * S;
+  const double cmS = ctS * s;       // S c_t m = S c_t s v
+  const double cn = Fn - ct * mn;   // normal remainder (the pressure-like part), times S n below
+  F.d = D * ctS;
+  F.v[0] = cmS * v0 + cn * sx;
+  F.v[1] = cmS * v1 + cn * sy;
+  F.v[2] = cmS * v2 + cn * sz;
   // F_E - F_D from the small terms only: sigma v_n outside the fan, and in the
   // star region m*_n - D* u* = u* (lam (sigma - p + p*) - sigma v_n) / (lam - u*)
   // (D* = D (lam - v_n) / (lam - u*)); forming F_E and F_D and subtracting
@@ -764,10 +775,10 @@ __device__ __forceinline__ void rad_source_tail(const State<double> &W, double U
   const double tot_op = abs_op + scat_op;
   const double rho = W.d;
   const double u[3] = {gamma * W.v[0], gamma * W.v[1], gamma * W.v[2]};
-  const double uF = dot3(u, Fr);
+  const double uF = A::kFast ? dot3c(u, Fr) : dot3(u, Fr);
   if (A::kFast && !Tr9) {
 #pragma clang fp contract(fast)
-    const double fm2 = dot3(Fr, Fr);
+    const double fm2 = dot3c(Fr, Fr);
     double cf = 0.0, dg = 0.33333333333333;
     if (!(fm2 < 1e-150 || Ur < 1e-150)) {
       const double ifm2 = A::rcp(fm2);
@@ -826,7 +837,7 @@ template <typename A = Ieee>
 __device__ __forceinline__ void rad_source(const State<double> &W, double Ur, const double *Fr, double abs_op,
                                            double scat_op, double *G, bool emission = false,
                                            const float *Tr9 = nullptr) {
-  double gamma = dot3(W.v, W.v);
+  double gamma = A::kFast ? dot3c(W.v, W.v) : dot3(W.v, W.v);
   double u2 = gamma;
   double gamma2;
   if (!A::kFast) {
