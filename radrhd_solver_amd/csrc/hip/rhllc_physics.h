@@ -58,6 +58,15 @@ __device__ __forceinline__ R dot3(const R *a, const R *b) {
 __device__ __forceinline__ double dot3c(const double *a, const double *b) {
   return fma(a[2], b[2], fma(a[1], b[1], a[0] * b[0]));
 }
+// a b + c for code shared by both policies: fused with A = Fast, the product
+// and the sum rounded separately with A = Ieee (the reference's expression).
+template <typename A>
+__device__ __forceinline__ double mad(double a, double b, double c) {
+  if constexpr (A::kFast)
+    return fma(a, b, c);
+  else
+    return a * b + c;
+}
 
 // Arithmetic policies.  Ieee: correctly rounded divide and square root (the
 // parity path; every expression keeps the reference's form).  Fast: hardware
@@ -105,10 +114,11 @@ struct Fast {
   }
 };
 
-// The mixed face kernel's policy: one Newton step on the hardware reciprocal
-// and reciprocal root, no correction of quotients and roots.  Relative errors
-// stay below 4.2e-15 (the one-step figures above), far below the 2^-31 to
-// which the face flux is stored; c2p keeps Fast.
+// The mixed face work's policy (also the M1 closure of the mixed radiation
+// source): one Newton step on the hardware reciprocal and reciprocal root, no
+// correction of quotients and roots.  Relative errors stay below 4.2e-15 (the
+// one-step figures above), far below the FP32 rounding of the cell records
+// the flux is formed from; c2p keeps Fast.
 struct FaceFast {
   static constexpr bool kFast = true;
   __device__ static __forceinline__ double rcp(double x) {
@@ -304,6 +314,7 @@ __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<doub
     m = A::sqrt_(m2);
     pmin = A::sqrt_(A::div(m2, 1.0 - eps2)) - E;
   } else {
+#pragma clang fp contract(fast)
     // E < sqrt(m^2 + D^2) without the root; sqrt(m^2/(1 - eps2)) = m / sqrt(1 - eps2).
     if (!(E >= 0.0 && E * E >= m2 + D * D)) return 1;
     m = A::sqrt_(m2);
@@ -332,7 +343,7 @@ __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<doub
       // One refined reciprocal root y = 1/sqrt(alpha^2 - m^2) gives
       // Gamma = alpha y (clamped to >= 1: rounding takes it to 1 - ulp for gas
       // at rest) and m^2 Gamma^3 / alpha^3 = m^2 y^3.
-      const double q = alpha2 - m2;
+      const double q = fma(alpha, alpha, -m2);
       if (!(q > 0.0)) {
         iters = iter + 1;
         return 5;
@@ -342,20 +353,32 @@ __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<doub
       lor2 = lor * lor;
       mlor3 = m2 * (y * y) * y;
     }
-    tau = lor * D_1;
-    theta = p * tau;
-    if (EOS == kIdeal) {
-      h = 1.0 + Gg * theta;
-      dh_dp = Gg * tau;
-      dh_dtau = Gg * p;
+    if (A::kFast && EOS == kIdeal) {
+      // The same f and f' for the ideal gas without 1/D: D tau = Gamma, so
+      // D h = D + g with g = Gg p Gamma, D Gamma dh_dp = Gg Gamma^2 and
+      // Gamma dh_dtau = g.
+      const double g = Gg * p * lor;
+      const double Dh = D + g;
+      yp = fma(Dh, lor, -E) - p;
+      dyp = fma(-mlor3, Dh + g, fma(Gg * lor, lor, -1.0));
     } else {
-      h = 2.5 * theta + A::sqrt_(2.25 * theta * theta + 1.0);
-      const double scrh = A::div(5.0 * h - 8.0 * theta, 2.0 * h - 5.0 * theta);
-      dh_dp = tau * scrh;
-      dh_dtau = p * scrh;
+      tau = lor * D_1;
+      theta = p * tau;
+      if (EOS == kIdeal) {
+        h = mad<A>(Gg, theta, 1.0);  // 1 + Gg theta
+        dh_dp = Gg * tau;
+        dh_dtau = Gg * p;
+      } else {
+        h = 2.5 * theta + A::sqrt_(2.25 * theta * theta + 1.0);
+        const double scrh = A::div(5.0 * h - 8.0 * theta, 2.0 * h - 5.0 * theta);
+        dh_dp = tau * scrh;
+        dh_dtau = p * scrh;
+      }
+      // yp = D h lor - E - p, dyp = D lor dh_dp - mlor3 (lor dh_dtau + D h) - 1
+      const double Dh = D * h;
+      yp = mad<A>(Dh, lor, -E) - p;
+      dyp = mad<A>(-mlor3, mad<A>(lor, dh_dtau, Dh), D * lor * dh_dp) - 1.0;
     }
-    yp = D * h * lor - E - p;
-    dyp = D * lor * dh_dp - mlor3 * (lor * dh_dtau + D * h) - 1.0;
     // Fast: the quotient to one Newton step of the reciprocal (4e-15): the
     // iteration's fixed point is yp = 0 whatever the accuracy of dyp.
     dp = A::kFast ? yp * FaceFast::rcp(dyp) : A::div(yp, dyp);
@@ -379,6 +402,7 @@ __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<doub
     const double den = U.p + p;
     W.v[0] = A::div(U.v[0], den), W.v[1] = A::div(U.v[1], den), W.v[2] = A::div(U.v[2], den);
   } else {
+#pragma clang fp contract(fast)
     // rho = D / Gamma = D sqrt(alpha^2 - m^2) / alpha, v = m / alpha.
     alpha = E + p;
     const double ia = A::rcp(alpha);
@@ -529,7 +553,7 @@ struct MixedIn {
 template <int EOS = kIdeal>
 __device__ __forceinline__ void mixed_record(double Gg, const State<double> &W, float4 &va, double2 &sg) {
 #pragma clang fp contract(fast)
-  using A = Fast;
+  using A = FaceFast;  // 4.2e-15: sigma and D match the FP32 v and p far below their rounding
   va = make_float4((float)W.v[0], (float)W.v[1], (float)W.v[2], (float)W.p);
   const double rho = (float)W.d, p = va.w;
   const double v[3] = {va.x, va.y, va.z};
@@ -778,16 +802,17 @@ __device__ __forceinline__ void rad_source_tail(const State<double> &W, double U
   const double uF = A::kFast ? dot3c(u, Fr) : dot3(u, Fr);
   if (A::kFast && !Tr9) {
 #pragma clang fp contract(fast)
+    using B = FaceFast;  // one Newton step on each hardware estimate (4.2e-15)
     const double fm2 = dot3c(Fr, Fr);
     double cf = 0.0, dg = 0.33333333333333;
     if (!(fm2 < 1e-150 || Ur < 1e-150)) {
-      const double ifm2 = A::rcp(fm2);
+      const double ifm2 = B::rcp(fm2);
       const double U2 = Ur * Ur;
       if (fm2 > U2) {
         cf = ifm2, dg = 0.0;
       } else {
-        const double f2 = fm2 * A::rcp(U2);
-        const double chi_2 = A::div(1.5 + 2.0 * f2, 5.0 + 2.0 * A::sqrt_(4.0 - 3.0 * f2));
+        const double f2 = fm2 * B::rcp(U2);
+        const double chi_2 = B::div(1.5 + 2.0 * f2, 5.0 + 2.0 * B::sqrt_(4.0 - 3.0 * f2));
         cf = (3.0 * chi_2 - 0.5) * ifm2, dg = 0.5 - chi_2;
       }
     }
