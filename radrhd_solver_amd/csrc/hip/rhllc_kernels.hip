@@ -85,22 +85,28 @@ extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max
 namespace rhllc_dev {
 
 // Per-block max of non-negative speeds into one of kSpeedSlots slots.
-// Non-negative doubles order like their bit patterns, so the maximum is an
-// exact unsigned 64-bit maximum: the wave's is the DPP reduction of the ROCm
-// device library (every lane calls this), the block's goes through LDS.
+// Non-negative doubles order like their bit patterns (a NaN above all), so the
+// maximum is an exact unsigned 64-bit maximum.  Every thread leaves its speed
+// in LDS; after the barrier the first wave alone takes the maximum of four
+// each and the DPP reduction of the ROCm device library, so the other waves
+// skip the cross-lane steps.
 __device__ __forceinline__ void fold_speed(const Dev &d, double speed) {
-  __shared__ unsigned long long part[kBlock / 64];
-  const unsigned long long w = __ockl_wfred_max_u64((unsigned long long)__double_as_longlong(speed));
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) part[wid] = w;
+  __shared__ unsigned long long part[kBlock];
+  part[threadIdx.x] = (unsigned long long)__double_as_longlong(speed);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long m = part[0];
+  if (threadIdx.x < 64) {
+    unsigned long long m = part[threadIdx.x];
 #pragma unroll
-    for (int k = 1; k < kBlock / 64; ++k) m = m < part[k] ? part[k] : m;
-    unsigned long long *slot =
-        reinterpret_cast<unsigned long long *>(d.speed_slots + (size_t)(blockIdx.x % kSpeedSlots) * kSlotStride);
-    atomicMax(slot, m);
+    for (int k = 1; k < kBlock / 64; ++k) {
+      const unsigned long long v = part[threadIdx.x + 64 * k];
+      m = m < v ? v : m;
+    }
+    m = __ockl_wfred_max_u64(m);
+    if (threadIdx.x == 0) {
+      unsigned long long *slot =
+          reinterpret_cast<unsigned long long *>(d.speed_slots + (size_t)(blockIdx.x % kSpeedSlots) * kSlotStride);
+      atomicMax(slot, m);
+    }
   }
 }
 
@@ -271,6 +277,7 @@ constexpr double kMlpGate = 0.05;
 // the packed record's unpack gives it.
 template <bool MIXED>
 struct GlobalFlux {
+  static constexpr bool kTau = false;  // F.p is the E flux
   const Dev &d;
   int4 cf;
   __device__ __forceinline__ bool neg(int j) const { return (&cf.x)[j] < 0; }
@@ -280,6 +287,9 @@ struct GlobalFlux {
 };
 template <bool MIXED>
 struct TileFlux {
+  // Mixed path: F.p is the flux of tau = E - D; the cell sums it apart and
+  // adds the D sum once (E flux = D flux + tau flux).
+  static constexpr bool kTau = MIXED;
   const double2 *A, *B;
   const double *C;
   uint32_t sv[4];
@@ -288,7 +298,7 @@ struct TileFlux {
     const uint32_t k = sv[j] & 0x7FFFu;
     const double2 a = A[k], b = B[k];
     F.d = a.x, F.v[0] = a.y, F.v[1] = b.x, F.v[2] = b.y;
-    F.p = MIXED ? a.x + C[k] : C[k];
+    F.p = C[k];
   }
 };
 
@@ -314,6 +324,7 @@ __device__ __forceinline__ void update_U(const Dev &d, int c, double tau, const 
     s.d = fma(sg, F.d, s.d), s.v[0] = fma(sg, F.v[0], s.v[0]), s.v[1] = fma(sg, F.v[1], s.v[1]);
     s.v[2] = fma(sg, F.v[2], s.v[2]), s.p = fma(sg, F.p, s.p);
   }
+  if constexpr (Src::kTau) s.p += s.d;
   load_U(d, c, U);
   double k;
   if (MIXED) {

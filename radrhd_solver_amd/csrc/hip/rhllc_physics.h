@@ -287,12 +287,13 @@ __device__ __forceinline__ int pressure_fix(double Gg, State<double> &U, State<d
 // Returns 0 or the reference's codes 1..4; 5 = the reference's DIE_IF(lor2 < 1).
 // On failure W is left unchanged.  *iters = Newton iterations done.
 // A = Ieee is the reference's arithmetic, operation for operation.  The mixed
-// path (A = Fast) runs the same iteration with the Fast divide/sqrt and two
-// changes:
-//  * per iteration one reciprocal r = 1/(alpha (alpha^2 - m^2)) gives both
-//    Gamma^2 = alpha^3 r (clamped to >= 1: rounding takes it to 1 - ulp for
-//    gas at rest) and m^2 Gamma^3/alpha^3 = m^2 Gamma r; the reference's
-//    lor2 < 1 failure becomes alpha^2 - m^2 <= 0, the same states;
+// path (A = Fast) runs the same iteration with the Fast divide/sqrt, fused
+// multiply-adds, and these changes:
+//  * per iteration one reciprocal root y = 1/sqrt(alpha^2 - m^2) gives
+//    Gamma = alpha y (clamped to >= 1: rounding takes it to 1 - ulp for gas
+//    at rest) and m^2 Gamma^3/alpha^3 = m^2 y^3; the reference's lor2 < 1
+//    failure becomes alpha^2 - m^2 <= 0, the same states;
+//  * for the ideal gas f and f' are formed without 1/D (D tau = Gamma);
 //  * it also stops once |dp| < 1e-6 p.  Newton's next correction is then
 //    K dp^2 with K = |f''/2f'|, and K p <= 23 over the surrogate's training
 //    box (rho 1e-4..1e2, p/rho 1e-6..1e2, W <= 50, gamma 4/3 and 5/3), so the
@@ -316,8 +317,9 @@ __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<doub
   } else {
 #pragma clang fp contract(fast)
     // E < sqrt(m^2 + D^2) without the root; sqrt(m^2/(1 - eps2)) = m / sqrt(1 - eps2).
+    // |m| to one Newton step (4.2e-15): it only places the floor pmin.
     if (!(E >= 0.0 && E * E >= m2 + D * D)) return 1;
-    m = A::sqrt_(m2);
+    m = FaceFast::sqrt_(m2);
     pmin = m * 1.0000000000005 - E;
   }
   double p = smax(m - E, 0.0);
@@ -403,10 +405,11 @@ __device__ __forceinline__ int c2p(double Gg, const State<double> &U, State<doub
     W.v[0] = A::div(U.v[0], den), W.v[1] = A::div(U.v[1], den), W.v[2] = A::div(U.v[2], den);
   } else {
 #pragma clang fp contract(fast)
-    // rho = D / Gamma = D sqrt(alpha^2 - m^2) / alpha, v = m / alpha.
+    // rho = D / Gamma = D sqrt(alpha^2 - m^2) / alpha, v = m / alpha, each to
+    // one Newton step (4.2e-15), far below their FP32 storage.
     alpha = E + p;
-    const double ia = A::rcp(alpha);
-    W.d = D * A::sqrt_(smax(alpha * alpha - m2, 0.0)) * ia;
+    const double ia = FaceFast::rcp(alpha);
+    W.d = D * FaceFast::sqrt_(smax(alpha * alpha - m2, 0.0)) * ia;
     W.p = p;
     W.v[0] = U.v[0] * ia, W.v[1] = U.v[1] * ia, W.v[2] = U.v[2] * ia;
   }
