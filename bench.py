@@ -108,8 +108,9 @@ def kernel_source_sha():
     return h.hexdigest()[:16]
 
 
-def load_traffic(kernel):
-    """Counted HBM bytes of `kernel` from profiles/traffic.json, if measured on the current kernel sources."""
+def load_traffic(kernel, n_cells):
+    """Counted HBM bytes of `kernel` from profiles/traffic.json, if measured on the current kernel
+    sources and on a mesh of this size."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None, "no profiles/traffic.json"
@@ -117,6 +118,8 @@ def load_traffic(kernel):
         t = json.load(f)
     if t.get("kernel_source_sha") != kernel_source_sha():
         return None, "profiles/traffic.json was measured on other kernel sources"
+    if t.get("n_cells") != n_cells:
+        return None, f"profiles/traffic.json was measured at {t.get('n_cells')} cells"
     k = t.get("kernels", {}).get(kernel)
     if not k:
         return None, f"no counters for {kernel}"
@@ -285,7 +288,7 @@ def main():
     kernel = "k_tile<3, true, false, 0>"
     dom_ms = kt["face_ms"]  # the tiled kernel (faces and cells of the step)
     achieved = step_bytes / (dom_ms * 1e-3) / 1e9
-    traffic, traffic_src = load_traffic(kernel)
+    traffic, traffic_src = load_traffic(kernel, N)
     out = {
         "metric": METRIC,
         "value": round(value, 2),
