@@ -233,6 +233,15 @@ int rhllc_tile_layout(int64_t n_cells, const void *geo_cells, int64_t n_faces, c
                       int32_t *n_tiles, int64_t *n_extra, int32_t *max_slots, int32_t *cell_new_to_old,
                       int32_t *face_new_to_old, int32_t *tile_meta, int32_t *tile_extra, uint32_t *cell_slots);
 
+/* The mixed path's 16-byte form of n face vectors S n (3 doubles each) into
+ * out (4 floats each), as rhllc_create stores them: the FP32 vector, and in
+ * the fourth word (as bits) three signed 10-bit multiples k (bits 0-9,
+ * 10-19, 20-29) of Q = 2^(E - 33), 2^E the power of two at or below the
+ * largest |component| of the FP32 vector; fp32 + k Q is the vector to 2^-33
+ * of its largest component.  No counterpart in the reference (its faces
+ * carry FP64 n and S); for callers that inspect or prepare the layout. */
+int rhllc_pack_face_vectors(int64_t n, const double *Sn, float *out);
+
 /* Sizes, and the algorithmic bytes of one step for roofline reporting:
  * the two-kernel structure's compulsory traffic at the variant's storage
  * precision (FP64 233 N + 120 F, mixed 161 N + 64 F, plus 24 N for the

@@ -112,6 +112,7 @@ SIGNATURES = {
     "rhllc_time_kernels": (C.c_int, [_vp, _i32, _vp]),
     "rhllc_info": (C.c_int, [_vp, _vp, _vp, _vp]),
     "rhllc_tile_layout": (C.c_int, [_i64, _vp, _i64, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "rhllc_pack_face_vectors": (C.c_int, [_i64, _vp, _vp]),
     "rhllc_eval_flux": (C.c_int, [_i32, _i32, _dbl, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_eval_c2p": (C.c_int, [_i32, _i32, _dbl, _i64, _vp, _vp, _vp, _vp, _vp]),
     "rhllc_eval_p2c": (C.c_int, [_i32, _dbl, _i64, _vp, _vp]),
@@ -323,6 +324,15 @@ def tile_layout(mesh, reorder=True):
                                                               "tile_extra", "cell_slots")]))
     out["tile_extra"] = out["tile_extra"][:ne.value]
     out["max_slots"] = ms.value
+    return out
+
+
+def pack_face_vectors(Sn):
+    """The mixed path's 16-byte form of face vectors S n (n x 3 float64) as rhllc_create stores
+    them: an (n, 4) float32 array whose fourth column holds the residual bits (no device)."""
+    Sn = np.ascontiguousarray(Sn, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((Sn.shape[0], 4), np.float32)
+    check(_lib().rhllc_pack_face_vectors(Sn.shape[0], Sn.ctypes.data, out.ctypes.data))
     return out
 
 

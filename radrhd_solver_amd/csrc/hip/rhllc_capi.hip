@@ -880,6 +880,13 @@ extern "C" int rhllc_tile_layout(int64_t n_cells, const void *geo_cells, int64_t
   return RHLLC_OK;
 }
 
+extern "C" int rhllc_pack_face_vectors(int64_t n, const double *Sn, float *out) {
+  if (n < 0) return fail(RHLLC_ERR_ARG, "n < 0");
+  if (n > 0 && (!Sn || !out)) return fail(RHLLC_ERR_ARG, "null argument");
+  for (int64_t i = 0; i < n; ++i) rhllc_layout::pack_sn(Sn + 3 * i, out + 4 * i);
+  return RHLLC_OK;
+}
+
 // ------------------------------------------------------------ pointwise evals
 namespace {
 struct DevBuf {

@@ -13,7 +13,6 @@ using rhllc_mesh::cell_ids;
 using rhllc_mesh::cell_V;
 using rhllc_mesh::RawVec;
 
-namespace {
 // S n into the 16-byte form described with Layout::Sn.
 void pack_sn(const double e[3], float *o) {
   uint32_t w = 0;
@@ -35,7 +34,6 @@ void pack_sn(const double e[3], float *o) {
   o[0] = h[0], o[1] = h[1], o[2] = h[2];
   std::memcpy(&o[3], &w, 4);
 }
-}  // namespace
 
 bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n_faces, const void *geo_faces,
            Layout &L, std::string &err) {

@@ -52,6 +52,9 @@ struct Layout {
   rhllc_mesh::RawVec<uint32_t> cell_slots;  // per cell two words: four 16-bit slots, bit 15 = id_r
 };
 
+// One face vector S n into the mixed path's 16-byte form (Layout::Sn).
+void pack_sn(const double e[3], float *o);
+
 // Validates the reference's geo_cell_t / geo_face_t records (every id in
 // range, boundary codes known, each cell slot agreeing with its face's
 // id_l / id_r and sign bit, V > 0) and builds the layout.  Returns false with
