@@ -128,6 +128,7 @@ struct rhllc_solver {
     Dev e = d;
     e.U4 = U4b[in], e.UE = UEb[in], e.W4 = W4b[in], e.Wp = Wpb[in], e.Wva = Wvab[in], e.Wsg = Wsgb[in];
     e.U4o = U4b[out], e.UEo = UEb[out], e.W4o = W4b[out], e.Wpo = Wpb[out], e.Wvao = Wvab[out], e.Wsgo = Wsgb[out];
+    e.tile_reverse = in;
     return e;
   }
   // The current state, updated in place.

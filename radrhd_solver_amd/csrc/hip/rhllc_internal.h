@@ -79,6 +79,7 @@ struct Dev {
   // of the face's own entry, so a tile reads its extra faces contiguously.
   const int4 *tile_meta;
   const uint2 *cell_slots;
+  int32_t tile_reverse;  // walk the tiles last to first (steps from buffer 1)
   // radiation source (optional): (Fx, Fy, Fz, Ur) and (kappa_a, kappa_s)
   const float4 *rad_FU;
   const float2 *rad_k;

@@ -529,7 +529,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
   static_assert(kTileCells == kBlock, "one cell per thread in the cell phase");
   constexpr bool MIXED = VARIANT >= kMixedPrev;
   extern __shared__ double2 lds_flux[];
-  const int t = (int)xcd_block(blockIdx.x, gridDim.x);
+  // Every other step walks the tiles backwards, so a step starts on the tiles
+  // (and the arrays' ends) the previous one touched last, which the
+  // memory-side cache still holds.
+  const int tx = (int)xcd_block(blockIdx.x, gridDim.x);
+  const int t = d.tile_reverse ? (int)gridDim.x - 1 - tx : tx;
   const int4 tm = ld(d.tile_meta, t);
   const int nf = tm.y + tm.w;
   double2 *A = lds_flux, *B = lds_flux + nf;
