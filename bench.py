@@ -12,8 +12,8 @@ A "step" is one hipGraph replay of the step over the whole mesh with the
 state resident in HBM: the tiled kernel (each workgroup computes the faces of
 256 cells into LDS and updates those cells), the surrogate-list kernel and the
 dt kernel.  `value` is the cell-updates per second of the mixed step (FP32
-storage of the cell records and face geometry; FP64 U, fluxes and arithmetic)
-with the c2p surrogate start (the stored model for gamma 5/3, trained on the
+cell records, face vectors in FP32 plus residual bits; FP64 U, 1/V, fluxes and
+arithmetic) with the c2p surrogate start (the stored model for gamma 5/3, trained on the
 device first when there is none).  Beside it, on the same mesh and state: the
 mixed step started from the previous pressure (`mixed_prev`), the plain FP64
 step under the tiled schedule (`fp64`) and under the two-kernel schedule
@@ -28,7 +28,12 @@ what this build's kernel itself must move at least; `traffic` is the HBM
 bytes rocprofv3 counted for the same kernel build (profiles/traffic.json,
 used only when its source hash matches the current kernels).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--hexes 159]
+With --eos taub the same workload runs with the Taub-Mathews EOS (the
+reference's shipped one); the mixed step then starts c2p from the previous
+pressure (the surrogate is trained for the ideal gas) and the line carries its
+own metric name.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--hexes 159] [--eos ideal|taub]
 """
 from __future__ import annotations
 
@@ -46,6 +51,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Mcell-updates/s RHLLC step @24M tets: mixed+MLP-c2p vs plain FP64, at L1<=1e-5"
+METRIC_TAUB = "Mcell-updates/s RHLLC step @24M tets, EOS TAUB: mixed (previous-pressure c2p start) vs plain FP64, at L1<=1e-5"
 HBM_PEAK_GBS = 8000.0
 SEED = 20261202
 KERNEL_SOURCES = ("rhllc_kernels.hip", "rhllc_physics.h", "flux_ext.h", "rhllc_internal.h", "mlp.h")
@@ -58,6 +64,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20,
                     help="untimed graph steps first: after an idle period the device clock takes ~15 steps to settle")
     ap.add_argument("--hexes", type=int, default=159, help="hexes per box edge (6 tets each)")
+    ap.add_argument("--eos", choices=("ideal", "taub"), default="ideal",
+                    help="equation of state (taub: the mixed step starts c2p from the previous pressure)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 legs")
     ap.add_argument("--two-kernel-mixed", action="store_true", help="also time the mixed two-kernel schedule")
@@ -127,14 +135,14 @@ def load_traffic(kernel, n_cells):
 
 
 def run_gpu(mesh, W0, rad, precision, guess, steps, warmup, kernel_steps, stats_steps, model=None,
-            schedule=None):
+            schedule=None, eos=0):
     from radrhd_solver_amd import capi
 
     sched = capi.SCHED_TILED if schedule is None else schedule
     h = mesh.h_min()
     t0 = time.time()
     s = capi.Solver(mesh, gamma=5.0 / 3.0, cfl=0.4, h_min=h, precision=precision, guess=guess, reorder=True,
-                    schedule=sched)
+                    schedule=sched, eos=eos)
     s.upload_W(W0)
     if rad is not None:
         s.set_radiation(*rad)
@@ -187,12 +195,12 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(mesh, W0, rad, steps_req):
+def cpu_baseline(mesh, W0, rad, steps_req, eos=0):
     """The oracle's FP64 AoS OpenMP step (the reference's loop structure and records), -O3
     timing build, on this host's cores: 2 warm-up steps, then the median of >= 5 steps."""
     import oracle as O
 
-    st = O.State(mesh, W0, fast=True)
+    st = O.State(mesh, W0, eos=eos, fast=True)
     h = mesh.h_min()
     Ur, Fr, ka, ks = rad
     threads = st.L.orc_num_threads()
@@ -251,16 +259,24 @@ def main():
 
     from radrhd_solver_amd import surrogate
 
-    t0 = time.time()
-    model, minfo = surrogate.load_or_train(5.0 / 3.0)
-    log(f"c2p surrogate: {minfo.get('source')} ({time.time() - t0:.1f} s)")
-    leg = dict(steps=args.steps, warmup=args.warmup, kernel_steps=args.kernel_steps, stats_steps=args.stats_steps)
-    mixed = run_gpu(mesh, W0, rad, capi.PREC_MIXED, capi.GUESS_MLP, model=model, **leg)
-    prev = run_gpu(mesh, W0, rad, capi.PREC_MIXED, capi.GUESS_PREVIOUS, **leg)
+    taub = args.eos == "taub"
+    eos = capi.EOS_TAUB if taub else capi.EOS_IDEAL
+    leg = dict(steps=args.steps, warmup=args.warmup, kernel_steps=args.kernel_steps, stats_steps=args.stats_steps,
+               eos=eos)
+    if taub:
+        model, minfo = None, {}
+        mixed = run_gpu(mesh, W0, rad, capi.PREC_MIXED, capi.GUESS_PREVIOUS, **leg)
+        prev = mixed
+    else:
+        t0 = time.time()
+        model, minfo = surrogate.load_or_train(5.0 / 3.0)
+        log(f"c2p surrogate: {minfo.get('source')} ({time.time() - t0:.1f} s)")
+        mixed = run_gpu(mesh, W0, rad, capi.PREC_MIXED, capi.GUESS_MLP, model=model, **leg)
+        prev = run_gpu(mesh, W0, rad, capi.PREC_MIXED, capi.GUESS_PREVIOUS, **leg)
     mixed2k = None
     if args.two_kernel_mixed:
-        mixed2k = run_gpu(mesh, W0, rad, capi.PREC_MIXED, capi.GUESS_MLP, model=model,
-                          schedule=capi.SCHED_TWO_KERNEL, **leg)
+        mixed2k = run_gpu(mesh, W0, rad, capi.PREC_MIXED, capi.GUESS_PREVIOUS if taub else capi.GUESS_MLP,
+                          model=model, schedule=capi.SCHED_TWO_KERNEL, **leg)
     fp64 = fp64_2k = None
     if not args.no_fp64:
         fp64 = run_gpu(mesh, W0, rad, capi.PREC_FP64, capi.GUESS_REFERENCE, **leg)
@@ -285,12 +301,12 @@ def main():
     fb, cb = model_bytes(N, F, True, True)
     step_bytes = fb + cb
     kt = mixed["kernels"]
-    kernel = "k_tile<3, true, false, 0>"
+    kernel = "k_tile<2, true, false, 1>" if taub else "k_tile<3, true, false, 0>"
     dom_ms = kt["face_ms"]  # the tiled kernel (faces and cells of the step)
     achieved = step_bytes / (dom_ms * 1e-3) / 1e9
     traffic, traffic_src = load_traffic(kernel, N)
     out = {
-        "metric": METRIC,
+        "metric": METRIC_TAUB if taub else METRIC,
         "value": round(value, 2),
         "unit": "Mcell-updates/s",
         "n_gpus": 1,
@@ -304,16 +320,18 @@ def main():
         "data": "synthetic: seeded jittered/permuted Kuhn-tet mesh, blast-wave state, seeded radiation field",
         "config": {
             "workload": f"full-form blast wave, {n}^3 hexes x 6 = {N} tets, {F} faces, radiation source, "
-            "mixed precision (FP32 cell records, face vectors FP32 plus residuals, FP64 U, fluxes and arithmetic), surrogate c2p "
-            "start, tiled step (faces into LDS, cells from LDS), hipGraph-replayed",
+            f"{'EOS Taub-Mathews' if taub else 'ideal gas gamma 5/3'}, mixed precision (FP32 cell records, face "
+            "vectors FP32 plus residuals, FP64 U, fluxes and arithmetic), tiled step (faces into LDS, cells from "
+            "LDS), hipGraph-replayed",
             "n_cells": N,
             "n_faces": F,
-            "c2p_start": "MLP surrogate where tau changed > 5% in the step, previous pressure elsewhere",
+            "c2p_start": "previous pressure (the surrogate is trained for the ideal gas)" if taub
+            else "MLP surrogate where tau changed > 5% in the step, previous pressure elsewhere",
             "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"{kernel} (tiled step, mixed, surrogate start, radiation)",
+            "kernel": f"{kernel} (tiled step, mixed, {'previous-pressure' if taub else 'surrogate'} start, radiation)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -368,7 +386,7 @@ def main():
                              "after_steps": args.steps + args.warmup}
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(mesh, W0, rad, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(mesh, W0, rad, args.cpu_steps, eos)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
