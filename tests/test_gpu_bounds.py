@@ -100,8 +100,9 @@ def test_walls_and_sources_mixed_l1():
     assert max(l1) <= 1e-5
 
 
+@pytest.mark.parametrize("eos", [capi.EOS_IDEAL, capi.EOS_TAUB], ids=["ideal", "taub"])
 @pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_MIXED])
-def test_walled_box_conserves_mass_and_energy(precision):
+def test_walled_box_conserves_mass_and_energy(precision, eos):
     """Every boundary face a wall: the blast moves no mass or energy out of the box."""
     m = M.kuhn_box(12, 12, 12, seed=SEED)
     for axis in range(3):
@@ -110,7 +111,7 @@ def test_walled_box_conserves_mass_and_energy(precision):
     W0 = M.init_state(m, M.INIT_BLAST, params=[0.5, 0.5, 0.5, 0.25, 1.0, 100.0, 1.0])
     V = m.cells["V"]
     guess = capi.GUESS_REFERENCE if precision == capi.PREC_FP64 else capi.GUESS_PREVIOUS
-    s = capi.Solver(m, cfl=0.4, precision=precision, guess=guess, lock_wall=True)
+    s = capi.Solver(m, cfl=0.4, precision=precision, guess=guess, lock_wall=True, eos=eos)
     try:
         s.upload_W(W0)
         s.sync()
@@ -124,5 +125,5 @@ def test_walled_box_conserves_mass_and_energy(precision):
     tot0, tot1 = (U0 * V[:, None]).sum(0), (U * V[:, None]).sum(0)
     scale = (np.abs(U) * V[:, None]).sum(0)
     drift = np.abs(tot1 - tot0) / scale
-    print("precision", precision, "relative drift of the totals", drift, flush=True)
+    print("precision", precision, "eos", eos, "relative drift of the totals", drift, flush=True)
     assert drift[0] <= 1e-13 and drift[4] <= 1e-13

@@ -98,8 +98,9 @@ def test_mixed_c2p_residual_and_iterations(case):
         assert st["max_residual"] <= 1e-10
 
 
+@pytest.mark.parametrize("eos", [capi.EOS_IDEAL, capi.EOS_TAUB], ids=["ideal", "taub"])
 @pytest.mark.parametrize("W", [(1.0, 0.0, 0.0, 0.0, 1.0), (1.0, 0.5, 0.3, -0.2, 2.0)], ids=["rest", "moving"])
-def test_uniform_state_stays_uniform(W):
+def test_uniform_state_stays_uniform(W, eos):
     """A uniform state on a jittered periodic box: every face flux is the same physical flux, so a
     cell changes only by that flux through the closure defect of its four face vectors.  The mixed
     step's vectors (FP32 plus 10-bit residuals) close to 2^-33 of their size: after 50 steps every
@@ -107,7 +108,7 @@ def test_uniform_state_stays_uniform(W):
     m = M.kuhn_box(16, 16, 16, periodic=7, seed=SEED)
     W0 = M.init_state(m, M.INIT_UNIFORM, params=list(W))
     for prec, guess in ((capi.PREC_FP64, capi.GUESS_REFERENCE), (capi.PREC_MIXED, capi.GUESS_PREVIOUS)):
-        s = capi.Solver(m, cfl=0.4, precision=prec, guess=guess)
+        s = capi.Solver(m, cfl=0.4, precision=prec, guess=guess, eos=eos)
         try:
             s.upload_W(W0)
             s.run(50)
@@ -116,5 +117,5 @@ def test_uniform_state_stays_uniform(W):
         finally:
             s.close()
         dev = np.abs(Wg - W0).max(axis=0)
-        print("precision", prec, "max deviation of (rho, v, p) after 50 steps", dev, flush=True)
+        print("precision", prec, "eos", eos, "max deviation of (rho, v, p) after 50 steps", dev, flush=True)
         assert dev.max() <= 1e-8
