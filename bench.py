@@ -100,8 +100,8 @@ def design_bytes(N, F, mixed, rad, tiled):
     """Compulsory bytes of this build's kernels (per step) at their actual layout."""
     r = 24 * N if rad else 0
     if tiled:
-        if mixed:  # records read 32 + U r/w 80 + 1/V 8 + slots 8 + records written 32; ids 8 + S n 16
-            return 160 * N + 24 * F + r
+        if mixed:  # records read 32 + U r/w 80 + 1/V 8 + slots 8 + records written 32; far cell 4 + S n 16
+            return 160 * N + 20 * F + r
         return 176 * N + 40 * F + r  # U + W read 80, U + W written 80, V 8, slots 8; ids 8 + (n, S) 32
     if mixed:  # face: record 32, ids 8 + S n 16 + packed flux 24; cell: ids 16 + 1/V 8 + U r/w 80
         return (32 * N + 48 * F) + (140 * N + 24 * F) + r  # + p32 4 + record 32; packed flux read

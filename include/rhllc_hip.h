@@ -44,7 +44,7 @@ extern "C" {
 #define RHLLC_PREC_FP64 0  /* all FP64, bitwise equal to the reference compiled strict */
 #define RHLLC_PREC_MIXED 1 /* FP32 storage of the primitives v and p (per-cell records, which
                               also hold s - D and D in FP64); face vectors S n in 16 bytes
-                              (FP32 plus 10-bit residuals, exact to 2^-33); the two-kernel
+                              (FP32 plus 7-bit residuals, exact to 2^-30); the two-kernel
                               step also stores its face flux in FP32 with small extensions;
                               FP64 U, 1/V and arithmetic; conserves to rounding */
 
@@ -235,11 +235,13 @@ int rhllc_tile_layout(int64_t n_cells, const void *geo_cells, int64_t n_faces, c
 
 /* The mixed path's 16-byte form of n face vectors S n (3 doubles each) into
  * out (4 floats each), as rhllc_create stores them: the FP32 vector, and in
- * the fourth word (as bits) three signed 10-bit multiples k (bits 0-9,
- * 10-19, 20-29) of Q = 2^(E - 33), 2^E the power of two at or below the
- * largest |component| of the FP32 vector; fp32 + k Q is the vector to 2^-33
- * of its largest component.  No counterpart in the reference (its faces
- * carry FP64 n and S); for callers that inspect or prepare the layout. */
+ * the fourth word (as bits) three signed 7-bit multiples k (bits 0-6,
+ * 7-13, 14-20) of Q = 2^(E - 30), 2^E the power of two at or below the
+ * largest |component| of the FP32 vector; fp32 + k Q is the vector to 2^-30
+ * of its largest component.  Bits 21-31 are 0 here; the tiled step's layout
+ * keeps each face's tile-local cell there.  No counterpart in the reference
+ * (its faces carry FP64 n and S); for callers that inspect or prepare the
+ * layout. */
 int rhllc_pack_face_vectors(int64_t n, const double *Sn, float *out);
 
 /* Sizes, and the algorithmic bytes of one step for roofline reporting:

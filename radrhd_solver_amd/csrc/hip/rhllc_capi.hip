@@ -277,11 +277,18 @@ extern "C" int rhllc_create(const rhllc_params *params, int64_t n_cells, const v
     if (e == hipSuccess && count) e = hipMemcpy(*dst, src, sizeof(**dst) * count, hipMemcpyHostToDevice);
     return e;
   };
-  // The face arrays, followed on the tiled step by the extra faces' copies.
+  // The face arrays, followed on the tiled step by the extra faces' copies
+  // (the tiled mixed step reads face_far and the tile-local cells in face_Sn
+  // in place of face_lr, which keeps the F faces for the flux download).
   const size_t nfe = (size_t)F + L.tile_extra.size();
   int2 *lr_d;
-  HIP_TRY(upload(&lr_d, L.lr.data(), nfe));
+  HIP_TRY(upload(&lr_d, L.lr.data(), mixed && tiled ? (size_t)F : nfe));
   d.face_lr = lr_d;
+  if (mixed && tiled) {
+    int32_t *far_d;
+    HIP_TRY(upload(&far_d, L.far.data(), nfe));
+    d.face_far = far_d;
+  }
   if (!mixed) {
     double *V_d;
     double4 *nS_d;

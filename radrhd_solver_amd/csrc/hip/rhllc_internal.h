@@ -41,6 +41,7 @@ struct Dev {
   const int2 *face_lr;      // (left, right): right n_cells / n_cells + 1 = ghost states, ~left = wall
   const double4 *face_nS;   // (nx, ny, nz, S), FP64 path
   const float4 *face_Sn;    // S * (nx, ny, nz), mixed path (16-byte form, unpack_sn)
+  const int32_t *face_far;  // tiled mixed path: the cell outside the entry's tile (tile_face_in)
   double4 *F4;              // flux (d, mx, my, mz) * S, FP64 path
   double *FE;               // flux E * S
   float4 *F4f;              // mixed path: flux * S, FP32 parts (flux_ext.h)

@@ -111,17 +111,17 @@ __device__ __forceinline__ void fold_speed(const Dev &d, double speed) {
 }
 
 // The mixed path's face vector S n from its 16-byte form: the FP32 vector
-// plus k Q per component, k the signed 10-bit fields of the fourth word and
-// Q = 2^(E - 33), 2^E the power of two at or below the largest |component|.
-// k Q is exact and so is the sum (at most 34 significant bits), so this is the
-// vector the host encoded, to 2^-33 of its largest component.
+// plus k Q per component, k the signed 7-bit fields (bits 0-20) of the fourth
+// word and Q = 2^(E - 30), 2^E the power of two at or below the largest
+// |component|.  k Q is exact and so is the sum (at most 31 significant bits),
+// so this is the vector the host encoded, to 2^-30 of its largest component.
 __device__ __forceinline__ void unpack_sn(const float4 &g, double s[3]) {
   const float m = fmaxf(fmaxf(fabsf(g.x), fabsf(g.y)), fabsf(g.z));
-  const double Q = (double)__int_as_float(__float_as_int(m) & 0x7F800000) * 0x1p-33;
+  const double Q = (double)__int_as_float(__float_as_int(m) & 0x7F800000) * 0x1p-30;
   const int w = __float_as_int(g.w);
-  s[0] = fma((double)((w << 22) >> 22), Q, (double)g.x);
-  s[1] = fma((double)((w << 12) >> 22), Q, (double)g.y);
-  s[2] = fma((double)((w << 2) >> 22), Q, (double)g.z);
+  s[0] = fma((double)((w << 25) >> 25), Q, (double)g.x);
+  s[1] = fma((double)((w << 18) >> 25), Q, (double)g.y);
+  s[2] = fma((double)((w << 11) >> 25), Q, (double)g.z);
 }
 
 // What a face reads before its cells: the two cell indices and the geometry
@@ -140,6 +140,24 @@ __device__ __forceinline__ FaceIn<MIXED> face_in(const Dev &d, int i) {
   else
     o.g = ld(d.face_nS, i);
   return o;
+}
+// Entry i of the tiled step's face arrays, a face of tile t.  On the mixed
+// path the cell in the tile is bits 21-28 of the vector's fourth word (bit 29
+// set when it is the right cell) and face_far holds the other side.
+template <bool MIXED>
+__device__ __forceinline__ FaceIn<MIXED> tile_face_in(const Dev &d, int t, int i) {
+  if constexpr (!MIXED) {
+    return face_in<MIXED>(d, i);
+  } else {
+    FaceIn<MIXED> o;
+    o.g = ld(d.face_Sn, i);
+    const int far = ld(d.face_far, i);
+    const unsigned int w = (unsigned int)__float_as_int(o.g.w);
+    const int nc = t * kTileCells + (int)((w >> 21) & 0xFFu);
+    const bool right = (w >> 29) & 1u;
+    o.lr = right ? make_int2(far, nc) : make_int2(nc, far);
+    return o;
+  }
 }
 
 // One face: gather of the two cells' state at the start of the step, HLLC
@@ -543,7 +561,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
     // One index into one set of arrays (the extra faces' copies follow the
     // faces): a select between two sets of base pointers would hold them in
     // VGPRs through the loop.
-    const FaceIn<MIXED> fi = face_in<MIXED>(d, k < tm.y ? tm.x + k : d.n_faces + tm.z + (k - tm.y));
+    const FaceIn<MIXED> fi = tile_face_in<MIXED>(d, t, k < tm.y ? tm.x + k : d.n_faces + tm.z + (k - tm.y));
     State<double> F;
     speed = smax(speed, face_flux<MIXED, EOS>(d, fi, F));
     A[k] = make_double2(F.d, F.v[0]);

@@ -24,12 +24,12 @@ void pack_sn(const double e[3], float *o) {
   mb &= 0x7F800000u;  // 2^E (0 for a zero or subnormal vector: no correction)
   float pw;
   std::memcpy(&pw, &mb, 4);
-  const double Q = (double)pw * 0x1p-33;
+  const double Q = (double)pw * 0x1p-30;
   if (Q > 0.0)
     for (int k = 0; k < 3; ++k) {
-      // |e - h| <= ulp(h) / 2 <= 2^(E - 24) = 512 Q; 512 itself is clamped
-      const long n = std::min(511L, std::max(-511L, std::lround((e[k] - (double)h[k]) / Q)));
-      w |= ((uint32_t)n & 0x3FFu) << (10 * k);
+      // |e - h| <= ulp(h) / 2 <= 2^(E - 24) = 64 Q; 64 itself is clamped
+      const long n = std::min(63L, std::max(-63L, std::lround((e[k] - (double)h[k]) / Q)));
+      w |= ((uint32_t)n & 0x7Fu) << (7 * k);
     }
   o[0] = h[0], o[1] = h[1], o[2] = h[2];
   std::memcpy(&o[3], &w, 4);
@@ -148,6 +148,10 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
   // cell (own faces: contiguous, since faces are grouped by that cell) and is
   // also an extra face of the tile of its later cell when that is another one.
   const int32_t T = opt.tile_cells;
+  if (opt.mixed && T > 256) {
+    err = "tile layout: the mixed path's tiles hold at most 256 cells";
+    return false;
+  }
   const int32_t nt = (N + T - 1) / T;
   L.n_tiles = nt;
   // The own faces of tile t are [first[t], first[t + 1]); own_tile is
@@ -231,6 +235,7 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
   }
   // The extra faces' copies, entry k at F + k.
   L.lr.resize(2 * ((size_t)F + ne));
+  if (opt.mixed) L.far.resize((size_t)F + ne);
   if (opt.mixed)
     L.Sn.resize(4 * ((size_t)F + ne));
   else
@@ -243,6 +248,28 @@ bool build(const Options &opt, int64_t n_cells, const void *geo_cells, int64_t n
       std::memcpy(&L.Sn[4 * j], &L.Sn[4 * i], 16);  // the fourth word is bits, copied as such
     else
       for (int a = 0; a < 4; ++a) L.nS[4 * j + a] = L.nS[4 * i + a];
+  }
+  if (opt.mixed) {
+    // Each entry's cell in its tile (bits 21-28 of the fourth word, bit 29 set
+    // when it is the right cell) and the other one, or the boundary encoding
+    // of the right side, in far: the tile's face work reads 4 bytes of cell
+    // indices instead of 8.
+#pragma omp parallel for schedule(static)
+    for (int32_t t = 0; t < nt; ++t) {
+      const int32_t *m = &L.tile_meta[4 * (size_t)t];
+      auto put = [&](size_t j) {
+        const int32_t x = L.lr[2 * j], y = L.lr[2 * j + 1];
+        const bool right = x / T != t;  // else the left cell is the one in the tile
+        const uint32_t near = (uint32_t)((right ? y : x) - t * T);
+        L.far[j] = right ? x : y;
+        uint32_t w;
+        std::memcpy(&w, &L.Sn[4 * j + 3], 4);
+        w |= (near << 21) | (right ? 1u << 29 : 0u);
+        std::memcpy(&L.Sn[4 * j + 3], &w, 4);
+      };
+      for (int32_t k = 0; k < m[1]; ++k) put((size_t)m[0] + k);
+      for (int32_t k = 0; k < m[3]; ++k) put((size_t)F + m[2] + k);
+    }
   }
   return true;
 }

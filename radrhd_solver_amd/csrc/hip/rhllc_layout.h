@@ -32,12 +32,16 @@ struct Layout {
   rhllc_mesh::RawVec<int32_t> lr;
   rhllc_mesh::RawVec<double> nS;       // per face (nx, ny, nz, S), FP64 path
   // per face S n in 16 bytes, mixed path: the FP32 vector and, in the fourth
-  // word, three signed 10-bit multiples (bits 0-9, 10-19, 20-29) of
-  // 2^(E - 33), 2^E the power of two at or below the largest |component| of
-  // the FP32 vector, that restore what the FP32 rounding took: fp32 + k Q is
-  // the vector to 2^-33 of its largest component, so a cell's four vectors
-  // close to that (FP32 alone: 2^-24)
+  // word, three signed 7-bit multiples (bits 0-6, 7-13, 14-20) of
+  // Q = 2^(E - 30), 2^E the power of two at or below the largest |component|
+  // of the FP32 vector, that restore what the FP32 rounding took: fp32 + k Q
+  // is the vector to 2^-30 of its largest component, so a cell's four vectors
+  // close to that (FP32 alone: 2^-24).  On the tiled step bits 21-28 hold the
+  // entry's cell in its tile (cell - 256 t) and bit 29 whether that is the
+  // right cell; far holds the other cell (or the right side's boundary
+  // encoding), so (left, right) = bit 29 ? (far, near) : (near, far).
   rhllc_mesh::RawVec<float> Sn;
+  rhllc_mesh::RawVec<int32_t> far;     // tiled mixed path, F + extra entries
   rhllc_mesh::RawVec<double> V;        // per cell, FP64 path
   rhllc_mesh::RawVec<double> Vinv;     // per cell 1/V, mixed path
   rhllc_mesh::RawVec<int32_t> cf;      // per cell 4 face indices | bit 31 when the cell is id_r (two-kernel step)

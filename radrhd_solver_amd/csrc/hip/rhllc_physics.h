@@ -631,7 +631,7 @@ __device__ __forceinline__ void mixed_side(double gamma, const MixedIn &w, const
   o.cmin = (a - sroot) * r;
 }
 
-// The face geometry is the vector S n (sx, sy, sz), exact to 2^-33 of its size,
+// The face geometry is the vector S n (sx, sy, sz), exact to 2^-30 of its size,
 // so a cell's four vectors close to that; the area S and v.n = v.(S n)/S are
 // recovered from it (the frame-free form relies on |n| = 1: m = m_n n + m_t),
 // and the flux's normal part is scaled by S n directly.  The contact solve is

@@ -103,8 +103,8 @@ def test_mixed_c2p_residual_and_iterations(case):
 def test_uniform_state_stays_uniform(W, eos):
     """A uniform state on a jittered periodic box: every face flux is the same physical flux, so a
     cell changes only by that flux through the closure defect of its four face vectors.  The mixed
-    step's vectors (FP32 plus 10-bit residuals) close to 2^-33 of their size: after 50 steps every
-    primitive stays within 1e-8 of the initial one (FP32 vectors alone, 2^-24: ~1e-6)."""
+    step's vectors (FP32 plus 7-bit residuals) close to 2^-30 of their size: after 50 steps every
+    primitive stays within 1e-8 of the initial one (measured 3e-9; FP32 vectors alone, 2^-24: ~1e-6)."""
     m = M.kuhn_box(16, 16, 16, periodic=7, seed=SEED)
     W0 = M.init_state(m, M.INIT_UNIFORM, params=list(W))
     for prec, guess in ((capi.PREC_FP64, capi.GUESS_REFERENCE), (capi.PREC_MIXED, capi.GUESS_PREVIOUS)):
