@@ -295,27 +295,29 @@ def read_records(path: str, dtype) -> np.ndarray:
 
 
 def write_hllc_cfg(path: str, T: float, CFL: float, h_min: float, save_timer: float, tau: float = 0.0,
-                   eos: str | None = None, geometry: str | None = None) -> None:
+                   eos: str | None = None, geometry: str | None = None, extra: dict | None = None) -> None:
     """hllc_cfg.json with the keys of lib/json/include/json_struct.h:15-20, and
-    the drop-in's optional "eos" ("taub" when absent, the reference's shipped EOS)
-    and "geometry" (the reference's GEOMETRY_TYPE, "cone" when absent)."""
+    the drop-in's optional "eos" ("taub" when absent, the reference's shipped EOS),
+    "geometry" (the reference's GEOMETRY_TYPE, "cone" when absent) and any other
+    of its keys in `extra` ("precision", "c2p_start", "mlp_model", "gamma")."""
     cfg = {"T": T, "CFL": CFL, "h_min": h_min, "save_timer": save_timer, "tau": tau}
     if eos is not None:
         cfg["eos"] = eos
     if geometry is not None:
         cfg["geometry"] = geometry
+    cfg.update(extra or {})
     with open(path, "w") as f:
         json.dump(cfg, f, indent=4)
 
 
 def write_case(dirpath: str, mesh: Mesh, W: np.ndarray, T: float, CFL: float, save_timer: float,
-               eos: str | None = None, geometry: str | None = None) -> dict:
+               eos: str | None = None, geometry: str | None = None, extra: dict | None = None) -> dict:
     """Write geo_faces.bin, geo_cells.bin, the flux_t initial state and the three JSON configs."""
     os.makedirs(dirpath, exist_ok=True)
     base = os.path.abspath(dirpath) + os.sep
     mesh.write(base + "geo_faces.bin", base + "geo_cells.bin")
     write_records(base + "hllc_init.bin", np.ascontiguousarray(W, dtype=np.float64))
-    write_hllc_cfg(base + "hllc_cfg.json", T, CFL, mesh.h_min(), save_timer, eos=eos, geometry=geometry)
+    write_hllc_cfg(base + "hllc_cfg.json", T, CFL, mesh.h_min(), save_timer, eos=eos, geometry=geometry, extra=extra)
     with open(base + "solve_cfg.json", "w") as f:
         json.dump({"class_vtk": 0, "max_number_of_iter": 1, "accuracy": 1e-5, "start_point": 0}, f, indent=4)
     os.makedirs(base + "Solve", exist_ok=True)

@@ -74,6 +74,38 @@ def test_rhllc_solver_geometry_cube_walls(tmp_path):
     assert np.abs(cone.W[:, 0] - rho).max() > 1e-3  # the wall made a difference
 
 
+@pytest.mark.parametrize("start", ["previous", "mlp"])
+def test_rhllc_solver_mixed_precision_keys(tmp_path, start):
+    """"precision": "mixed" with "c2p_start" previous / mlp (and the stored gamma 5/3 model as
+    "mlp_model"): the executable's solution within 1e-5 relative L1 of the oracle's FP64 one."""
+    from radrhd_solver_amd import surrogate
+
+    m = M.kuhn_box(24, 3, 3, length=(1.0, 0.125, 0.125), seed=SEED)
+    W0 = M.init_state(m, M.INIT_SHOCK_TUBE)
+    h = m.h_min()
+    T = 60 * 0.4 * h
+    extra = {"precision": "mixed", "c2p_start": start}
+    if start == "mlp":
+        path = surrogate.model_path(5.0 / 3.0)
+        assert os.path.exists(path), path
+        extra["mlp_model"] = path
+    dirs = M.write_case(str(tmp_path), m, W0, T=T, CFL=0.4, save_timer=1e9, eos="ideal", extra=extra)
+    st, _ = _oracle_loop(m, W0, 0.4, h, T)
+    r = subprocess.run([SOLVER_EXE, str(tmp_path / "directories_cfg.json")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    base = dirs["solve_address"] + "1"
+    rho, p, v = _read(base + "density.bin", 1), _read(base + "pressure.bin", 1), _read(base + "velocity.bin", 3)
+    Vc = m.cells["V"]
+    for got, ref in ((rho, st.W[:, 0]), (p, st.W[:, 4]), (v, st.W[:, 1:4])):
+        got, ref = got.reshape(len(Vc), -1), ref.reshape(len(Vc), -1)
+        for k in range(got.shape[1]):
+            den = (np.abs(ref[:, k]) * Vc).sum()
+            if den > 0:
+                l1 = (np.abs(got[:, k] - ref[:, k]) * Vc).sum() / den
+                assert l1 <= 1e-5, (start, k, l1)
+    assert not np.array_equal(rho, st.W[:, 0])  # the mixed path ran, not the FP64 one
+
+
 def test_rhllc_solver_mpi_one_rank_writes_flux_bin(tmp_path):
     """rhllc_solver_mpi (src/RHD/rhllc_solver_mpi.cpp, RunRhllcMpiModule) at one rank:
     the partition file puts every cell on rank 0, and each WriteSolutionMPI writes
