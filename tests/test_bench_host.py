@@ -45,6 +45,7 @@ def test_traffic_guard(monkeypatch):
     t, why = bench.load_traffic(kernel, N)
     assert t is not None, f"the committed rocprofv3 traffic no longer matches the kernel sources: {why}"
     assert 4e9 < t < 8e9
+    assert bench.load_traffic("k_tile<2, true, false, 1>", N)[0] is not None  # the EOS TAUB line's kernel
     t, why = bench.load_traffic(kernel, N // 2)
     assert t is None and "cells" in why
     t, why = bench.load_traffic("k_unknown", N)
